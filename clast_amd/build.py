@@ -1,0 +1,102 @@
+"""In-tree build of the native parts of clast_amd.
+
+* ``libclast_hip.so``  — the gfx950 HIP kernels behind the C ABI of
+  ``include/clast_hip.h`` (hipcc, cross-compiled; no GPU needed to build).
+* ``libclast_gen.so``  — the seeded synthetic-data generator (g++).
+* ``libclast_host.so`` + ``clast`` — the C++ host driver (CLI, FASTA chunking,
+  windows, result holder, TSV writer) as a library and an executable, both
+  linked against ``libclast_hip.so``.
+
+Everything lands in ``clast_amd/lib`` and ``clast_amd/bin`` so the built files
+travel with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+GEN = os.path.join(PKG, "gen")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# Packed-FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INCLUDE}",
+]
+
+HIP_UNITS = ["clast_hip.hip", "dp_global.hip", "dp_local.hip"]
+HOST_UNITS = ["settings.cpp", "fasta.cpp", "seqlist.cpp", "holder.cpp", "scheduler.cpp", "host_capi.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}")
+
+
+def build_hip() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    cmds = []
+    for unit in HIP_UNITS:
+        src = os.path.join(CSRC, unit)
+        obj = os.path.join(LIB, unit.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(_run, cmds))
+    out = os.path.join(LIB, "libclast_hip.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_gen() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libclast_gen.so")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", os.path.join(GEN, "clast_gen.cpp"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_host() -> str:
+    """libclast_host.so (driver code as a library) and the clast executable."""
+    os.makedirs(BIN, exist_ok=True)
+    srcs = [os.path.join(HOST, u) for u in HOST_UNITS]
+    lib = os.path.join(LIB, "libclast_host.so")
+    common = ["g++", "-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}"]
+    _run([*common, "-shared", *srcs, f"-L{LIB}", "-lclast_hip", "-Wl,-rpath,$ORIGIN", "-o", lib + ".tmp"])
+    os.replace(lib + ".tmp", lib)
+    exe = os.path.join(BIN, "clast")
+    _run([*common, os.path.join(HOST, "main.cpp"), f"-L{LIB}", "-lclast_host", "-lclast_hip",
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
+def build_all() -> None:
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        f_hip = ex.submit(build_hip)
+        f_gen = ex.submit(build_gen)
+        f_hip.result()
+        f_gen.result()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, BIN)

@@ -1,0 +1,802 @@
+// libclast_hip: C ABI over the gfx950 search pipeline (see include/clast_hip.h).
+#include "../../include/clast_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.hpp"
+#include "prims.hpp"
+#include "runtime.hpp"
+
+namespace clast {
+
+static std::mutex g_rt_mu;
+static std::map<int, Runtime*> g_rt;
+
+Runtime& runtime() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_rt_mu);
+  auto it = g_rt.find(dev);
+  if (it != g_rt.end()) return *it->second;
+  Runtime* rt = new Runtime(dev);
+  g_rt[dev] = rt;
+  return *rt;
+}
+
+static int bits_for(uint64_t count) {  // bits to hold values 0 .. count-1
+  int b = 0;
+  while (b < 64 && (1ull << b) < count) ++b;
+  return b;
+}
+
+static void check_params(const clast_params* p) {
+  if (!p) throw ClastError(CLAST_ERR_ARG, "null params");
+  if (p->lmer < 1 || p->lmer > 31) throw ClastError(CLAST_ERR_ARG, "lmer must be in 1..31");
+  if (p->stride < 1) throw ClastError(CLAST_ERR_ARG, "stride must be >= 1");
+  if (p->gap < 2 || p->gap > 16) throw ClastError(CLAST_ERR_ARG, "gap must be in 2..16");
+}
+
+static void check_window(const clast_window& w) {
+  if (w.n_seq < 0 || w.n_slots < 0) throw ClastError(CLAST_ERR_ARG, "negative window size");
+  if (w.n_slots > 0 && !w.bases) throw ClastError(CLAST_ERR_ARG, "null window bases");
+  if (w.n_seq > 0 && (!w.lengths || !w.gateway)) throw ClastError(CLAST_ERR_ARG, "null window arrays");
+  // gateway is rebased (first sequence at slot 0) and ascending, as
+  // CHostSeqList::getGateway returns it; every sequence lies inside the window
+  if (w.n_seq > 0 && w.gateway[0] != 0) throw ClastError(CLAST_ERR_ARG, "window gateway must start at 0");
+  for (int k = 0; k < w.n_seq; ++k) {
+    if (w.lengths[k] < 0 || w.gateway[k] < 0 || w.gateway[k] > w.n_slots ||
+        static_cast<int64_t>(w.gateway[k]) + w.lengths[k] > w.n_slots)
+      throw ClastError(CLAST_ERR_ARG, "window gateway/length out of range");
+    if (k > 0 && w.gateway[k] < static_cast<int64_t>(w.gateway[k - 1]) + w.lengths[k - 1])
+      throw ClastError(CLAST_ERR_ARG, "window gateway not ascending");
+  }
+  if (w.n_seq == 0 && w.n_slots != 0) throw ClastError(CLAST_ERR_ARG, "window slots without sequences");
+}
+
+// Score threshold of one query length: the smallest integer score s with
+// !(cutOff < K * D * qLen * exp(-lambda * s)) (…deleteBadHits.cuh:32-39
+// with krnlCalculateEvalue.cu:21), found by bisection on that exact
+// expression; INT_MAX when no score passes, INT_MIN when every score does.
+static int32_t score_threshold(const clast_params& p, int qlen) {
+  auto keep = [&](int64_t s) {
+    const double e = p.K * p.total_db_size * qlen * std::exp(-p.lambda * static_cast<double>(s));
+    return !(p.cut_off < e);
+  };
+  int64_t lo = -(1ll << 30), hi = (1ll << 30);
+  if (!keep(hi)) return INT32_MAX;
+  if (keep(lo)) return INT32_MIN;
+  while (hi - lo > 1) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (keep(mid)) hi = mid;
+    else lo = mid;
+  }
+  return static_cast<int32_t>(hi);
+}
+
+}  // namespace clast
+
+using namespace clast;
+
+struct clast_index_s {
+  int device = 0;
+  int lmer = 0, stride = 0, cut_repeat = 0;
+  int nwin = 0, nseq = 0, max_tlen = 0;
+  bool key64 = false;
+  int pbits = 0, shift = 0;
+  uint32_t nentries = 0;
+  DevArray<uint8_t> tbase;
+  DevArray<uint64_t> win_slot;
+  DevArray<uint64_t> seq_slot;
+  DevArray<int32_t> seq_len, seq_win, seq_ref;
+  DevArray<uint32_t> ekey32;
+  DevArray<uint64_t> ekey64;
+  DevArray<int32_t> egs, etidx;
+  DevArray<uint32_t> bucket;
+  std::vector<int32_t> h_seq_len, h_seq_win, h_seq_ref, h_id_begin;
+  std::vector<uint64_t> h_win_slot;
+};
+
+struct clast_query_s {
+  int device = 0;
+  int lmer = 0;
+  int nwin = 0;
+  uint32_t nseq = 0, npos = 0;
+  int max_qlen = 0;
+  DevArray<uint8_t> qbase;
+  DevArray<uint32_t> qslot, pos_off;
+  DevArray<int32_t> qlen, qref, qwin, smin;
+  std::vector<int32_t> h_qlen, h_qref, h_qwin, h_id_begin;
+  bool smin_valid = false;
+  double smin_key[4] = {0, 0, 0, 0};
+};
+
+struct clast_hits_s {
+  std::vector<int32_t> col[10];  // tID tIdx qID qIdx tHL qHL match score twin qwin
+  std::vector<double> ev;
+  std::vector<int32_t> stage[5][8];
+  int64_t stage_n[5] = {0, 0, 0, 0, 0};
+  float times[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return CLAST_OK;
+  } catch (const ClastError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return CLAST_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CLAST_ERR_ARG;
+  }
+}
+
+void set_device(int dev) { hip_check(hipSetDevice(dev), "hipSetDevice"); }
+
+// ---------------------------------------------------------------- index
+
+void build_index(const clast_params& p, const clast_window* wins, int nwin, clast_index_s& ix) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mutex());
+  hipStream_t st = rt.stream();
+  ix.device = rt.device();
+  ix.lmer = p.lmer;
+  ix.stride = p.stride;
+  ix.cut_repeat = p.cut_repeat;
+  ix.nwin = nwin;
+  ix.key64 = p.lmer > 16;
+
+  // host layout: window slot offsets, per target sequence tables
+  std::vector<uint64_t> win_slot(nwin + 1, 0);
+  std::vector<uint32_t> win_stride(nwin + 1, 0), win_seq(nwin + 1, 0);
+  std::vector<uint64_t> seq_slot;
+  uint64_t ns = 0;
+  for (int w = 0; w < nwin; ++w) {
+    check_window(wins[w]);
+    win_slot[w + 1] = win_slot[w] + static_cast<uint64_t>(wins[w].n_slots);
+    ns += static_cast<uint64_t>(wins[w].n_slots) / static_cast<uint64_t>(p.stride);
+    if (ns >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "more than 2^32 strided target k-mers");
+    win_stride[w + 1] = static_cast<uint32_t>(ns);
+    for (int k = 0; k < wins[w].n_seq; ++k) {
+      seq_slot.push_back(win_slot[w] + static_cast<uint64_t>(wins[w].gateway[k]));
+      ix.h_seq_len.push_back(wins[w].lengths[k]);
+      ix.h_seq_win.push_back(w);
+      ix.h_seq_ref.push_back(wins[w].id_begin + k);
+      ix.max_tlen = std::max(ix.max_tlen, wins[w].lengths[k]);
+    }
+    win_seq[w + 1] = static_cast<uint32_t>(seq_slot.size());
+    ix.h_id_begin.push_back(wins[w].id_begin);
+  }
+  ix.nseq = static_cast<int>(seq_slot.size());
+  ix.h_win_slot = win_slot;
+  // a sequence's slot is never past the next sequence's; windows are in order
+  ix.tbase.alloc(win_slot[nwin]);
+  for (int w = 0; w < nwin; ++w)
+    if (wins[w].n_slots)
+      hip_check(hipMemcpyAsync(ix.tbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
+                "upload target bases");
+  ix.win_slot.upload(win_slot.data(), win_slot.size(), st);
+  ix.seq_slot.upload(seq_slot.data(), seq_slot.size(), st);
+  ix.seq_len.upload(ix.h_seq_len.data(), ix.h_seq_len.size(), st);
+  ix.seq_win.upload(ix.h_seq_win.data(), ix.h_seq_win.size(), st);
+  ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
+  DevArray<uint32_t> d_win_stride, d_win_seq;
+  d_win_stride.upload(win_stride.data(), win_stride.size(), st);
+  d_win_seq.upload(win_seq.data(), win_seq.size(), st);
+
+  const uint32_t n = static_cast<uint32_t>(ns);
+  const int keybits = 2 * p.lmer;
+  const uint64_t invalid = 1ull << keybits;
+  uint32_t nvalid = 0, nent = 0;
+  if (n > 0) {
+    Workspace& ws = rt.ws();
+    uint64_t* keys = ws.get<uint64_t>("ix.keys", n);
+    uint32_t* vals = ws.get<uint32_t>("ix.vals", n);
+    k_target_keys<<<grid_for(n, kBlock), kBlock, 0, st>>>(ix.tbase.ptr, ix.win_slot.ptr, d_win_stride.ptr, nwin,
+                                                          p.lmer, p.stride, n, invalid, keys, vals);
+    rt.check_launch("k_target_keys");
+    radix_sort_pairs(rt, keys, vals, n, keybits + 1);
+    uint32_t* head = ws.get<uint32_t>("ix.head", n);
+    uint32_t* scal = ws.get<uint32_t>("ix.scalars", 4);
+    k_fill<uint32_t><<<1, 64, 0, st>>>(scal, 4, 0u);
+    k_index_heads<<<grid_for(n, kBlock), kBlock, 0, st>>>(keys, vals, d_win_stride.ptr, nwin, n, invalid, head, scal);
+    rt.check_launch("k_index_heads");
+    rt.read_scalars(scal, &nvalid, 1);
+    if (nvalid > 0) {
+      uint32_t* run_ex = ws.get<uint32_t>("ix.runex", nvalid);
+      uint32_t* run_start = ws.get<uint32_t>("ix.runstart", nvalid + 1);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, head, nvalid, run_ex, scal + 1, false);
+      k_index_runstart<<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(head, run_ex, nvalid, run_start);
+      uint32_t* keep = ws.get<uint32_t>("ix.keep", nvalid);
+      int32_t* tgs = ws.get<int32_t>("ix.tgs", nvalid);
+      int32_t* tti = ws.get<int32_t>("ix.tti", nvalid);
+      k_index_keep<<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(
+          vals, head, run_ex, run_start, nvalid, ix.win_slot.ptr, d_win_stride.ptr, d_win_seq.ptr, nwin,
+          ix.seq_slot.ptr, ix.seq_len.ptr, p.lmer, p.stride, p.cut_repeat, keep, tgs, tti);
+      rt.check_launch("k_index_keep");
+      uint32_t* eoff = ws.get<uint32_t>("ix.eoff", nvalid);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, keep, nvalid, eoff, scal + 2, false);
+      rt.read_scalars(scal + 2, &nent, 1);
+      ix.egs.alloc(nent);
+      ix.etidx.alloc(nent);
+      if (ix.key64) {
+        ix.ekey64.alloc(nent);
+        k_index_scatter<uint64_t><<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(keys, keep, eoff, tgs, tti, nvalid,
+                                                                               ix.ekey64.ptr, ix.egs.ptr, ix.etidx.ptr);
+      } else {
+        ix.ekey32.alloc(nent);
+        k_index_scatter<uint32_t><<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(keys, keep, eoff, tgs, tti, nvalid,
+                                                                               ix.ekey32.ptr, ix.egs.ptr, ix.etidx.ptr);
+      }
+      rt.check_launch("k_index_scatter");
+    }
+  }
+  ix.nentries = nent;
+  // bucket table over the top P key bits: about two buckets per entry
+  int pb = bits_for(static_cast<uint64_t>(nent) + 1) + 1;
+  pb = std::max(pb, 8);
+  pb = std::min(pb, std::min(keybits, 30));
+  ix.pbits = pb;
+  ix.shift = keybits - pb;
+  const uint64_t nb = (1ull << pb) + 1;
+  ix.bucket.alloc(nb);
+  k_fill<uint32_t><<<grid_for(nb, kBlock), kBlock, 0, st>>>(ix.bucket.ptr, nb, nent);
+  if (nent > 0) {
+    if (ix.key64)
+      k_bucket_heads<uint64_t><<<grid_for(nent, kBlock), kBlock, 0, st>>>(ix.ekey64.ptr, nent, ix.shift, ix.bucket.ptr);
+    else
+      k_bucket_heads<uint32_t><<<grid_for(nent, kBlock), kBlock, 0, st>>>(ix.ekey32.ptr, nent, ix.shift, ix.bucket.ptr);
+  }
+  rt.check_launch("k_bucket_heads");
+  rt.sync();
+}
+
+// ---------------------------------------------------------------- query
+
+void build_query(const clast_params& p, const clast_window* wins, int nwin, clast_query_s& q) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mutex());
+  hipStream_t st = rt.stream();
+  q.device = rt.device();
+  q.lmer = p.lmer;
+  q.nwin = nwin;
+  std::vector<uint64_t> win_slot(nwin + 1, 0);
+  std::vector<uint32_t> qslot, pos_off(1, 0);
+  uint64_t npos = 0;
+  for (int w = 0; w < nwin; ++w) {
+    check_window(wins[w]);
+    if (wins[w].n_seq % 2) throw ClastError(CLAST_ERR_ARG, "a query window holds '+'/'-' pairs (even n_seq)");
+    win_slot[w + 1] = win_slot[w] + static_cast<uint64_t>(wins[w].n_slots);
+    if (win_slot[w + 1] >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
+    for (int k = 0; k < wins[w].n_seq; ++k) {
+      const int len = wins[w].lengths[k];
+      qslot.push_back(static_cast<uint32_t>(win_slot[w] + wins[w].gateway[k]));
+      q.h_qlen.push_back(len);
+      q.h_qref.push_back(wins[w].id_begin + k);
+      q.h_qwin.push_back(w);
+      q.max_qlen = std::max(q.max_qlen, len);
+      npos += len >= p.lmer ? static_cast<uint64_t>(len - p.lmer + 1) : 0ull;
+      if (npos >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 k-mer positions");
+      pos_off.push_back(static_cast<uint32_t>(npos));
+    }
+    q.h_id_begin.push_back(wins[w].id_begin);
+  }
+  q.nseq = static_cast<uint32_t>(qslot.size());
+  q.npos = static_cast<uint32_t>(npos);
+  q.qbase.alloc(win_slot[nwin]);
+  for (int w = 0; w < nwin; ++w)
+    if (wins[w].n_slots)
+      hip_check(hipMemcpyAsync(q.qbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
+                "upload query bases");
+  q.qslot.upload(qslot.data(), qslot.size(), st);
+  q.pos_off.upload(pos_off.data(), pos_off.size(), st);
+  q.qlen.upload(q.h_qlen.data(), q.h_qlen.size(), st);
+  q.qref.upload(q.h_qref.data(), q.h_qref.size(), st);
+  q.qwin.upload(q.h_qwin.data(), q.h_qwin.size(), st);
+  q.smin.alloc(std::max<uint64_t>(q.nseq, 1));
+  rt.sync();
+}
+
+void ensure_smin(Runtime& rt, const clast_params& p, clast_query_s& q) {
+  const double key[4] = {p.cut_off, p.total_db_size, p.K, p.lambda};
+  if (q.smin_valid && std::memcmp(key, q.smin_key, sizeof key) == 0) return;
+  std::unordered_map<int, int32_t> memo;
+  std::vector<int32_t> sm(q.nseq);
+  for (uint32_t s = 0; s < q.nseq; ++s) {
+    const int len = q.h_qlen[s];
+    auto it = memo.find(len);
+    if (it == memo.end()) it = memo.emplace(len, score_threshold(p, len)).first;
+    sm[s] = it->second;
+  }
+  if (q.nseq)
+    hip_check(hipMemcpyAsync(q.smin.ptr, sm.data(), sizeof(int32_t) * q.nseq, hipMemcpyHostToDevice, rt.stream()),
+              "upload smin");
+  rt.sync();
+  std::memcpy(q.smin_key, key, sizeof key);
+  q.smin_valid = true;
+}
+
+template <class T>
+void d2h(std::vector<T>& dst, const T* src, uint64_t n, hipStream_t st) {
+  dst.resize(n);
+  if (n) hip_check(hipMemcpyAsync(dst.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost, st), "D2H");
+}
+
+struct StageEvents {
+  hipEvent_t e[7];
+  StageEvents() {
+    for (auto& x : e) hip_check(hipEventCreate(&x), "hipEventCreate");
+  }
+  ~StageEvents() {
+    for (auto& x : e) (void)hipEventDestroy(x);
+  }
+};
+
+// The CDeviceHitList chain (CDeviceHitList.cu:228-423) for every unit of
+// idx x q.  final_order: apply the fixResult order across units.
+void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int flags, bool final_order,
+                clast_hits_s& out) {
+  if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
+    throw ClastError(CLAST_ERR_ARG, "params differ from those the index/query were built with");
+  if (ix.device != q.device) throw ClastError(CLAST_ERR_ARG, "index and query on different devices");
+  set_device(ix.device);
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mutex());
+  hipStream_t st = rt.stream();
+  Workspace& ws = rt.ws();
+  const bool keep_stages = (flags & CLAST_KEEP_STAGES) != 0;
+  const bool use_cut = p.cut_off != -1;
+  if (use_cut) ensure_smin(rt, p, q);
+  StageEvents ev;
+  hip_check(hipEventRecord(ev.e[0], st), "event");
+
+  const uint32_t np = q.npos;
+  const int sbits = bits_for(q.nseq), gbits = bits_for(static_cast<uint64_t>(ix.nseq));
+  const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 1);
+  if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
+  SeedPack pk{gbits, dbits, q.max_qlen};
+  SeedDecode dc{gbits, dbits, q.max_qlen};
+
+  uint32_t nseed = 0;
+  uint64_t* skey = nullptr;
+  uint32_t* sval = nullptr;
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 2);
+  uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
+  if (np > 0 && ix.nentries > 0) {
+    uint32_t* m0 = ws.get<uint32_t>("s.m0", np);
+    uint32_t* rawc = ws.get<uint32_t>("s.raw", np);
+    uint32_t* cnt = ws.get<uint32_t>("s.cnt", np);
+    uint32_t* off = ws.get<uint32_t>("s.off", np);
+    auto run_seed = [&](auto* ekey) {
+      using KeyT = std::remove_pointer_t<decltype(ekey)>;
+      SeedArgs<KeyT> a{q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, q.pos_off.ptr, q.nseq, np,
+                       ekey,        ix.egs.ptr,  ix.etidx.ptr, ix.bucket.ptr, ix.shift, ix.seq_len.ptr,
+                       p.lmer,      p.gap,       p.local};
+      k_seed_count<KeyT><<<grid_for(np, kBlock), kBlock, 0, st>>>(a, m0, rawc, cnt);
+      rt.check_launch("k_seed_count");
+      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt, np, off, tot64, false);
+      uint64_t total = 0;
+      rt.read_scalars(tot64, &total, 1);
+      if (total >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
+      nseed = static_cast<uint32_t>(total);
+      if (nseed) {
+        skey = ws.get<uint64_t>("s.key", nseed);
+        sval = ws.get<uint32_t>("s.val", nseed);
+        k_seed_write<KeyT><<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, m0, rawc, off, skey, sval);
+        rt.check_launch("k_seed_write");
+      }
+    };
+    if (ix.key64) run_seed(ix.ekey64.ptr);
+    else run_seed(ix.ekey32.ptr);
+  }
+  auto dump_seeds = [&](int slot) {
+    std::vector<uint64_t> hk;
+    std::vector<uint32_t> hv;
+    d2h(hk, skey, nseed, st);
+    d2h(hv, sval, nseed, st);
+    rt.sync();
+    auto& S = out.stage[slot];
+    for (int c = 0; c < 4; ++c) S[c].resize(nseed);
+    for (uint32_t i = 0; i < nseed; ++i) {
+      const uint64_t k = hk[i];
+      const uint32_t s = static_cast<uint32_t>(k >> (gbits + dbits));
+      const uint32_t gs = static_cast<uint32_t>((k >> dbits) & ((1ull << gbits) - 1ull));
+      const int d = static_cast<int>(k & ((1ull << dbits) - 1ull)) - q.max_qlen;
+      S[0][i] = ix.h_seq_ref[gs];
+      S[1][i] = d + static_cast<int>(hv[i]);
+      S[2][i] = q.h_qref[s];
+      S[3][i] = static_cast<int>(hv[i]);
+    }
+    out.stage_n[slot] = nseed;
+  };
+  if (keep_stages) dump_seeds(CLAST_STAGE_SEEDS);
+  hip_check(hipEventRecord(ev.e[1], st), "event");
+  if (nseed > 1) radix_sort_pairs(rt, skey, sval, nseed, sbits + gbits + dbits);
+  if (keep_stages) dump_seeds(CLAST_STAGE_SORTED);
+  hip_check(hipEventRecord(ev.e[2], st), "event");
+
+  // isolate + duplicate filters -> hits
+  uint32_t nhit = 0;
+  int32_t *hs = nullptr, *hgs = nullptr, *ht = nullptr, *hq = nullptr;
+  if (nseed > 0) {
+    const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
+    uint32_t* unit_first = ws.get<uint32_t>("f.unit", nunits);
+    k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
+    uint8_t* nearf = ws.get<uint8_t>("f.near", nseed);
+    uint32_t* gidx = ws.get<uint32_t>("f.gidx", nseed);
+    uint32_t* kidx = ws.get<uint32_t>("f.kidx", nseed);
+    uint32_t* lastk = ws.get<uint32_t>("f.lastk", nseed);
+    uint32_t* gstart = ws.get<uint32_t>("f.gstart", nseed);
+    uint32_t* dflag = ws.get<uint32_t>("f.dflag", nseed);
+    uint32_t* hoff = ws.get<uint32_t>("f.hoff", nseed);
+    const uint32_t g = grid_for(nseed, kBlock);
+    k_seed_marks<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, p.width, p.gap, ix.seq_win.ptr, q.qwin.ptr, q.nwin,
+                                       nearf, gidx, unit_first);
+    k_seed_keep<<<g, kBlock, 0, st>>>(skey, nseed, dc, ix.seq_win.ptr, q.qwin.ptr, q.nwin, nearf, unit_first, kidx);
+    rt.check_launch("k_seed_marks/keep");
+    device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, kidx, nseed, lastk, static_cast<uint32_t*>(nullptr), true);
+    device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, gidx, nseed, gstart, static_cast<uint32_t*>(nullptr), true);
+    k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
+    rt.check_launch("k_seed_dup");
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nseed, hoff, tot32, false);
+    rt.read_scalars(tot32, &nhit, 1);
+    if (nhit) {
+      hs = ws.get<int32_t>("h.s", nhit);
+      hgs = ws.get<int32_t>("h.gs", nhit);
+      ht = ws.get<int32_t>("h.t", nhit);
+      hq = ws.get<int32_t>("h.q", nhit);
+      k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, dflag, hoff, hs, hgs, ht, hq);
+      rt.check_launch("k_hits_scatter");
+    }
+  }
+  if (keep_stages) {
+    std::vector<int32_t> a, b, c, d;
+    d2h(a, hs, nhit, st); d2h(b, hgs, nhit, st); d2h(c, ht, nhit, st); d2h(d, hq, nhit, st);
+    rt.sync();
+    auto& S = out.stage[CLAST_STAGE_HITS];
+    for (int k = 0; k < 4; ++k) S[k].resize(nhit);
+    for (uint32_t i = 0; i < nhit; ++i) {
+      S[0][i] = ix.h_seq_ref[b[i]]; S[1][i] = c[i]; S[2][i] = q.h_qref[a[i]]; S[3][i] = d[i];
+    }
+    out.stage_n[CLAST_STAGE_HITS] = nhit;
+  }
+  hip_check(hipEventRecord(ev.e[3], st), "event");
+
+  // banded DP
+  int32_t *ot = nullptr, *oq = nullptr, *thl = nullptr, *qhl = nullptr, *om = nullptr, *osc = nullptr;
+  if (nhit) {
+    ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
+    thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
+    om = ws.get<int32_t>("d.m", nhit); osc = ws.get<int32_t>("d.s", nhit);
+    DPArgs a{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
+             ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, p.lmer, ot, oq, thl, qhl, om, osc};
+    if (p.local) launch_dp_local(p.gap, a, st);
+    else launch_dp_global(p.gap, a, st);
+    rt.check_launch("k_dp");
+  }
+  if (keep_stages) {
+    std::vector<int32_t> c[6], hsv, hgv;
+    d2h(c[0], ot, nhit, st); d2h(c[1], oq, nhit, st); d2h(c[2], thl, nhit, st);
+    d2h(c[3], qhl, nhit, st); d2h(c[4], om, nhit, st); d2h(c[5], osc, nhit, st);
+    d2h(hsv, hs, nhit, st); d2h(hgv, hgs, nhit, st);
+    rt.sync();
+    auto& S = out.stage[CLAST_STAGE_ALIGNED];
+    for (int k = 0; k < 8; ++k) S[k].resize(nhit);
+    for (uint32_t i = 0; i < nhit; ++i) {
+      S[0][i] = ix.h_seq_ref[hgv[i]]; S[1][i] = c[0][i]; S[2][i] = q.h_qref[hsv[i]]; S[3][i] = c[1][i];
+      S[4][i] = c[2][i]; S[5][i] = c[3][i]; S[6][i] = c[4][i]; S[7][i] = c[5][i];
+    }
+    out.stage_n[CLAST_STAGE_ALIGNED] = nhit;
+  }
+  hip_check(hipEventRecord(ev.e[4], st), "event");
+
+  // E-value, filters, result order, dedup
+  uint32_t nkeep = 0, nfinal = 0;
+  uint32_t* order = nullptr;
+  uint32_t* permB = nullptr;
+  double* hev = nullptr;
+  if (nhit) {
+    hev = ws.get<double>("r.ev", nhit);
+    uint32_t* kflag = ws.get<uint32_t>("r.kflag", nhit);
+    uint32_t* koff = ws.get<uint32_t>("r.koff", nhit);
+    int32_t* srange = ws.get<int32_t>("r.srange", 4);
+    k_fill<int32_t><<<1, 64, 0, st>>>(srange, 1, INT32_MAX);
+    k_fill<int32_t><<<1, 64, 0, st>>>(srange + 1, 1, INT32_MIN);
+    const uint32_t g = grid_for(nhit, kBlock);
+    k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, osc, qhl, q.qlen.ptr, q.smin.ptr, use_cut ? 1 : 0,
+                                       p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev, kflag, srange);
+    rt.check_launch("k_hit_filter");
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, kflag, nhit, koff, tot32 + 1, false);
+    hip_check(hipMemcpyAsync(tot32 + 2, srange, 8, hipMemcpyDeviceToDevice, st), "copy");
+    uint32_t sc3[3];
+    rt.read_scalars(tot32 + 1, sc3, 3);
+    nkeep = sc3[0];
+    const int32_t smin_v = static_cast<int32_t>(sc3[1]), smax_v = static_cast<int32_t>(sc3[2]);
+    if (nkeep) {
+      const int scb = bits_for(static_cast<uint64_t>(static_cast<int64_t>(smax_v) - smin_v + 1));
+      const int pbits = bits_for((q.nseq + 1) / 2);
+      const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
+      const int qb = bits_for(static_cast<uint64_t>(q.max_qlen) + 1);
+      if (wbits + pbits + scb > 64) throw ClastError(CLAST_ERR_LIMIT, "order key over 64 bits");
+      uint64_t* w0 = ws.get<uint64_t>("r.w0", nkeep);
+      uint64_t* w1 = ws.get<uint64_t>("r.w1", nkeep);
+      uint32_t* rec = ws.get<uint32_t>("r.rec", nkeep);
+      uint32_t* perm = ws.get<uint32_t>("r.perm", nkeep);
+      k_order_keys<<<g, kBlock, 0, st>>>(nhit, kflag, koff, hs, hgs, hq, osc, ix.seq_win.ptr, q.max_qlen, pbits, scb,
+                                         smax_v, w0, w1, rec);
+      rt.check_launch("k_order_keys");
+      const uint64_t* words[2] = {w0, w1};
+      const int wb[2] = {qb + 1, wbits + pbits + scb};
+      sort_perm(rt, nkeep, 2, words, wb, perm);
+      uint32_t* dflag = ws.get<uint32_t>("r.dflag", nkeep);
+      uint32_t* doff = ws.get<uint32_t>("r.doff", nkeep);
+      const uint32_t gk = grid_for(nkeep, kBlock);
+      k_dedup<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, hs, hgs, ot, oq, ix.seq_win.ptr, q.qwin.ptr, dflag);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nkeep, doff, tot32 + 5, false);
+      rt.read_scalars(tot32 + 5, &nfinal, 1);
+      order = ws.get<uint32_t>("r.order", nfinal);
+      k_compact_perm<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, dflag, doff, order);
+      rt.check_launch("k_dedup/compact");
+      if (final_order && (ix.nwin > 1 || q.nwin > 1) && nfinal > 1) {
+        uint64_t* wf = ws.get<uint64_t>("r.wf", nfinal);
+        permB = ws.get<uint32_t>("r.permB", nfinal);
+        k_final_keys<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(nfinal, order, hs, hgs, osc, ix.seq_win.ptr, scb,
+                                                                  wbits, smax_v, wf);
+        const uint64_t* fw[1] = {wf};
+        const int fb[1] = {pbits + scb + wbits};
+        sort_perm(rt, nfinal, 1, fw, fb, permB);
+      }
+    }
+  }
+  // gather + D2H
+  for (auto& c : out.col) c.clear();
+  out.ev.clear();
+  if (nfinal) {
+    int32_t* o32 = ws.get<int32_t>("o.i32", static_cast<uint64_t>(nfinal) * 10);
+    double* oev = ws.get<double>("o.ev", nfinal);
+    OutArgs oa{nfinal, order, permB, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
+               ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
+    k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
+    rt.check_launch("k_gather_out");
+    for (int c = 0; c < 10; ++c) d2h(out.col[c], o32 + static_cast<uint64_t>(c) * nfinal, nfinal, st);
+    d2h(out.ev, oev, nfinal, st);
+  }
+  hip_check(hipEventRecord(ev.e[5], st), "event");
+  rt.sync();
+  float t;
+  for (int k = 0; k < 5; ++k) {
+    hip_check(hipEventElapsedTime(&t, ev.e[k], ev.e[k + 1]), "elapsed");
+    out.times[k] = t;
+  }
+  hip_check(hipEventElapsedTime(&t, ev.e[0], ev.e[5]), "elapsed");
+  out.times[5] = t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+int clast_init(int32_t device) {
+  return guard([&] {
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw ClastError(CLAST_ERR_ARG, "no such device");
+    set_device(device);
+    (void)runtime();
+  });
+}
+
+const char* clast_last_error(void) { return g_err.c_str(); }
+
+const char* clast_version(void) { return "libclast_hip 0.1 (gfx950)"; }
+
+int clast_index_create(const clast_params* p, const clast_window* t, clast_index* out) {
+  return clast_index_create_multi(p, t, 1, out);
+}
+
+int clast_index_create_multi(const clast_params* p, const clast_window* t, int32_t n, clast_index* out) {
+  return guard([&] {
+    check_params(p);
+    if (!t || n < 1 || !out) throw ClastError(CLAST_ERR_ARG, "bad target windows");
+    auto ix = std::make_unique<clast_index_s>();
+    build_index(*p, t, n, *ix);
+    *out = ix.release();
+  });
+}
+
+int64_t clast_index_entry_count(clast_index idx) { return idx ? static_cast<int64_t>(idx->nentries) : -1; }
+
+int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* tID, int32_t* tIdx) {
+  return guard([&] {
+    if (!ix) throw ClastError(CLAST_ERR_ARG, "null index");
+    set_device(ix->device);
+    Runtime& rt = runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    const uint64_t n = ix->nentries;
+    std::vector<int32_t> gs, ti;
+    d2h(gs, ix->egs.ptr, n, rt.stream());
+    d2h(ti, ix->etidx.ptr, n, rt.stream());
+    std::vector<int64_t> k(n);
+    if (ix->key64) {
+      std::vector<uint64_t> kk;
+      d2h(kk, ix->ekey64.ptr, n, rt.stream());
+      rt.sync();
+      for (uint64_t i = 0; i < n; ++i) k[i] = static_cast<int64_t>(kk[i]);
+    } else {
+      std::vector<uint32_t> kk;
+      d2h(kk, ix->ekey32.ptr, n, rt.stream());
+      rt.sync();
+      for (uint64_t i = 0; i < n; ++i) k[i] = static_cast<int64_t>(kk[i]);
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+      if (key) key[i] = k[i];
+      if (window) window[i] = ix->h_seq_win[gs[i]];
+      if (tID) tID[i] = ix->h_seq_ref[gs[i]];
+      if (tIdx) tIdx[i] = ti[i];
+    }
+  });
+}
+
+int clast_index_destroy(clast_index idx) {
+  return guard([&] {
+    if (!idx) return;
+    set_device(idx->device);
+    delete idx;
+  });
+}
+
+int clast_query_create(const clast_params* p, const clast_window* q, clast_query* out) {
+  return clast_query_create_multi(p, q, 1, out);
+}
+
+int clast_query_create_multi(const clast_params* p, const clast_window* q, int32_t n, clast_query* out) {
+  return guard([&] {
+    check_params(p);
+    if (!q || n < 1 || !out) throw ClastError(CLAST_ERR_ARG, "bad query windows");
+    auto qq = std::make_unique<clast_query_s>();
+    build_query(*p, q, n, *qq);
+    *out = qq.release();
+  });
+}
+
+int clast_query_destroy(clast_query q) {
+  return guard([&] {
+    if (!q) return;
+    set_device(q->device);
+    delete q;
+  });
+}
+
+int clast_search(const clast_params* p, clast_index idx, clast_query q, int32_t t_begin, int32_t q_begin,
+                 clast_hits* out) {
+  return guard([&] {
+    check_params(p);
+    if (!idx || !q || !out) throw ClastError(CLAST_ERR_ARG, "null handle");
+    if (idx->nwin != 1 || q->nwin != 1) throw ClastError(CLAST_ERR_ARG, "clast_search takes one window each");
+    if (idx->h_id_begin[0] != t_begin || q->h_id_begin[0] != q_begin)
+      throw ClastError(CLAST_ERR_ARG, "t_begin/q_begin differ from the windows' id_begin");
+    auto h = std::make_unique<clast_hits_s>();
+    run_search(*p, *idx, *q, 0, false, *h);
+    *out = h.release();
+  });
+}
+
+int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, int32_t flags, clast_hits* out) {
+  return guard([&] {
+    check_params(p);
+    if (!idx || !q || !out) throw ClastError(CLAST_ERR_ARG, "null handle");
+    auto h = std::make_unique<clast_hits_s>();
+    run_search(*p, *idx, *q, flags, true, *h);
+    *out = h.release();
+  });
+}
+
+int64_t clast_hits_count(clast_hits h) { return h ? static_cast<int64_t>(h->ev.size()) : -1; }
+
+int clast_hits_copy(clast_hits h, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                    int32_t* qHL, int32_t* match, int32_t* score, double* evalue) {
+  return guard([&] {
+    if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
+    int32_t* dst[8] = {tID, tIdx, qID, qIdx, tHL, qHL, match, score};
+    for (int c = 0; c < 8; ++c)
+      if (dst[c] && !h->col[c].empty()) std::memcpy(dst[c], h->col[c].data(), h->col[c].size() * sizeof(int32_t));
+    if (evalue && !h->ev.empty()) std::memcpy(evalue, h->ev.data(), h->ev.size() * sizeof(double));
+  });
+}
+
+int clast_hits_copy_windows(clast_hits h, int32_t* twin, int32_t* qwin) {
+  return guard([&] {
+    if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
+    if (twin && !h->col[8].empty()) std::memcpy(twin, h->col[8].data(), h->col[8].size() * sizeof(int32_t));
+    if (qwin && !h->col[9].empty()) std::memcpy(qwin, h->col[9].data(), h->col[9].size() * sizeof(int32_t));
+  });
+}
+
+int64_t clast_hits_stage_count(clast_hits h, int32_t stage) {
+  if (!h || stage < 1 || stage > 4) return -1;
+  return h->stage_n[stage];
+}
+
+int clast_hits_stage_copy(clast_hits h, int32_t stage, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx,
+                          int32_t* tHL, int32_t* qHL, int32_t* match, int32_t* score) {
+  return guard([&] {
+    if (!h || stage < 1 || stage > 4) throw ClastError(CLAST_ERR_ARG, "bad stage");
+    int32_t* dst[8] = {tID, tIdx, qID, qIdx, tHL, qHL, match, score};
+    for (int c = 0; c < 8; ++c)
+      if (dst[c] && !h->stage[stage][c].empty())
+        std::memcpy(dst[c], h->stage[stage][c].data(), h->stage[stage][c].size() * sizeof(int32_t));
+  });
+}
+
+int clast_hits_stage_times(clast_hits h, float* out6) {
+  return guard([&] {
+    if (!h || !out6) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out6, h->times, sizeof h->times);
+  });
+}
+
+int clast_hits_destroy(clast_hits h) {
+  delete h;
+  return CLAST_OK;
+}
+
+int clast_result_sort(int64_t n, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                      int32_t* qHL, int32_t* match, int32_t* score, double* evalue) {
+  return guard([&] {
+    if (n < 0 || n >= (1ll << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "bad row count");
+    if (n <= 1) return;
+    int32_t* cols[8] = {tID, tIdx, qID, qIdx, tHL, qHL, match, score};
+    for (auto* c : cols)
+      if (!c) throw ClastError(CLAST_ERR_ARG, "null column");
+    if (!evalue) throw ClastError(CLAST_ERR_ARG, "null column");
+    Runtime& rt = runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    hipStream_t st = rt.stream();
+    const uint32_t m = static_cast<uint32_t>(n);
+    DevArray<int32_t> dq, ds;
+    dq.upload(qID, m, st);
+    ds.upload(score, m, st);
+    uint64_t* w = rt.ws().get<uint64_t>("rs.w", m);
+    uint32_t* perm = rt.ws().get<uint32_t>("rs.perm", m);
+    k_result_keys<<<grid_for(m, kBlock), kBlock, 0, st>>>(m, dq.ptr, ds.ptr, w);
+    const uint64_t* words[1] = {w};
+    const int b[1] = {64};
+    sort_perm(rt, m, 1, words, b, perm);
+    std::vector<uint32_t> hp;
+    d2h(hp, perm, m, st);
+    rt.sync();
+    std::vector<int32_t> tmp(m);
+    for (auto* c : cols) {
+      for (uint32_t i = 0; i < m; ++i) tmp[i] = c[hp[i]];
+      std::memcpy(c, tmp.data(), sizeof(int32_t) * m);
+    }
+    std::vector<double> te(m);
+    for (uint32_t i = 0; i < m; ++i) te[i] = evalue[hp[i]];
+    std::memcpy(evalue, te.data(), sizeof(double) * m);
+  });
+}
+
+}  // extern "C"

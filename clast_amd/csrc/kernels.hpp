@@ -1,0 +1,730 @@
+// Search-path kernels of libclast_hip for gfx950.
+//
+// Data layout in HBM (all resident per handle):
+//   target index  : tbase (window slot arrays, concatenated, 1 B/slot),
+//                   per target sequence gs: slot start, length, window, list ID;
+//                   entries sorted by (k-mer, window, position): ekey
+//                   (u32 for lMer <= 16 else u64), egs, etidx (4 B each), and
+//                   bucket[2^P + 1] = first entry of each P-bit key prefix.
+//   query set     : qbase (window slot arrays), per strand s: slot start,
+//                   length, list ID, window, first valid k-mer position.
+//   seeds         : u64 sort key (s | gs | diagonal + offset) + u32 qIdx.
+//   hits          : SoA int32 (s, gs, tIdx, qIdx) -> DP -> 6 int32 outputs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "prims.hpp"
+
+namespace clast {
+
+// ---------------------------------------------------------------- bases
+
+// 2-bit code of A/C/G/T (A0 C1 G2 T3, SRead2bit.cuh:12-18); 4 for any other byte.
+__device__ __forceinline__ uint32_t base_code(uint32_t c) {
+  const bool ok = c == 'A' || c == 'C' || c == 'G' || c == 'T';
+  const uint32_t v = ((c >> 1) & 3u) ^ ((c >> 2) & 1u);
+  return ok ? v : 4u;
+}
+
+// Substitution class bits for the DP (krnlMatrix.cuh:11-67 as a rule):
+// A/C/G/T one-hot in bits 0-3, any other letter 0x10 (never matches, N vs N
+// included), E F I J L O P Q Z 0x20 (-1), X 0x40 (bad area).
+__device__ __forceinline__ uint32_t dp_class(uint32_t c) {
+  switch (c) {
+    case 'A': return 1u;
+    case 'C': return 2u;
+    case 'G': return 4u;
+    case 'T': return 8u;
+    case 'X': return 0x40u;
+    case 'E': case 'F': case 'I': case 'J': case 'L': case 'O': case 'P': case 'Q': case 'Z': return 0x20u;
+    default: return 0x10u;
+  }
+}
+constexpr uint32_t kClassX = 0x40u;
+
+template <class T>
+__device__ __forceinline__ uint32_t upper_bound_u(const T* a, uint32_t lo, uint32_t hi, T v) {
+  // first index in [lo, hi) with a[i] > v
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+template <class T>
+__device__ __forceinline__ uint32_t lower_bound_u(const T* a, uint32_t lo, uint32_t hi, T v) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- index build
+
+// Strided target k-mers (CDeviceHashTable.cu:20-85): element g of window w
+// is position p = (g - win_stride_off[w]) * stride of that window; invalid
+// k-mers (non-ACGT or running past the window end) get `invalid`, which
+// sorts after every valid code.
+static __global__ __launch_bounds__(kBlock) void k_target_keys(const uint8_t* __restrict__ tbase,
+                                                        const uint64_t* __restrict__ win_slot_off,
+                                                        const uint32_t* __restrict__ win_stride_off, int nwin,
+                                                        int lmer, int stride, uint32_t ns, uint64_t invalid,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= ns) return;
+  const uint32_t w = upper_bound_u(win_stride_off, 0u, static_cast<uint32_t>(nwin + 1), g) - 1;
+  const uint64_t p = static_cast<uint64_t>(g - win_stride_off[w]) * stride;
+  const uint64_t nw = win_slot_off[w + 1] - win_slot_off[w];
+  const uint8_t* b = tbase + win_slot_off[w] + p;
+  uint64_t key = 0;
+  bool ok = true;
+  for (int j = 0; j < lmer; ++j) {
+    const uint32_t c = (p + j < nw) ? base_code(b[j]) : 4u;
+    ok = ok && c < 4u;
+    key = (key << 2) | (c & 3u);
+  }
+  keys[g] = ok ? key : invalid;
+  vals[g] = g;
+}
+
+__device__ __forceinline__ uint32_t window_of(const uint32_t* win_stride_off, int nwin, uint32_t g) {
+  return upper_bound_u(win_stride_off, 0u, static_cast<uint32_t>(nwin + 1), g) - 1;
+}
+
+// Heads of (key, window) runs over the sorted strided k-mers, and the count
+// of valid ones (*nvalid; invalid codes sort last).
+static __global__ __launch_bounds__(kBlock) void k_index_heads(const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals,
+                                                        const uint32_t* __restrict__ win_stride_off, int nwin,
+                                                        uint32_t ns, uint64_t invalid, uint32_t* __restrict__ head,
+                                                        uint32_t* __restrict__ nvalid) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= ns) return;
+  const uint64_t k = keys[j];
+  const bool valid = k < invalid;
+  uint32_t h = 0;
+  if (valid) {
+    if (j == 0) {
+      h = 1;
+    } else {
+      const uint64_t pk = keys[j - 1];
+      h = (pk != k || window_of(win_stride_off, nwin, vals[j - 1]) != window_of(win_stride_off, nwin, vals[j])) ? 1u : 0u;
+    }
+    if (j + 1 == ns || keys[j + 1] >= invalid) *nvalid = j + 1;
+  }
+  head[j] = h;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_index_runstart(const uint32_t* __restrict__ head,
+                                                           const uint32_t* __restrict__ run_ex, uint32_t nvalid,
+                                                           uint32_t* __restrict__ run_start) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= nvalid) return;
+  if (head[j]) run_start[run_ex[j]] = j;
+  if (j + 1 == nvalid) run_start[run_ex[j] + head[j]] = nvalid;
+}
+
+// Repeat cut per (key, window) cell (CDeviceHashTable.cu:122-137: counts
+// taken over every strided position of the cell) and the target-boundary
+// seed filter (…deleteSeedsOnSequenceBoundary.cu:74-127: tIdx + lMer > tLen
+// never seeds) applied once here instead of per query.
+static __global__ __launch_bounds__(kBlock) void k_index_keep(const uint32_t* __restrict__ vals,
+                                                       const uint32_t* __restrict__ head,
+                                                       const uint32_t* __restrict__ run_ex,
+                                                       const uint32_t* __restrict__ run_start, uint32_t nvalid,
+                                                       const uint64_t* __restrict__ win_slot_off,
+                                                       const uint32_t* __restrict__ win_stride_off,
+                                                       const uint32_t* __restrict__ win_seq_off, int nwin,
+                                                       const uint64_t* __restrict__ seq_slot,
+                                                       const int32_t* __restrict__ seq_len, int lmer, int stride,
+                                                       int cut_repeat, uint32_t* __restrict__ keep,
+                                                       int32_t* __restrict__ tmp_gs, int32_t* __restrict__ tmp_tidx) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= nvalid) return;
+  const uint32_t r = run_ex[j] - (head[j] ? 0u : 1u);
+  const uint32_t len = run_start[r + 1] - run_start[r];
+  bool k = cut_repeat == -1 || static_cast<int64_t>(len) < cut_repeat;
+  const uint32_t g = vals[j];
+  const uint32_t w = window_of(win_stride_off, nwin, g);
+  const uint64_t slot = win_slot_off[w] + static_cast<uint64_t>(g - win_stride_off[w]) * stride;
+  const uint32_t gs = upper_bound_u(seq_slot, win_seq_off[w], win_seq_off[w + 1], slot) - 1;
+  const int64_t tidx = static_cast<int64_t>(slot - seq_slot[gs]);
+  k = k && tidx + lmer <= seq_len[gs];
+  keep[j] = k ? 1u : 0u;
+  tmp_gs[j] = static_cast<int32_t>(gs);
+  tmp_tidx[j] = static_cast<int32_t>(tidx);
+}
+
+template <class KeyT>
+static __global__ __launch_bounds__(kBlock) void k_index_scatter(const uint64_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ keep,
+                                                          const uint32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ tmp_gs,
+                                                          const int32_t* __restrict__ tmp_tidx, uint32_t nvalid,
+                                                          KeyT* __restrict__ ekey, int32_t* __restrict__ egs,
+                                                          int32_t* __restrict__ etidx) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= nvalid || !keep[j]) return;
+  const uint32_t o = off[j];
+  ekey[o] = static_cast<KeyT>(keys[j]);
+  egs[o] = tmp_gs[j];
+  etidx[o] = tmp_tidx[j];
+}
+
+// bucket[h] = first entry whose key prefix (key >> shift) is >= h; the
+// caller pre-fills every bucket with n (entries past the last prefix).
+template <class KeyT>
+static __global__ __launch_bounds__(kBlock) void k_bucket_heads(const KeyT* __restrict__ ekey, uint32_t n, int shift,
+                                                         uint32_t* __restrict__ bucket) {
+  const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const uint64_t h = static_cast<uint64_t>(ekey[e]) >> shift;
+  const int64_t hp = e == 0 ? -1 : static_cast<int64_t>(static_cast<uint64_t>(ekey[e - 1]) >> shift);
+  for (int64_t x = hp + 1; x <= static_cast<int64_t>(h); ++x) bucket[x] = e;
+}
+
+// ---------------------------------------------------------------- seeding
+
+template <class KeyT>
+struct SeedArgs {
+  const uint8_t* qbase;
+  const uint32_t* qslot;
+  const int32_t* qlen;
+  const uint32_t* pos_off;  // nseq + 1
+  uint32_t nseq, npos;
+  const KeyT* ekey;
+  const int32_t* egs;
+  const int32_t* etidx;
+  const uint32_t* bucket;
+  int shift;
+  const int32_t* tlen;  // per target sequence
+  int lmer, gap, local;
+};
+
+// Strand of flat k-mer position p; the workgroup narrows the search range
+// to the strands its positions span.
+__device__ __forceinline__ uint32_t strand_of(const uint32_t* pos_off, uint32_t nseq, uint32_t npos, uint32_t p,
+                                              uint32_t* lds2) {
+  const uint32_t first = blockIdx.x * kBlock;
+  uint32_t last = first + kBlock - 1;
+  if (last >= npos) last = npos - 1;
+  if (threadIdx.x == 0) lds2[0] = upper_bound_u(pos_off, 0u, nseq + 1, first) - 1;
+  if (threadIdx.x == 1) lds2[1] = upper_bound_u(pos_off, 0u, nseq + 1, last);
+  __syncthreads();
+  return upper_bound_u(pos_off, lds2[0], lds2[1], p) - 1;
+}
+
+// Query k-mer lookup (krnlBinarySearch.cu + …createRawSeedList.cu:31-135):
+// the matching entry range of position p, and how many survive the corner
+// filter (…deleteSeedsOnSequenceBoundary.cu:129-188, global mode only).
+// Only positions with qIdx + lMer <= qLen exist here: the query-boundary
+// filter (:19-72) removes every other one.
+template <class KeyT>
+static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs<KeyT> a, uint32_t* __restrict__ m0,
+                                                       uint32_t* __restrict__ rawcnt, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t lds2[2];
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
+  if (p >= a.npos) return;
+  const int qIdx = static_cast<int>(p - a.pos_off[s]);
+  const int qLen = a.qlen[s];
+  const uint8_t* b = a.qbase + a.qslot[s] + qIdx;
+  uint64_t key = 0;
+  bool ok = true;
+  for (int j = 0; j < a.lmer; ++j) {
+    const uint32_t c = base_code(b[j]);
+    ok = ok && c < 4u;
+    key = (key << 2) | (c & 3u);
+  }
+  uint32_t lo = 0, hi = 0, c = 0;
+  if (ok) {
+    const uint64_t h = key >> a.shift;
+    const uint32_t b0 = a.bucket[h], b1 = a.bucket[h + 1];
+    lo = lower_bound_u(a.ekey, b0, b1, static_cast<KeyT>(key));
+    hi = upper_bound_u(a.ekey, lo, b1, static_cast<KeyT>(key));
+    if (a.local) {
+      c = hi - lo;
+    } else {
+      for (uint32_t j = lo; j < hi; ++j) {
+        const int d = a.etidx[j] - qIdx;
+        const int tl = a.tlen[a.egs[j]];
+        c += (d + a.gap < 0 || d + qLen - a.gap > tl) ? 0u : 1u;
+      }
+    }
+  }
+  m0[p] = lo;
+  rawcnt[p] = hi - lo;
+  cnt[p] = c;
+}
+
+struct SeedPack {
+  int gs_bits, d_bits;
+  int d_off;
+};
+
+// Writes the surviving seeds of each position at its scanned offset, in
+// query-position then cell order (krnlWriteSeedList.cu:3-37 order).
+template <class KeyT>
+static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs<KeyT> a, SeedPack pk,
+                                                       const uint32_t* __restrict__ m0,
+                                                       const uint32_t* __restrict__ rawcnt,
+                                                       const uint32_t* __restrict__ off, uint64_t* __restrict__ skey,
+                                                       uint32_t* __restrict__ sval) {
+  __shared__ uint32_t lds2[2];
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
+  if (p >= a.npos) return;
+  const uint32_t n = rawcnt[p];
+  if (n == 0) return;
+  const int qIdx = static_cast<int>(p - a.pos_off[s]);
+  const int qLen = a.qlen[s];
+  uint32_t o = off[p];
+  const uint32_t lo = m0[p];
+  for (uint32_t j = lo; j < lo + n; ++j) {
+    const int gs = a.egs[j];
+    const int d = a.etidx[j] - qIdx;
+    if (!a.local && (d + a.gap < 0 || d + qLen - a.gap > a.tlen[gs])) continue;
+    skey[o] = (static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits)) | (static_cast<uint64_t>(gs) << pk.d_bits) |
+              static_cast<uint64_t>(d + pk.d_off);
+    sval[o] = static_cast<uint32_t>(qIdx);
+    ++o;
+  }
+}
+
+struct SeedDecode {
+  int gs_bits, d_bits, d_off;
+  __device__ __forceinline__ uint32_t s(uint64_t k) const { return static_cast<uint32_t>(k >> (gs_bits + d_bits)); }
+  __device__ __forceinline__ uint32_t gs(uint64_t k) const {
+    return static_cast<uint32_t>((k >> d_bits) & ((1ull << gs_bits) - 1ull));
+  }
+  __device__ __forceinline__ int d(uint64_t k) const {
+    return static_cast<int>(k & ((1ull << d_bits) - 1ull)) - d_off;
+  }
+};
+
+// hasNearPair core (…deleteDuplicateSeeds.cuh:29-38) for seeds of one
+// (qID, tID) group; `post` is the predecessor.
+__device__ __forceinline__ bool near_pair(int t, int d, int pt, int pd, int width, int gap) {
+  return width >= t - pt && width >= pt - t && gap >= pd - d;
+}
+
+// Per sorted seed: near-predecessor flag, group head index (for the
+// group-start max-scan), and the first seed of every (target window, query
+// window) unit — the element 0 that …deleteIsolateSeeds.cu:28-76 keeps.
+static __global__ __launch_bounds__(kBlock) void k_seed_marks(const uint64_t* __restrict__ skey,
+                                                       const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
+                                                       int width, int gap, const int32_t* __restrict__ seq_win_t,
+                                                       const int32_t* __restrict__ seq_win_q, int nqwin,
+                                                       uint8_t* __restrict__ nearf, uint32_t* __restrict__ gidx,
+                                                       uint32_t* __restrict__ unit_first) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ns) return;
+  const uint64_t k = skey[i];
+  const uint64_t grp = k >> dc.d_bits;
+  const uint32_t s = dc.s(k), gs = dc.gs(k);
+  const int d = dc.d(k), q = static_cast<int>(sval[i]), t = d + q;
+  bool nr = false, ghead = true, uhead = true;
+  const int tw = seq_win_t[gs];
+  if (i > 0) {
+    const uint64_t pk = skey[i - 1];
+    if ((pk >> dc.d_bits) == grp) {
+      ghead = false;
+      const int pd = dc.d(pk), pq = static_cast<int>(sval[i - 1]);
+      nr = near_pair(t, d, pd + pq, pd, width, gap);
+    }
+    uhead = dc.s(pk) != s || seq_win_t[dc.gs(pk)] != tw;
+  }
+  if (uhead) atomicMin(&unit_first[tw * nqwin + seq_win_q[s]], i);
+  nearf[i] = nr ? 1 : 0;
+  gidx[i] = ghead ? i : 0u;
+}
+
+// Isolate filter keep flag K(i) = near(i) or unit element 0; stored as i+1
+// for the last-kept max-scan.
+static __global__ __launch_bounds__(kBlock) void k_seed_keep(const uint64_t* __restrict__ skey, uint32_t ns, SeedDecode dc,
+                                                      const int32_t* __restrict__ seq_win_t,
+                                                      const int32_t* __restrict__ seq_win_q, int nqwin,
+                                                      const uint8_t* __restrict__ nearf,
+                                                      const uint32_t* __restrict__ unit_first,
+                                                      uint32_t* __restrict__ kidx) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ns) return;
+  const uint64_t k = skey[i];
+  const uint32_t u = seq_win_t[dc.gs(k)] * nqwin + seq_win_q[dc.s(k)];
+  const bool keep = nearf[i] || unit_first[u] == i;
+  kidx[i] = keep ? i + 1 : 0u;
+}
+
+// Duplicate filter (…deleteDuplicateSeeds.cu:15-71) on the isolate output:
+// a kept seed survives unless its previous kept seed in the same group is
+// near.  lastk = inclusive max-scan of kidx, gstart = of gidx.
+static __global__ __launch_bounds__(kBlock) void k_seed_dup(const uint64_t* __restrict__ skey,
+                                                     const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
+                                                     int width, int gap, const uint32_t* __restrict__ kidx,
+                                                     const uint32_t* __restrict__ lastk,
+                                                     const uint32_t* __restrict__ gstart, uint32_t* __restrict__ dflag) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ns) return;
+  uint32_t f = 0;
+  if (kidx[i]) {
+    f = 1;
+    const uint32_t pk = i > 0 ? lastk[i - 1] : 0u;
+    if (pk >= 1 && pk - 1 >= gstart[i]) {
+      const uint32_t j = pk - 1;
+      const uint64_t k = skey[i], kj = skey[j];
+      const int d = dc.d(k), t = d + static_cast<int>(sval[i]);
+      const int dj = dc.d(kj), tj = dj + static_cast<int>(sval[j]);
+      f = near_pair(t, d, tj, dj, width, gap) ? 0u : 1u;
+    }
+  }
+  dflag[i] = f;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* __restrict__ skey,
+                                                         const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
+                                                         const uint32_t* __restrict__ dflag,
+                                                         const uint32_t* __restrict__ off, int32_t* __restrict__ hs,
+                                                         int32_t* __restrict__ hgs, int32_t* __restrict__ ht,
+                                                         int32_t* __restrict__ hq) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ns || !dflag[i]) return;
+  const uint32_t o = off[i];
+  const uint64_t k = skey[i];
+  const int q = static_cast<int>(sval[i]);
+  hs[o] = static_cast<int32_t>(dc.s(k));
+  hgs[o] = static_cast<int32_t>(dc.gs(k));
+  ht[o] = dc.d(k) + q;
+  hq[o] = q;
+}
+
+// ---------------------------------------------------------------- banded DP
+
+struct DPArgs {
+  uint32_t n;
+  const int32_t* hs;
+  const int32_t* hgs;
+  const int32_t* ht;
+  const int32_t* hq;
+  const uint8_t* qbase;
+  const uint32_t* qslot;
+  const int32_t* qlen;
+  const uint8_t* tbase;
+  const uint64_t* tslot;   // slot of each target sequence's first base
+  const int32_t* tlen;
+  const int32_t* twin;     // window of each target sequence
+  const uint64_t* wslot;   // window slot bounds (nwin + 1)
+  int lmer;
+  int32_t* o_tidx;
+  int32_t* o_qidx;
+  int32_t* o_thl;
+  int32_t* o_qhl;
+  int32_t* o_match;
+  int32_t* o_score;
+};
+
+constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
+
+// Target base class at intra-sequence index ti.  The backward direction
+// guards ti >= tLen and the forward direction ti < 0, exactly like
+// krnlAlignment.cu:121-127,160-165 / 234-240,271-276; a read that leaves the
+// window's slot array is 'X'.
+__device__ __forceinline__ uint32_t t_class(const uint8_t* tbase, uint64_t seq_slot, int ti, int tlen, bool backward,
+                                            uint64_t wlo, uint64_t whi) {
+  if (backward ? ti >= tlen : ti < 0) return kClassX;
+  const int64_t at = static_cast<int64_t>(seq_slot) + ti;
+  if (at < static_cast<int64_t>(wlo) || at >= static_cast<int64_t>(whi)) return kClassX;
+  return dp_class(tbase[at]);
+}
+
+// One query row of the band (calcutlateCell, krnlAlignment.cu:42-77, for
+// bands 1..W in order; bands 0 and W+1 are the constant margins).
+template <int W, bool LOCAL>
+__device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&H)[W + 2], int (&M)[W + 2],
+                                       const uint32_t (&T)[W], uint32_t qc, int r, int& bestS, int& bestM, int& bestR,
+                                       int& bestB) {
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const int v = max(S[b + 1] + (kGapOpen + kGapExt), V[b + 1] + kGapExt);
+    const int h = max(S[b - 1] + (kGapOpen + kGapExt), H[b - 1] + kGapExt);
+    const uint32_t tc = T[b - 1];
+    const uint32_t u = tc | qc;
+    const bool m = (tc & qc & 0xFu) != 0u;
+    const int pm = (u & kClassX) ? kBad : ((u & 0x20u) ? -1 : (m ? kMatch : kMismatch));
+    const int mc = (u & kClassX) ? kBad : ((u & 0x20u) ? -1 : (m ? 1 : 0));
+    const int pt = S[b] + pm;
+    int ns, nm;
+    if (pt > max(v, h)) {
+      ns = pt;
+      nm = M[b] + mc;
+    } else if (v > h) {
+      ns = v;
+      nm = M[b + 1];
+    } else {
+      ns = h;
+      nm = M[b - 1];
+    }
+    S[b] = ns;
+    V[b] = v;
+    H[b] = h;
+    M[b] = nm;
+    if (LOCAL) {
+      if (bestS < ns) {
+        bestS = ns;
+        bestM = nm;
+        bestR = r;
+        bestB = b - 1;
+      }
+    }
+  }
+}
+
+// initTempNodeArray (krnlAlignment.cu:6-38) for one hit's band.
+template <int G>
+__device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3], int (&H)[2 * G + 3],
+                                        int (&M)[2 * G + 3]) {
+  constexpr int half = G + 1;
+#pragma unroll
+  for (int b = 0; b < 2 * G + 3; ++b) {
+    if (b < half) {
+      S[b] = V[b] = H[b] = M[b] = -30000;
+    } else if (b == half) {
+      S[b] = 0; V[b] = kGapOpen; H[b] = kGapOpen; M[b] = 0;
+    } else {
+      const int sc = (b - half) * kGapExt + kGapOpen;
+      S[b] = sc; V[b] = sc + kGapOpen; H[b] = sc; M[b] = 0;
+    }
+  }
+}
+
+// One thread per hit, the band held in VGPRs: the right extension
+// ({global,local}AlignBackward) then the left one ({global,local}AlignForward).
+// The two directions are independent (the forward pass reads the seed's
+// tIdx/qIdx, which the backward pass does not change), so one thread runs
+// both and sums the results.
+template <int G, bool LOCAL>
+static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
+  constexpr int W = 2 * G + 1;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
+  const int L = a.lmer;
+  const int qLen = a.qlen[s];
+  const uint8_t* qb = a.qbase + a.qslot[s];
+  const int tLen = a.tlen[gs];
+  const uint64_t tslot = a.tslot[gs];
+  const int w = a.twin[gs];
+  const uint64_t wlo = a.wslot[w], whi = a.wslot[w + 1];
+  int S[W + 2], V[W + 2], H[W + 2], M[W + 2];
+  uint32_t T[W];
+  int oT = tIdx, oQ = qIdx, tHL = L, qHL = L, mm = L, sc = L * kMatch;
+
+#pragma unroll 1
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool bwd = dir == 0;
+    const int rows = bwd ? qLen - (qIdx + L) : qIdx;
+    if (rows == 0) continue;
+    dp_init<G>(S, V, H, M);
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+      T[k] = bwd ? t_class(a.tbase, tslot, tIdx + L - G + k, tLen, true, wlo, whi)
+                 : t_class(a.tbase, tslot, tIdx + G - k - 1, tLen, false, wlo, whi);
+    int bestS = 0, bestM = 0, bestR = -1, bestB = G;
+    const int q0 = bwd ? qIdx + L : qIdx - 1;
+    const int qstep = bwd ? 1 : -1;
+    const int tnext0 = bwd ? tIdx + L + G + 1 : tIdx - G - 2;
+    for (int r = 0; r < rows; ++r) {
+      const uint32_t qc = dp_class(qb[q0 + qstep * r]);
+      dp_row<W, LOCAL>(S, V, H, M, T, qc, r, bestS, bestM, bestR, bestB);
+#pragma unroll
+      for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
+      T[W - 1] = t_class(a.tbase, tslot, tnext0 + qstep * r, tLen, bwd, wlo, whi);
+    }
+    int move, endp;
+    if (LOCAL) {
+      move = bestR + 1;
+      endp = bestB - G;
+    } else {
+      // select end node: strict '<' from the constant band 0 (:170-186)
+      bestS = S[0];
+      bestM = M[0];
+      int e = -1;
+#pragma unroll
+      for (int b = 1; b <= W; ++b)
+        if (bestS < S[b]) {
+          bestS = S[b];
+          bestM = M[b];
+          e = b - 1;
+        }
+      move = rows;
+      endp = e - G;
+    }
+    if (!bwd) {
+      oT -= move + endp;
+      oQ -= move;
+    }
+    tHL += move + endp;
+    qHL += move;
+    mm += bestM;
+    sc += bestS;
+  }
+  a.o_tidx[i] = oT;
+  a.o_qidx[i] = oQ;
+  a.o_thl[i] = tHL;
+  a.o_qhl[i] = qHL;
+  a.o_match[i] = mm;
+  a.o_score[i] = sc;
+}
+
+// Launchers of k_dp for a runtime gap (2..16); each lives in its own
+// translation unit (dp_global.hip, dp_local.hip).
+void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
+void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
+
+// ---------------------------------------------------------------- results
+
+// calculateEvalue (krnlCalculateEvalue.cu:18-21) in FP64, the E-value cut
+// as an integer score threshold per strand (score >= smin[s] <=> E <= cutOff,
+// threshold computed on the host with the same formula), and the too-short
+// cut (…deleteBadHits.cuh:41-48).  Kept hits also widen the score range.
+static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const int32_t* __restrict__ hs,
+                                                       const int32_t* __restrict__ score,
+                                                       const int32_t* __restrict__ qhl,
+                                                       const int32_t* __restrict__ qlen,
+                                                       const int32_t* __restrict__ smin, int use_cut, int min_qhl,
+                                                       double kd, double lambda, double* __restrict__ ev,
+                                                       uint32_t* __restrict__ flag, int32_t* __restrict__ srange) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int s = hs[i];
+  const int sc = score[i];
+  ev[i] = kd * static_cast<double>(qlen[s]) * exp(-lambda * static_cast<double>(sc));
+  const bool keep = (!use_cut || sc >= smin[s]) && !(min_qhl > qhl[i]);
+  flag[i] = keep ? 1u : 0u;
+  if (keep) {
+    atomicMin(&srange[0], sc);
+    atomicMax(&srange[1], sc);
+  }
+}
+
+// Order keys: word0 = (maxq - seed qIdx, strand), word1 = (target window,
+// query pair, maxscore - score).  A stable sort by word0 then word1 over
+// hits in seed order yields every unit's CDeviceHitList order.
+static __global__ __launch_bounds__(kBlock) void k_order_keys(uint32_t n, const uint32_t* __restrict__ kept,
+                                                       const uint32_t* __restrict__ hoff,
+                                                       const int32_t* __restrict__ hs, const int32_t* __restrict__ hgs,
+                                                       const int32_t* __restrict__ hq,
+                                                       const int32_t* __restrict__ score,
+                                                       const int32_t* __restrict__ seq_win_t, int maxq, int pair_bits,
+                                                       int score_bits, int smax, uint64_t* __restrict__ w0,
+                                                       uint64_t* __restrict__ w1, uint32_t* __restrict__ rec) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !kept[i]) return;
+  const uint32_t o = hoff[i];
+  const uint32_t s = static_cast<uint32_t>(hs[i]);
+  w0[o] = (static_cast<uint64_t>(maxq - hq[i]) << 1) | (s & 1u);
+  w1[o] = (static_cast<uint64_t>(seq_win_t[hgs[i]]) << (pair_bits + score_bits)) |
+          (static_cast<uint64_t>(s >> 1) << score_bits) | static_cast<uint64_t>(smax - score[i]);
+  rec[o] = i;
+}
+
+// deleteHits_duplicateResult (CDeviceHitList.cu:32-128): in unit order, drop
+// a row equal in (tID, tIdx, qID, qIdx) to its predecessor of the same unit.
+static __global__ __launch_bounds__(kBlock) void k_dedup(uint32_t n, const uint32_t* __restrict__ perm,
+                                                  const uint32_t* __restrict__ rec, const int32_t* __restrict__ hs,
+                                                  const int32_t* __restrict__ hgs, const int32_t* __restrict__ ot,
+                                                  const int32_t* __restrict__ oq,
+                                                  const int32_t* __restrict__ seq_win_t,
+                                                  const int32_t* __restrict__ seq_win_q, uint32_t* __restrict__ flag) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t f = 1;
+  if (i > 0) {
+    const uint32_t a = rec[perm[i]], b = rec[perm[i - 1]];
+    const bool same_unit = seq_win_t[hgs[a]] == seq_win_t[hgs[b]] && seq_win_q[hs[a]] == seq_win_q[hs[b]];
+    if (same_unit && hgs[a] == hgs[b] && ot[a] == ot[b] && hs[a] == hs[b] && oq[a] == oq[b]) f = 0;
+  }
+  flag[i] = f;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_compact_perm(uint32_t n, const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ rec,
+                                                         const uint32_t* __restrict__ flag,
+                                                         const uint32_t* __restrict__ off, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  out[off[i]] = rec[perm[i]];
+}
+
+// fixResult order key (query pair, maxscore - score, target window).
+static __global__ __launch_bounds__(kBlock) void k_final_keys(uint32_t n, const uint32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ hs, const int32_t* __restrict__ hgs,
+                                                       const int32_t* __restrict__ score,
+                                                       const int32_t* __restrict__ seq_win_t, int score_bits,
+                                                       int win_bits, int smax, uint64_t* __restrict__ w) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = order[i];
+  w[i] = (static_cast<uint64_t>(static_cast<uint32_t>(hs[r]) >> 1) << (score_bits + win_bits)) |
+         (static_cast<uint64_t>(smax - score[r]) << win_bits) | static_cast<uint64_t>(seq_win_t[hgs[r]]);
+}
+
+struct OutArgs {
+  uint32_t n;
+  const uint32_t* order;   // record index per output row (may be composed with perm)
+  const uint32_t* perm;    // optional second level (null = identity)
+  const int32_t* hs;
+  const int32_t* hgs;
+  const int32_t* ot;
+  const int32_t* oq;
+  const int32_t* thl;
+  const int32_t* qhl;
+  const int32_t* match;
+  const int32_t* score;
+  const double* ev;
+  const int32_t* tref;
+  const int32_t* qref;
+  const int32_t* twin;
+  const int32_t* qwin;
+  int32_t* out_i32;   // 10 planes of n: tID tIdx qID qIdx tHL qHL match score twin qwin
+  double* out_ev;
+};
+
+static __global__ __launch_bounds__(kBlock) void k_gather_out(OutArgs a) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const uint32_t r = a.order[a.perm ? a.perm[i] : i];
+  const int gs = a.hgs[r], s = a.hs[r];
+  int32_t* o = a.out_i32;
+  const uint32_t n = a.n;
+  o[0 * n + i] = a.tref[gs];
+  o[1 * n + i] = a.ot[r];
+  o[2 * n + i] = a.qref[s];
+  o[3 * n + i] = a.oq[r];
+  o[4 * n + i] = a.thl[r];
+  o[5 * n + i] = a.qhl[r];
+  o[6 * n + i] = a.match[r];
+  o[7 * n + i] = a.score[r];
+  o[8 * n + i] = a.twin[gs];
+  o[9 * n + i] = a.qwin[s];
+  a.out_ev[i] = a.ev[r];
+}
+
+// resultSorting key for clast_result_sort: (qID/2, -score) biased to u64.
+static __global__ void k_result_keys(uint32_t n, const int32_t* __restrict__ qid, const int32_t* __restrict__ score,
+                              uint64_t* __restrict__ w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // signed -> order-preserving unsigned by flipping the sign bit; the score
+  // half is complemented so that a larger score sorts first
+  const uint64_t q = static_cast<uint64_t>(static_cast<uint32_t>(qid[i] / 2) ^ 0x80000000u);
+  const uint64_t sc = static_cast<uint64_t>(0xFFFFFFFFu - (static_cast<uint32_t>(score[i]) ^ 0x80000000u));
+  w[i] = (q << 32) | sc;
+}
+
+}  // namespace clast

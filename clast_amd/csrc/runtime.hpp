@@ -1,0 +1,134 @@
+// Per-device runtime of libclast_hip: one HIP stream (default priority), a
+// growable named workspace for stage scratch, pinned staging for scalar
+// read-backs, and the error type that the C ABI turns into status codes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+namespace clast {
+
+struct ClastError : std::runtime_error {
+  int code;
+  ClastError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? 3 : 2;
+  throw ClastError(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// A device allocation owned by a handle.
+template <class T>
+struct DevArray {
+  T* ptr = nullptr;
+  uint64_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  ~DevArray() { reset(); }
+  void reset() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void alloc(uint64_t count) {
+    reset();
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(&ptr, sizeof(T) * count), "hipMalloc");
+    n = count;
+  }
+  void upload(const T* host, uint64_t count, hipStream_t s) {
+    alloc(count);
+    if (count) hip_check(hipMemcpyAsync(ptr, host, sizeof(T) * count, hipMemcpyHostToDevice, s), "upload");
+  }
+};
+
+// Named scratch buffers that only grow.  A slot keeps its pointer until a
+// larger request for the same name; growth waits for the stream first so no
+// queued kernel still uses the old buffer.
+class Workspace {
+ public:
+  explicit Workspace(hipStream_t s) : stream_(s) {}
+  ~Workspace() {
+    for (auto& kv : slots_) (void)hipFree(kv.second.ptr);
+  }
+  template <class T>
+  T* get(const std::string& name, uint64_t count) {
+    const uint64_t bytes = (count == 0 ? 1 : count) * sizeof(T);
+    Slot& s = slots_[name];
+    if (s.bytes < bytes) {
+      if (s.ptr) {
+        hip_check(hipStreamSynchronize(stream_), "workspace sync");
+        hip_check(hipFree(s.ptr), "hipFree");
+        s.ptr = nullptr;
+        s.bytes = 0;
+      }
+      const uint64_t want = bytes + bytes / 4;
+      hip_check(hipMalloc(&s.ptr, want), "workspace hipMalloc");
+      s.bytes = want;
+    }
+    return static_cast<T*>(s.ptr);
+  }
+  uint64_t bytes() const {
+    uint64_t b = 0;
+    for (auto& kv : slots_) b += kv.second.bytes;
+    return b;
+  }
+
+ private:
+  struct Slot {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+  };
+  hipStream_t stream_;
+  std::map<std::string, Slot> slots_;
+};
+
+class Runtime {
+ public:
+  explicit Runtime(int device) : device_(device) {
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipHostMalloc(&pinned_, kPinned, hipHostMallocDefault), "hipHostMalloc");
+    ws_ = new Workspace(stream_);
+  }
+  ~Runtime() {
+    delete ws_;
+    (void)hipHostFree(pinned_);
+    (void)hipStreamDestroy(stream_);
+  }
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+  Workspace& ws() { return *ws_; }
+  std::mutex& mutex() { return mu_; }
+
+  void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+  void sync() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+  // Reads `count` device scalars of type T (count * sizeof(T) <= 4 KiB).
+  template <class T>
+  void read_scalars(const T* dev, T* host, int count) {
+    hip_check(hipMemcpyAsync(pinned_, dev, sizeof(T) * count, hipMemcpyDeviceToHost, stream_), "scalar D2H");
+    sync();
+    for (int i = 0; i < count; ++i) host[i] = static_cast<const T*>(pinned_)[i];
+  }
+
+ private:
+  static constexpr size_t kPinned = 4096;
+  int device_;
+  hipStream_t stream_ = nullptr;
+  void* pinned_ = nullptr;
+  Workspace* ws_ = nullptr;
+  std::mutex mu_;
+};
+
+// The runtime of the calling thread's current device (created on first use).
+Runtime& runtime();
+
+}  // namespace clast

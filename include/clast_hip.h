@@ -1,0 +1,161 @@
+/*
+ * libclast_hip — MI355X (gfx950) implementation of CLAST's device stages
+ * behind a plain C ABI.  It replaces the reference's three device objects
+ * and the result sort they feed:
+ *
+ *   CDeviceHashTable(setting, targetList, t_begin, t_end)   CDeviceHashTable.cuh:17-21
+ *       -> clast_index_create / clast_index_destroy
+ *   CDeviceSeqList_query(setting, &queryList, q_begin, q_end) CDeviceSeqList_query.cuh:14-18
+ *       -> clast_query_create / clast_query_destroy
+ *   CDeviceHitList(setting, hashTable, queryList, t_begin, 2*q_begin)
+ *                                                           CDeviceHitList.cuh:23-28
+ *       -> clast_search, then hitList.getResult(holder)     CDeviceHitList.cuh:29
+ *       -> clast_hits_count / clast_hits_copy / clast_hits_destroy
+ *   resultSorting(...) used by CHostResultHolder::fixResult utilResultSorting.cuh:28-37
+ *       -> clast_result_sort
+ *
+ * Batched extension (no single counterpart in the reference): an index over
+ * several target windows and a query set over several query windows are
+ * searched in one launch chain by clast_search_batch; the output equals the
+ * reference's per-window results appended in (target window, query window)
+ * order and stable-sorted by (qID/2, score desc) as fixResult does
+ * (CHostResultHolder.cu:75-128).
+ *
+ * Conventions: every call returns CLAST_OK (0) or an error code and never
+ * throws; the message of the last failure on the calling thread is
+ * available from clast_last_error().  Host buffers belong to the caller and
+ * are not retained after a call returns; device memory belongs to the handle.
+ * Handles are single-owner; calls on different handles may come from
+ * different threads, but the library serialises its device work on one
+ * stream per device.
+ */
+#ifndef CLAST_HIP_H_
+#define CLAST_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CLAST_OK = 0,
+  CLAST_ERR_ARG = 1,    /* invalid argument or handle */
+  CLAST_ERR_HIP = 2,    /* a HIP runtime call failed */
+  CLAST_ERR_NOMEM = 3,  /* device allocation failed */
+  CLAST_ERR_LIMIT = 4   /* input exceeds a representable size (split it) */
+};
+
+/* Stage-retention flag for clast_search_batch (stage-level parity checks). */
+enum { CLAST_KEEP_STAGES = 1 };
+
+/* Stages retained with CLAST_KEEP_STAGES, readable with clast_hits_stage_*. */
+enum {
+  CLAST_STAGE_SEEDS = 1,    /* seeds after the 3 boundary filters, generation order */
+  CLAST_STAGE_SORTED = 2,   /* seeds after sortSeeds */
+  CLAST_STAGE_HITS = 3,     /* seeds surviving isolate + duplicate (DP input) */
+  CLAST_STAGE_ALIGNED = 4   /* DP output, one row per CLAST_STAGE_HITS entry */
+};
+
+/* CHostSetting's search parameters (CHostSetting.cuh:10-28).  total_db_size
+ * is calcTotalDbSize(targets) (CHostSetting.cu:46-58); K and lambda are
+ * GLOBAL_K / GLOBAL_LAMBDA (common.hpp:42-43). */
+typedef struct clast_params {
+  int32_t lmer;        /* -lMer   (<= 31) */
+  int32_t stride;      /* -stride */
+  int32_t cut_repeat;  /* -repeat (-1: unlimited) */
+  int32_t width;       /* -width  */
+  int32_t gap;         /* -gap    (2..16) */
+  int32_t local;       /* -local  (0: global) */
+  double cut_off;      /* -cutOff (-1: no cut off) */
+  double total_db_size;
+  double K;
+  double lambda;
+} clast_params;
+
+/* One window of a host sequence list, exactly what
+ * CHostSeqList::get{Base,Length,Gateway}Array(startID, endID) hands the
+ * device (CHostSeqList.cu:15-70): the window's slot array (each sequence
+ * followed by a copy of its first lMer-1 bases), per-sequence lengths and
+ * per-sequence start slots rebased to 0.  id_begin is the list ID of the
+ * window's first sequence (t_begin for targets, 2*q_begin for queries); the
+ * per-slot ID and index arrays are implied and not passed. */
+typedef struct clast_window {
+  const char* bases;
+  int64_t n_slots;
+  const int32_t* lengths;
+  const int32_t* gateway;
+  int32_t n_seq;
+  int32_t id_begin;
+} clast_window;
+
+typedef struct clast_index_s* clast_index;
+typedef struct clast_query_s* clast_query;
+typedef struct clast_hits_s* clast_hits;
+
+/* cudaSetDevice(setting.getDeviceID()) (main.cu:30). Optional: the first
+ * call on a thread without it uses device 0. */
+int clast_init(int32_t device);
+const char* clast_last_error(void);
+/* Library version string. */
+const char* clast_version(void);
+
+/* CDeviceHashTable ctor (CDeviceHashTable.cu:143-174): target k-mer index of
+ * one target window. */
+int clast_index_create(const clast_params* p, const clast_window* target, clast_index* out);
+/* The same index over n_windows target windows at once (kept resident;
+ * windows are numbered 0..n_windows-1 in the given order). */
+int clast_index_create_multi(const clast_params* p, const clast_window* targets, int32_t n_windows,
+                             clast_index* out);
+/* Entries of the index (test introspection): per entry its k-mer code,
+ * window, list ID and intra-sequence index, in key order. */
+int64_t clast_index_entry_count(clast_index idx);
+int clast_index_entries(clast_index idx, int64_t* key, int32_t* window, int32_t* tID, int32_t* tIdx);
+int clast_index_destroy(clast_index idx);
+
+/* CDeviceSeqList_query ctor (CDeviceSeqList_query.cu:61-71): uploads one
+ * query window ('+' and '-' strand of each query, IDs 2k and 2k+1). */
+int clast_query_create(const clast_params* p, const clast_window* query, clast_query* out);
+int clast_query_create_multi(const clast_params* p, const clast_window* queries, int32_t n_windows,
+                             clast_query* out);
+int clast_query_destroy(clast_query q);
+
+/* CDeviceHitList ctor (CDeviceHitList.cu:228-423): the whole seed -> filter
+ * -> DP -> E-value -> sort -> dedup chain for one (target window, query
+ * window).  idx and q must each hold one window whose id_begin equals
+ * t_begin and q_begin. */
+int clast_search(const clast_params* p, clast_index idx, clast_query q, int32_t t_begin, int32_t q_begin,
+                 clast_hits* out);
+/* Every (target window, query window) unit of idx x q in one launch chain,
+ * output in fixResult order (see above).  flags: 0 or CLAST_KEEP_STAGES. */
+int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, int32_t flags,
+                       clast_hits* out);
+
+/* CDeviceHitList::getResult (CDeviceHitList.cu:425-451): the 9 result
+ * arrays (tID and qID are list IDs).  Null pointers are skipped. */
+int64_t clast_hits_count(clast_hits h);
+int clast_hits_copy(clast_hits h, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                    int32_t* qHL, int32_t* match, int32_t* score, double* evalue);
+/* Target / query window index of each result row (batched search). */
+int clast_hits_copy_windows(clast_hits h, int32_t* twin, int32_t* qwin);
+/* Retained stages (CLAST_KEEP_STAGES): seed stages fill tID..qIdx; the
+ * ALIGNED stage fills all eight int arrays. */
+int64_t clast_hits_stage_count(clast_hits h, int32_t stage);
+int clast_hits_stage_copy(clast_hits h, int32_t stage, int32_t* tID, int32_t* tIdx, int32_t* qID,
+                          int32_t* qIdx, int32_t* tHL, int32_t* qHL, int32_t* match, int32_t* score);
+/* Device time of the last search's stages, milliseconds (hipEvents):
+ * out[0] seed count+write, [1] seed sort, [2] isolate/duplicate compaction,
+ * [3] DP, [4] E-value + filters + result order, [5] total. */
+int clast_hits_stage_times(clast_hits h, float* out6);
+int clast_hits_destroy(clast_hits h);
+
+/* resultSorting (utilResultSorting.cu:6-47): stable sort of n result rows by
+ * (qID/2 ascending, score descending) on the device, in place. */
+int clast_result_sort(int64_t n, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                      int32_t* qHL, int32_t* match, int32_t* score, double* evalue);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLAST_HIP_H_ */

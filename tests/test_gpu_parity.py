@@ -1,0 +1,173 @@
+"""GPU parity: every stage of the HIP pipeline against the CPU oracle on the
+same seeded inputs (integers bit-exact, E-values within 1e-9 relative)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from clast_amd import gen, native
+from common import FIELDS, assert_rows_equal, db_size, lists, sorted_rows, workload
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    print("init device 0", flush=True)
+    native.init(0)
+
+
+def expected_entries(p, tl, twins):
+    """Index entries implied by the oracle's CDeviceHashTable of each window:
+    cells with a non-zero (post-cut) size, positions whose k-mer does not run
+    past the sequence end, in (key, window, position) order."""
+    rows = []
+    for w, (a, b) in enumerate(twins):
+        ix = oracle.index(p, tl, a, b)
+        lo = int(tl.gateway[a])
+        gw = tl.gateway[a:b + 1] - lo
+        for k in range(len(ix["key"])):
+            c = int(ix["cell_size"][k])
+            if c <= 0 or ix["key"][k] < 0:
+                continue
+            o = int(ix["gateway_index"][k])
+            for pos in ix["positions"][o:o + c]:
+                s = int(np.searchsorted(gw, pos, side="right")) - 1
+                tidx = int(pos - gw[s])
+                if tidx + p.lmer <= int(tl.length[a + s]):
+                    rows.append((int(ix["key"][k]), w, a + s, tidx))
+    rows.sort(key=lambda r: (r[0], r[1]))  # stable: windows in order, positions ascending
+    return np.array(rows, dtype=np.int64).reshape(-1, 4)
+
+
+@pytest.mark.parametrize("lmer,stride,cut", [(15, 5, 20), (11, 3, 4), (20, 4, -1)])
+def test_index_entries(lmer, stride, cut):
+    t, r = workload(n_t=5, t_len=12_000, n_r=10, repeat_frac=0.2, lowc=0.02)
+    p = native.Params(lmer=lmer, stride=stride, cut_repeat=cut, total_db_size=db_size(t))
+    tl, _ = lists(t, r, lmer)
+    twins = [(0, 2), (2, 5)]
+    print(f"index lmer={lmer} stride={stride} cut={cut}", flush=True)
+    ix = native.Index(p, tl, twins)
+    got = ix.entries()
+    exp = expected_entries(p, tl, twins)
+    assert len(got["key"]) == len(exp)
+    np.testing.assert_array_equal(got["key"], exp[:, 0])
+    np.testing.assert_array_equal(got["window"], exp[:, 1])
+    np.testing.assert_array_equal(got["tID"], exp[:, 2])
+    np.testing.assert_array_equal(got["tIdx"], exp[:, 3])
+
+
+def run_unit(p, t, r, tw=None, qw=None):
+    tl, ql = lists(t, r, p.lmer)
+    tw = tw or (0, tl.n_seq)
+    qw = qw or (0, ql.n_seq // 2)
+    ix = native.Index(p, tl, [tw])
+    q = native.Query(p, ql, [qw])
+    hits = native.search_batch(p, ix, q, keep_stages=True)
+    o = oracle.search(p, tl, [tw], ql, [qw], dump=True)
+    return hits, o
+
+
+SETTINGS = [
+    dict(),
+    dict(local=1),
+    dict(gap=2),
+    dict(gap=16, lmer=12),
+    dict(lmer=21, stride=3),
+    dict(cut_repeat=-1, width=30),
+    dict(cut_off=-1),
+]
+
+
+@pytest.mark.parametrize("kw", SETTINGS, ids=[str(k) or "default" for k in SETTINGS])
+def test_unit_stages(kw):
+    t, r = workload(n_t=6, t_len=25_000, n_r=250, repeat_frac=0.1, indel=0.02)
+    p = native.Params(total_db_size=db_size(t), **kw)
+    print(f"unit stages {kw}", flush=True)
+    hits, o = run_unit(p, t, r)
+    # seeds after the three boundary filters, in the reference's order
+    assert_rows_equal(hits.stage(native.STAGE_SEEDS), o["boundary"], FIELDS[:4], what="seeds")
+    assert_rows_equal(hits.stage(native.STAGE_SORTED), o["sorted"], FIELDS[:4], what="sorted seeds")
+    assert_rows_equal(hits.stage(native.STAGE_HITS), o["duplicate"], FIELDS[:4], what="hits")
+    # DP outputs: same multiset of extended hits
+    np.testing.assert_array_equal(sorted_rows(hits.stage(native.STAGE_ALIGNED)), sorted_rows(o["aligned"]))
+    got = hits.arrays()
+    assert_rows_equal(got, o["final"], what="final")
+    assert len(o["final"]["tID"]) > 0
+
+
+def test_plain_search_matches_batch_unit():
+    t, r = workload(n_t=4, t_len=20_000, n_r=200)
+    p = native.Params(total_db_size=db_size(t))
+    tl, ql = lists(t, r, p.lmer)
+    ix = native.Index(p, tl, [(0, 4)])
+    q = native.Query(p, ql, [(50, 200)])
+    got = native.search(p, ix, q, 0, 100).arrays()
+    o = oracle.search(p, tl, [(0, 4)], ql, [(50, 200)])
+    assert_rows_equal(got, o["final"], what="clast_search")
+
+
+def test_batch_many_windows():
+    t, r = workload(n_t=9, t_len=15_000, n_r=400, repeat_frac=0.15, random_frac=0.1)
+    p = native.Params(total_db_size=db_size(t))
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 40_000, query=False)
+    qwins = native.windows(ql.gateway, 30_000, query=True)
+    assert len(twins) > 2 and len(qwins) > 2
+    print(f"batch {len(twins)}x{len(qwins)} windows", flush=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    got = native.search_batch(p, ix, q).arrays()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert_rows_equal(got, o["final"], what="batch")
+    np.testing.assert_array_equal(got["twin"], o["final"]["twin"])
+    # the same rows, unit by unit, through the plain entry point
+    for w, (a, b) in enumerate(twins[:2]):
+        ixw = native.Index(p, tl, [(a, b)])
+        for qa, qb in qwins[:2]:
+            qq = native.Query(p, ql, [(qa, qb)])
+            one = native.search(p, ixw, qq, a, 2 * qa).arrays()
+            ou = oracle.search(p, tl, [(a, b)], ql, [(qa, qb)])
+            assert_rows_equal(one, ou["final"], what=f"unit {w},{qa}")
+
+
+def test_uneven_lengths_and_local():
+    t, r = workload(n_t=4, t_len=40_000, n_r=60, len_mode=gen.LEN_LOGUNIFORM, read_len=100, len_b=3000,
+                    random_frac=0.2, indel=0.05)
+    for local in (0, 1):
+        p = native.Params(total_db_size=db_size(t), local=local)
+        print(f"uneven local={local}", flush=True)
+        hits, o = run_unit(p, t, r)
+        assert_rows_equal(hits.arrays(), o["final"], what=f"uneven local={local}")
+
+
+def test_edge_inputs():
+    # reads shorter than lMer, reads of N, reads with X and other letters
+    t, _ = workload(n_t=3, t_len=10_000, n_r=1)
+    seqs = [b"ACGT", b"N" * 40, t.seq(0)[100:260], t.seq(1)[500:650].replace(b"A", b"X", 3),
+            t.seq(2)[1000:1150].replace(b"C", b"R", 5), b"", t.seq(0)[9900:10000]]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(s) for s in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t))
+    hits, o = run_unit(p, t, r)
+    assert_rows_equal(hits.arrays(), o["final"], what="edge")
+    # a query set with no k-mer position at all
+    short = gen.Reads(np.frombuffer(b"ACGTACGT", np.uint8).copy(), np.array([0, 8], np.int64))
+    hits, o = run_unit(p, t, short)
+    assert len(hits) == 0 and len(o["final"]["tID"]) == 0
+
+
+def test_result_sort():
+    rng = np.random.default_rng(3)
+    n = 5000
+    rows = {k: rng.integers(0, 1000, n).astype(np.int32) for k in FIELDS}
+    rows["qID"] = rng.integers(0, 200, n).astype(np.int32)
+    rows["score"] = rng.integers(-50, 50, n).astype(np.int32)
+    rows["evalue"] = rng.random(n)
+    order = np.lexsort((np.arange(n), -rows["score"].astype(np.int64), rows["qID"] // 2))
+    exp = {k: v[order] for k, v in rows.items()}
+    native.result_sort(rows)
+    for k in exp:
+        np.testing.assert_array_equal(rows[k], exp[k])
