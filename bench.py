@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the CLAST search path on one MI355X.
+
+Workload (default ``large``): 1 Gbp target (1,000 x 1 Mbp, 10% repeat
+content) and 1,000,000 reads of 150 bp, reference defaults (lMer 15, stride
+5, repeat 20, width 100, gap 8, global, cutOff 10; tRAM = qRAM = tVRAM = 64
+MBase, qVRAM = 2 MBase).  The target's 16 RAM chunks (one window each) are
+indexed once and kept resident in HBM; each of the 5 query RAM chunks (~35
+query windows each) is uploaded before the timed region.  One step = every
+query chunk searched against every target window (clast_search_batch per
+query chunk), hits copied back to host memory.
+
+value = query Mbp (one strand, as read) aligned per second over the K timed
+steps.  latency = per query-chunk search time (p50/p99 over all timed
+searches).  roofline = the seed lookup kernel (k_seed_count) against HBM
+peak.  cpu_baseline = the CPU oracle on a bounded sample (one target window
+x one query window, OpenMP), scaled to the whole target; the same sample is
+searched on the GPU and must match it bit-exactly.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_target(name: str, scale: float, p, t_ram: float, t_vram: float):
+    """Target chunks as the FASTA loader would cut them, their host lists and
+    windows, flattened in (chunk, window) order."""
+    from clast_amd import gen, native
+
+    tspec, _ = gen.workload(name, scale)
+    t = gen.make_target(tspec)
+    chunks, keep = native.chunks(t.lengths, int(t_ram))
+    out = []
+    for a, b in chunks:
+        idx = [i for i in range(a, b) if keep[i]]
+        if not idx:
+            continue
+        sub = t.subset(idx[0], idx[-1] + 1)
+        lst = native.build_list(sub.bases, sub.offsets, p.lmer, False)
+        wins = native.windows(lst.gateway, t_vram, False)
+        out.append((lst, wins))
+    return t, out
+
+
+def build_queries(name: str, scale: float, p, t, q_ram: float, q_vram: float):
+    from clast_amd import gen, native
+
+    _, rspec = gen.workload(name, scale)
+    r = gen.make_reads(rspec, t)
+    chunks, keep = native.chunks(r.lengths, int(q_ram / 2))
+    out = []
+    for a, b in chunks:
+        idx = [i for i in range(a, b) if keep[i]]
+        if not idx:
+            continue
+        sub = r.subset(idx[0], idx[-1] + 1)
+        lst = native.build_list(sub.bases, sub.offsets, p.lmer, True)
+        wins = native.windows(lst.gateway, q_vram, True)
+        out.append((lst, wins, int(sub.lengths.sum())))
+    return r, out
+
+
+def cpu_baseline(p, tchunks, qchunks, n_twin_total: int, threads: int):
+    """The oracle on one target window x one query window; the GPU searches
+    the same unit and must match bit-exactly."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from clast_amd import native
+
+    tl, twins = tchunks[0]
+    ql, qwins, _ = qchunks[0]
+    tw, qw = twins[0], qwins[0]
+    q_bases = int(ql.length[2 * qw[0]:2 * qw[1]:2].sum())
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.perf_counter()
+    exp = oracle.search(p, tl, [tw], ql, [qw])["final"]
+    cpu_s = time.perf_counter() - t0
+    ix = native.Index(p, tl, [tw])
+    q = native.Query(p, ql, [qw])
+    got = native.search(p, ix, q, tw[0], 2 * qw[0]).arrays()
+    ix.close()
+    q.close()
+    exact = len(got["tID"]) == len(exp["tID"]) and all(
+        np.array_equal(got[k], exp[k]) for k in ("tID", "tIdx", "qID", "qIdx", "tHL", "qHL", "match", "score"))
+    rel = float(np.max(np.abs(got["evalue"] - exp["evalue"]) / np.maximum(exp["evalue"], 1e-300))) \
+        if exact and len(exp["tID"]) else 0.0
+    mbps = q_bases / 1e6 / (cpu_s * n_twin_total)
+    sample = (f"oracle, 1 of {n_twin_total} target windows ({tw[1] - tw[0]} seqs) x 1 query window "
+              f"({qw[1] - qw[0]} reads, {q_bases / 1e6:.3f} Mbp) in {cpu_s:.1f} s; Mbp/s scaled by "
+              f"{n_twin_total} windows")
+    return {"value": round(mbps, 4), "unit": "query Mbp/s aligned", "cores": threads, "kind": "port",
+            "sample": sample}, {"rows": int(len(exp["tID"])), "bit_exact": bool(exact), "evalue_max_rel": rel}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="large")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="also time the per-window (one launch chain per unit) path")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--index", choices=("merged", "per-chunk"), default="merged",
+                    help="one resident index over every target window, or one per target chunk")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (the search shards by query "
+              "with no exchange; see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    from clast_amd import native
+
+    native.init(0)
+    p = native.Params()
+    t_ram = t_vram = 64e6
+    q_ram, q_vram = 64e6, 2e6
+    t0 = time.perf_counter()
+    t, tchunks = build_target(args.workload, args.scale, p, t_ram, t_vram)
+    p.total_db_size = float(t.bases.shape[0])  # all sequence-line bytes of the target FASTA
+    r, qchunks = build_queries(args.workload, args.scale, p, t, q_ram, q_vram)
+    log(f"generated {len(t)} targets ({t.bases.shape[0] / 1e6:.0f} Mbp), {len(r)} reads in "
+        f"{time.perf_counter() - t0:.1f} s; {len(tchunks)} target chunks, {len(qchunks)} query chunks")
+
+    # resident index: every window of every target chunk, (chunk, window) order
+    t1 = time.perf_counter()
+    indexes = []
+    n_twin = sum(len(w) for _, w in tchunks)
+    if args.index == "merged":
+        indexes.append(native.Index.merged(p, tchunks))
+    else:
+        for lst, wins in tchunks:
+            indexes.append(native.Index(p, lst, wins))
+    index_s = time.perf_counter() - t1
+    log(f"index built in {index_s:.2f} s over {n_twin} target windows")
+    queries = [native.Query(p, lst, wins) for lst, wins, _ in qchunks]
+    total_bases = sum(b for _, _, b in qchunks)
+    log(f"{len(queries)} query chunks resident, {total_bases / 1e6:.1f} Mbp")
+
+    def step(record):
+        rows = 0
+        for q in queries:
+            for ix in indexes:
+                ts = time.perf_counter()
+                h = native.search_batch(p, ix, q)
+                rows += len(h)
+                record.append((time.perf_counter() - ts, h.times()))
+                h.close()
+        return rows
+
+    for w in range(args.warmup):
+        step([])
+        log(f"warmup {w + 1}/{args.warmup} done")
+    rec = []
+    step_ms = []
+    rows = 0
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        rows = step(rec)
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {rows} hits")
+    ms = float(np.mean(step_ms))
+    value = total_bases / 1e6 / (ms / 1e3)
+    lat = np.array([x[0] * 1e3 for x in rec])
+    times = np.array([x[1] for x in rec])  # per search: count, write, sort, filt, dp, res, total, seeds
+    # k_seed_count algorithmic bytes per launch: 16 B per query k-mer position
+    # (key read + cell range) + 8 B per matched index entry; positions and
+    # entries per launch are taken from the query chunks and the seed totals
+    npos = []
+    for lst, wins, _ in qchunks:
+        ln = lst.length.astype(np.int64)
+        npos.append(int(np.clip(ln - p.lmer + 1, 0, None).sum()))
+    n_idx = len(indexes)
+    pos_per_launch = np.repeat(np.array(npos, dtype=np.float64), n_idx)
+    pos_per_launch = np.tile(pos_per_launch, args.steps)
+    seeds = times[:, 7].astype(np.float64)
+    count_ms = times[:, 0].astype(np.float64)
+    bytes_count = 16.0 * pos_per_launch + 8.0 * seeds
+    achieved = float(bytes_count.sum() / (count_ms.sum() / 1e3) / 1e9)
+    stage_names = ["seed_count", "seed_write", "seed_sort", "isolate_dup", "dp", "results", "search_total"]
+    stage_ms = {n: round(float(times[:, i].sum() / args.steps), 3) for i, n in enumerate(stage_names)}
+    dp_ms = float(times[:, 4].sum())
+    log(f"stage ms per step: {stage_ms}")
+
+    out = {
+        "metric": "query Mbp/s aligned (bit-exact hits) + p50/p99 request latency, 1 MI355X",
+        "value": round(value, 3),
+        "unit": "query Mbp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {t.bases.shape[0] / 1e9:.3g} Gbp target ({len(t)} seqs) x {len(r)} reads "
+                        f"({total_bases / 1e6:.0f} Mbp), reference defaults, batched resident index",
+            "target_windows": n_twin, "query_chunks": len(queries),
+            "query_windows": sum(len(w) for _, w, _ in qchunks), "hits": rows,
+        },
+        "latency_ms": {"unit": "one query chunk x one target chunk search",
+                       "p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3)},
+        "stage_ms_per_step": stage_ms,
+        "index_build_s": round(index_s, 3),
+        "roofline": {"kernel": "k_seed_count", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if args.plain:
+        tp = time.perf_counter()
+        for (lst, wins, _) in qchunks[:1]:
+            for (tl, twins) in tchunks:
+                for a, b in twins:
+                    ix = native.Index(p, tl, [(a, b)])
+                    for qa, qb in wins:
+                        q = native.Query(p, lst, [(qa, qb)])
+                        native.search(p, ix, q, a, 2 * qa).close()
+                        q.close()
+                    ix.close()
+        plain_s = time.perf_counter() - tp
+        out["plain_value"] = round(qchunks[0][2] / 1e6 / plain_s, 3)
+    if not args.no_cpu_baseline:
+        cb, parity = cpu_baseline(p, tchunks, qchunks, n_twin, args.cpu_threads)
+        out["cpu_baseline"] = cb
+        out["parity_sample"] = parity
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -225,6 +225,26 @@ class Index:
         lst.release_windows()
         self.h = h
 
+    @classmethod
+    def merged(cls, params: Params, lists_and_windows) -> "Index":
+        """One index over the windows of several host lists (e.g. every
+        target RAM chunk), numbered in the given (list, window) order."""
+        self = cls.__new__(cls)
+        self.params = params
+        self.wins = []
+        ws = []
+        for lst, wins in lists_and_windows:
+            ws.extend(lst.window(a, b) for a, b in wins)
+            self.wins.extend(wins)
+        arr = (CWindow * len(ws))(*ws)
+        h = C.c_void_p()
+        _check(hip().clast_index_create_multi(C.byref(params.c()), arr, len(ws), C.byref(h)),
+               "clast_index_create_multi")
+        for lst, _ in lists_and_windows:
+            lst.release_windows()
+        self.h = h
+        return self
+
     def entries(self) -> dict[str, np.ndarray]:
         n = int(hip().clast_index_entry_count(self.h))
         out = {"key": np.empty(n, np.int64), "window": np.empty(n, np.int32), "tID": np.empty(n, np.int32),
@@ -307,7 +327,8 @@ class Hits:
         return out
 
     def times(self) -> np.ndarray:
-        t = np.zeros(6, np.float32)
+        """seed count, seed write, seed sort, filters, DP, results, total (ms), seeds."""
+        t = np.zeros(8, np.float32)
         _check(hip().clast_hits_stage_times(self.h, _ptr(t)), "clast_hits_stage_times")
         return t
 

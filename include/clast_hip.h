@@ -143,10 +143,12 @@ int clast_hits_copy_windows(clast_hits h, int32_t* twin, int32_t* qwin);
 int64_t clast_hits_stage_count(clast_hits h, int32_t stage);
 int clast_hits_stage_copy(clast_hits h, int32_t stage, int32_t* tID, int32_t* tIdx, int32_t* qID,
                           int32_t* qIdx, int32_t* tHL, int32_t* qHL, int32_t* match, int32_t* score);
-/* Device time of the last search's stages, milliseconds (hipEvents):
- * out[0] seed count+write, [1] seed sort, [2] isolate/duplicate compaction,
- * [3] DP, [4] E-value + filters + result order, [5] total. */
-int clast_hits_stage_times(clast_hits h, float* out6);
+/* Device time of the search's stages in milliseconds, from hipEvents on the
+ * library stream: out[0] seed lookup/count kernel, [1] seed write kernel,
+ * [2] seed sort, [3] isolate/duplicate stage, [4] DP kernel, [5] E-value +
+ * filters + result order + copy-out, [6] whole search; out[7] is the number
+ * of seeds (after the boundary filters) as a float. */
+int clast_hits_stage_times(clast_hits h, float* out8);
 int clast_hits_destroy(clast_hits h);
 
 /* resultSorting (utilResultSorting.cu:6-47): stable sort of n result rows by
