@@ -114,7 +114,7 @@ struct clast_query_s {
   int lmer = 0;
   int nwin = 0;
   uint32_t nseq = 0, npos = 0;
-  int max_qlen = 0;
+  int max_qlen = 0, min_qlen = 0;
   DevArray<uint8_t> qbase;
   DevArray<uint32_t> qslot, pos_off;
   DevArray<int32_t> qlen, qref, qwin, smin;
@@ -304,6 +304,7 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
   }
   q.nseq = static_cast<uint32_t>(qslot.size());
   q.npos = static_cast<uint32_t>(npos);
+  q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
   q.qbase.alloc(win_slot[nwin]);
   for (int w = 0; w < nwin; ++w)
     if (wins[w].n_slots)
@@ -511,8 +512,16 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
     thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
     om = ws.get<int32_t>("d.m", nhit); osc = ws.get<int32_t>("d.s", nhit);
+    // length buckets: when query lengths differ, process hits longest first
+    uint32_t* dporder = nullptr;
+    if (q.min_qlen != q.max_qlen && nhit > 1) {
+      uint64_t* dk = ws.get<uint64_t>("d.key", nhit);
+      dporder = ws.get<uint32_t>("d.order", nhit);
+      k_dp_rows_key<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, hs, q.qlen.ptr, q.max_qlen, dk, dporder);
+      radix_sort_pairs(rt, dk, dporder, nhit, bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
+    }
     DPArgs a{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-             ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, p.lmer, ot, oq, thl, qhl, om, osc};
+             ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, p.lmer, ot, oq, thl, qhl, om, osc};
     mark(EV_DP0);
     if (p.local) launch_dp_local(p.gap, a, st);
     else launch_dp_global(p.gap, a, st);

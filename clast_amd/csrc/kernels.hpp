@@ -419,6 +419,7 @@ struct DPArgs {
   const int32_t* tlen;
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
+  const uint32_t* order;   // optional: hit processed by thread i (length buckets)
   int lmer;
   int32_t* o_tidx;
   int32_t* o_qidx;
@@ -510,8 +511,9 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const uint32_t i = a.order ? a.order[t] : t;
   const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
   const int L = a.lmer;
   const int qLen = a.qlen[s];
@@ -585,6 +587,18 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
 // translation unit (dp_global.hip, dp_local.hip).
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
+
+// Length-bucket key of a hit: its DP row count (both directions together
+// always total qLen - lMer) complemented, so a stable ascending sort puts the
+// longest hits first.
+static __global__ __launch_bounds__(kBlock) void k_dp_rows_key(uint32_t n, const int32_t* __restrict__ hs,
+                                                               const int32_t* __restrict__ qlen, int maxq,
+                                                               uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  key[i] = static_cast<uint64_t>(maxq - qlen[hs[i]]);
+  idx[i] = i;
+}
 
 // ---------------------------------------------------------------- results
 

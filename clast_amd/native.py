@@ -278,6 +278,25 @@ class Query:
         lst.release_windows()
         self.h = h
 
+    @classmethod
+    def merged(cls, params: Params, lists_and_windows) -> "Query":
+        """One query set over the pair windows of several host lists (e.g.
+        several requests), numbered in the given (list, window) order."""
+        self = cls.__new__(cls)
+        self.wins = []
+        ws = []
+        for lst, wins in lists_and_windows:
+            ws.extend(lst.window(2 * a, 2 * b) for a, b in wins)
+            self.wins.extend(wins)
+        arr = (CWindow * len(ws))(*ws)
+        h = C.c_void_p()
+        _check(hip().clast_query_create_multi(C.byref(params.c()), arr, len(ws), C.byref(h)),
+               "clast_query_create_multi")
+        for lst, _ in lists_and_windows:
+            lst.release_windows()
+        self.h = h
+        return self
+
     def close(self) -> None:
         if self.h:
             hip().clast_query_destroy(self.h)

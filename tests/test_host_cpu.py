@@ -123,3 +123,19 @@ def test_driver_prints_settings_before_device(tmp_path):
     assert "allowable seed-seed width: 50" in r.stdout
     assert "RAM usage for target (MB): 1.5" in r.stdout
     assert "No.2 b.fa" in r.stdout
+
+
+def test_poisson_requests_partition_reads():
+    from clast_amd import serve
+
+    rng = np.random.default_rng(0)
+    lens = rng.integers(50, 400, 5000)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    bases = rng.choice(np.frombuffer(b"ACGT", np.uint8), int(off[-1]))
+    reqs = serve.poisson_requests(bases, off, rate=50.0, seed=1, max_reads=400)
+    assert sum(len(r.offsets) - 1 for r in reqs) == 5000
+    assert b"".join(r.bases.tobytes() for r in reqs) == bases.tobytes()
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    # mean inter-arrival close to 1/rate
+    assert 0.01 < float(np.mean(np.diff(arr))) < 0.04
