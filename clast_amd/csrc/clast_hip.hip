@@ -101,10 +101,12 @@ struct clast_index_s {
   DevArray<uint64_t> win_slot;
   DevArray<uint64_t> seq_slot;
   DevArray<int32_t> seq_len, seq_win, seq_ref;
+  uint32_t ncell = 0;
   DevArray<uint32_t> ekey32;
   DevArray<uint64_t> ekey64;
-  DevArray<int32_t> egs, etidx;
-  DevArray<uint32_t> bucket;
+  DevArray<int2> epos;
+  DevArray<unsigned long long> bitmap;
+  DevArray<uint32_t> rank, cell_off;
   std::vector<int32_t> h_seq_len, h_seq_win, h_seq_ref, h_id_begin;
   std::vector<uint64_t> h_win_slot;
 };
@@ -155,6 +157,51 @@ int guard(F&& f) {
 void set_device(int dev) { hip_check(hipSetDevice(dev), "hipSetDevice"); }
 
 // ---------------------------------------------------------------- index
+
+// Presence bitmap, rank directory and cell offsets over the top B key bits
+// of the sorted entries (keys in ek, u64); keeps the keys as u32 or u64.
+void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t nent, int keybits) {
+  hipStream_t st = rt.stream();
+  Workspace& ws = rt.ws();
+  const int B = std::min(keybits, 30);
+  ix.pbits = B;
+  ix.shift = keybits - B;
+  const uint64_t nwords = std::max<uint64_t>(1, (1ull << B) / 64);
+  const uint64_t nblk = (nwords + 7) / 8;
+  ix.bitmap.alloc(nwords);
+  k_fill<unsigned long long><<<grid_for(nwords, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, 0ull);
+  uint32_t ncell = 0;
+  if (nent > 0) {
+    uint32_t* head = ws.get<uint32_t>("ix.chead", nent);
+    uint32_t* cell = ws.get<uint32_t>("ix.cell", nent);
+    uint32_t* scal = ws.get<uint32_t>("ix.cscal", 2);
+    k_cell_heads<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head);
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, head, nent, cell, scal, false);
+    rt.read_scalars(scal, &ncell, 1);
+    ix.cell_off.alloc(static_cast<uint64_t>(ncell) + 1);
+    k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, ix.cell_off.ptr,
+                                                             ix.bitmap.ptr);
+    rt.check_launch("k_cell_build");
+    if (ix.key64) {
+      ix.ekey64.alloc(nent);
+      hip_check(hipMemcpyAsync(ix.ekey64.ptr, ek, sizeof(uint64_t) * nent, hipMemcpyDeviceToDevice, st), "keys");
+    } else {
+      ix.ekey32.alloc(nent);
+      k_narrow_keys<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.ekey32.ptr);
+    }
+  } else {
+    ix.cell_off.alloc(1);
+    k_fill<uint32_t><<<1, 64, 0, st>>>(ix.cell_off.ptr, 1, 0u);
+    ix.ekey32.alloc(1);
+    ix.ekey64.alloc(1);
+  }
+  ix.ncell = ncell;
+  uint32_t* bc = ws.get<uint32_t>("ix.bcnt", nblk);
+  ix.rank.alloc(nblk);
+  k_rank_count<<<grid_for(nblk, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, nblk, bc);
+  device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, bc, nblk, ix.rank.ptr, static_cast<uint32_t*>(nullptr), false);
+  rt.check_launch("rank");
+}
 
 void build_index(const clast_params& p, const clast_window* wins, int nwin, clast_index_s& ix) {
   Runtime& rt = runtime();
@@ -238,37 +285,16 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
       uint32_t* eoff = ws.get<uint32_t>("ix.eoff", nvalid);
       device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, keep, nvalid, eoff, scal + 2, false);
       rt.read_scalars(scal + 2, &nent, 1);
-      ix.egs.alloc(nent);
-      ix.etidx.alloc(nent);
-      if (ix.key64) {
-        ix.ekey64.alloc(nent);
-        k_index_scatter<uint64_t><<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(keys, keep, eoff, tgs, tti, nvalid,
-                                                                               ix.ekey64.ptr, ix.egs.ptr, ix.etidx.ptr);
-      } else {
-        ix.ekey32.alloc(nent);
-        k_index_scatter<uint32_t><<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(keys, keep, eoff, tgs, tti, nvalid,
-                                                                               ix.ekey32.ptr, ix.egs.ptr, ix.etidx.ptr);
-      }
+      ix.epos.alloc(nent);
+      uint64_t* ek = ws.get<uint64_t>("ix.ekey", nent);
+      k_index_scatter<<<grid_for(nvalid, kBlock), kBlock, 0, st>>>(keys, keep, eoff, tgs, tti, nvalid, ek,
+                                                                    ix.epos.ptr);
       rt.check_launch("k_index_scatter");
+      build_cells(rt, ix, ek, nent, keybits);
     }
   }
   ix.nentries = nent;
-  // bucket table over the top P key bits: about two buckets per entry
-  int pb = bits_for(static_cast<uint64_t>(nent) + 1) + 1;
-  pb = std::max(pb, 8);
-  pb = std::min(pb, std::min(keybits, 30));
-  ix.pbits = pb;
-  ix.shift = keybits - pb;
-  const uint64_t nb = (1ull << pb) + 1;
-  ix.bucket.alloc(nb);
-  k_fill<uint32_t><<<grid_for(nb, kBlock), kBlock, 0, st>>>(ix.bucket.ptr, nb, nent);
-  if (nent > 0) {
-    if (ix.key64)
-      k_bucket_heads<uint64_t><<<grid_for(nent, kBlock), kBlock, 0, st>>>(ix.ekey64.ptr, nent, ix.shift, ix.bucket.ptr);
-    else
-      k_bucket_heads<uint32_t><<<grid_for(nent, kBlock), kBlock, 0, st>>>(ix.ekey32.ptr, nent, ix.shift, ix.bucket.ptr);
-  }
-  rt.check_launch("k_bucket_heads");
+  if (nent == 0) build_cells(rt, ix, nullptr, 0, keybits);
   rt.sync();
 }
 
@@ -363,6 +389,14 @@ struct StageEvents {
 
 // The CDeviceHitList chain (CDeviceHitList.cu:228-423) for every unit of
 // idx x q.  final_order: apply the fixResult order across units.
+//
+// Seeds of a query strand are contiguous after the count scan, so the sort
+// by (qID, tID, diagonal, qIdx) and the isolate / duplicate filters run per
+// strand in LDS (k_seed_segments, one wavefront per strand).  Strands with
+// more seeds than fit take the global path: radix sort of (strand, target
+// sequence, diagonal) keys and scan-based filters.  CLAST_KEEP_STAGES sends
+// every strand through the global path so its intermediate arrays can be
+// read back.
 void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int flags, bool final_order,
                 clast_hits_s& out) {
   if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
@@ -377,134 +411,174 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const bool use_cut = p.cut_off != -1;
   if (use_cut) ensure_smin(rt, p, q);
   StageEvents ev;
-  auto rec = [&](int k) { hip_check(hipEventRecord(ev.e[k], st), "event"); };
   bool timed[kEvents] = {};
-  auto mark = [&](int k) { rec(k); timed[k] = true; };
+  auto mark = [&](int k) {
+    hip_check(hipEventRecord(ev.e[k], st), "event");
+    timed[k] = true;
+  };
   mark(EV_START);
 
   const uint32_t np = q.npos;
   const int sbits = bits_for(q.nseq), gbits = bits_for(static_cast<uint64_t>(ix.nseq));
   const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 1);
-  if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
-  SeedPack pk{gbits, dbits, q.max_qlen};
-  SeedDecode dc{gbits, dbits, q.max_qlen};
-
-  uint32_t nseed = 0;
-  uint64_t* skey = nullptr;
-  uint32_t* sval = nullptr;
+  const int d_off = q.max_qlen;
+  const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
   uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 2);
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
-  if (np > 0 && ix.nentries > 0) {
-    uint32_t* m0 = ws.get<uint32_t>("s.m0", np);
-    uint32_t* rawc = ws.get<uint32_t>("s.raw", np);
-    uint32_t* cnt = ws.get<uint32_t>("s.cnt", np);
-    uint32_t* off = ws.get<uint32_t>("s.off", np);
-    auto run_seed = [&](auto* ekey) {
-      using KeyT = std::remove_pointer_t<decltype(ekey)>;
-      SeedArgs<KeyT> a{q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, q.pos_off.ptr, q.nseq, np,
-                       ekey,        ix.egs.ptr,  ix.etidx.ptr, ix.bucket.ptr, ix.shift, ix.seq_len.ptr,
-                       p.lmer,      p.gap,       p.local};
-      mark(EV_COUNT0);
-      k_seed_count<KeyT><<<grid_for(np, kBlock), kBlock, 0, st>>>(a, m0, rawc, cnt);
-      mark(EV_COUNT1);
-      rt.check_launch("k_seed_count");
-      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt, np, off, tot64, false);
-      uint64_t total = 0;
-      rt.read_scalars(tot64, &total, 1);
-      if (total >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
-      nseed = static_cast<uint32_t>(total);
-      if (nseed) {
-        skey = ws.get<uint64_t>("s.key", nseed);
-        sval = ws.get<uint32_t>("s.val", nseed);
-        mark(EV_WRITE0);
-        k_seed_write<KeyT><<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, m0, rawc, off, skey, sval);
-        mark(EV_WRITE1);
-        rt.check_launch("k_seed_write");
-      }
-    };
-    if (ix.key64) run_seed(ix.ekey64.ptr);
-    else run_seed(ix.ekey32.ptr);
-  }
-  auto dump_seeds = [&](int slot) {
-    std::vector<uint64_t> hk;
-    std::vector<uint32_t> hv;
-    d2h(hk, skey, nseed, st);
-    d2h(hv, sval, nseed, st);
-    rt.sync();
-    auto& S = out.stage[slot];
-    for (int c = 0; c < 4; ++c) S[c].resize(nseed);
-    for (uint32_t i = 0; i < nseed; ++i) {
-      const uint64_t k = hk[i];
-      const uint32_t s = static_cast<uint32_t>(k >> (gbits + dbits));
-      const uint32_t gs = static_cast<uint32_t>((k >> dbits) & ((1ull << gbits) - 1ull));
-      const int d = static_cast<int>(k & ((1ull << dbits) - 1ull)) - q.max_qlen;
-      S[0][i] = ix.h_seq_ref[gs];
-      S[1][i] = d + static_cast<int>(hv[i]);
-      S[2][i] = q.h_qref[s];
-      S[3][i] = static_cast<int>(hv[i]);
-    }
-    out.stage_n[slot] = nseed;
-  };
-  if (keep_stages) dump_seeds(CLAST_STAGE_SEEDS);
-  if (nseed > 1) {
-    // workspace slots of the sort grow before timing so no sync lands inside
-    (void)ws.get<uint64_t>("sort.k2", nseed);
-    (void)ws.get<uint32_t>("sort.v2", nseed);
-    (void)ws.get<uint32_t>("sort.counts", static_cast<uint64_t>(grid_for(nseed, kSortTile)) * kRadixBins);
-    (void)ws.get<uint32_t>("scan.tiles", static_cast<uint64_t>(grid_for(nseed, kSortTile)) * kRadixBins / kScanTile + 1);
-    mark(EV_SORT0);
-    radix_sort_pairs(rt, skey, sval, nseed, sbits + gbits + dbits);
-    mark(EV_SORT1);
-  }
-  if (keep_stages) dump_seeds(CLAST_STAGE_SORTED);
-  mark(EV_FILT0);
+  uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
+  k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
 
-  // isolate + duplicate filters -> hits
+  SeedArgs a{};
+  a.qbase = q.qbase.ptr;
+  a.qslot = q.qslot.ptr;
+  a.qlen = q.qlen.ptr;
+  a.qwin = q.qwin.ptr;
+  a.pos_off = q.pos_off.ptr;
+  a.nseq = q.nseq;
+  a.npos = np;
+  a.lk = Lookup{ix.bitmap.ptr, ix.rank.ptr, ix.cell_off.ptr,
+                ix.key64 ? static_cast<const void*>(ix.ekey64.ptr) : static_cast<const void*>(ix.ekey32.ptr),
+                ix.shift, ix.key64 ? 1 : 0};
+  a.epos = ix.epos.ptr;
+  a.tlen = ix.seq_len.ptr;
+  a.twin = ix.seq_win.ptr;
+  a.nqwin = q.nwin;
+  a.lmer = p.lmer;
+  a.gap = p.gap;
+  a.width = p.width;
+  a.local = p.local;
+  a.unit_first = unit_first;
+
+  // lookup + count
+  uint32_t nseed = 0;
+  uint32_t *m0 = nullptr, *rawc = nullptr, *cnt = nullptr, *off = nullptr;
+  if (np > 0 && ix.nentries > 0) {
+    m0 = ws.get<uint32_t>("s.m0", np);
+    rawc = ws.get<uint32_t>("s.raw", np);
+    cnt = ws.get<uint32_t>("s.cnt", np);
+    off = ws.get<uint32_t>("s.off", np);
+    mark(EV_COUNT0);
+    k_seed_count<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, m0, rawc, cnt);
+    mark(EV_COUNT1);
+    rt.check_launch("k_seed_count");
+    device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt, np, off, tot64, false);
+    uint64_t total = 0;
+    rt.read_scalars(tot64, &total, 1);
+    if (total >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
+    nseed = static_cast<uint32_t>(total);
+  }
+
+  // per-strand sort + isolate/duplicate, global path for the rest
   uint32_t nhit = 0;
-  int32_t *hs = nullptr, *hgs = nullptr, *ht = nullptr, *hq = nullptr;
+  const uint64_t cap = std::max<uint64_t>(nseed, 1);
+  int32_t* hs = ws.get<int32_t>("h.s", cap);
+  int32_t* hgs = ws.get<int32_t>("h.gs", cap);
+  int32_t* ht = ws.get<int32_t>("h.t", cap);
+  int32_t* hq = ws.get<int32_t>("h.q", cap);
   if (nseed > 0) {
-    const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
-    uint32_t* unit_first = ws.get<uint32_t>("f.unit", nunits);
-    k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
-    uint8_t* nearf = ws.get<uint8_t>("f.near", nseed);
-    uint32_t* gidx = ws.get<uint32_t>("f.gidx", nseed);
-    uint32_t* kidx = ws.get<uint32_t>("f.kidx", nseed);
-    uint32_t* lastk = ws.get<uint32_t>("f.lastk", nseed);
-    uint32_t* gstart = ws.get<uint32_t>("f.gstart", nseed);
-    uint32_t* dflag = ws.get<uint32_t>("f.dflag", nseed);
-    uint32_t* hoff = ws.get<uint32_t>("f.hoff", nseed);
-    const uint32_t g = grid_for(nseed, kBlock);
-    k_seed_marks<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, p.width, p.gap, ix.seq_win.ptr, q.qwin.ptr, q.nwin,
-                                       nearf, gidx, unit_first);
-    k_seed_keep<<<g, kBlock, 0, st>>>(skey, nseed, dc, ix.seq_win.ptr, q.qwin.ptr, q.nwin, nearf, unit_first, kidx);
-    rt.check_launch("k_seed_marks/keep");
-    device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, kidx, nseed, lastk, static_cast<uint32_t*>(nullptr), true);
-    device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, gidx, nseed, gstart, static_cast<uint32_t*>(nullptr), true);
-    k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
-    rt.check_launch("k_seed_dup");
-    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nseed, hoff, tot32, false);
-    rt.read_scalars(tot32, &nhit, 1);
-    if (nhit) {
-      hs = ws.get<int32_t>("h.s", nhit);
-      hgs = ws.get<int32_t>("h.gs", nhit);
-      ht = ws.get<int32_t>("h.t", nhit);
-      hq = ws.get<int32_t>("h.q", nhit);
-      k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, nseed, dc, dflag, hoff, hs, hgs, ht, hq);
-      rt.check_launch("k_hits_scatter");
+    uint32_t* ovf_list = ws.get<uint32_t>("s.ovf", q.nseq);
+    uint8_t* ovf_flag = ws.get<uint8_t>("s.ovfflag", q.nseq);
+    uint32_t novf = 0;
+    if (!keep_stages) {
+      k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
+      k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
+      const uint32_t grid = std::min<uint32_t>(grid_for(q.nseq, kSegWaves), 1u << 20);
+      mark(EV_WRITE0);
+      k_seed_segments<<<grid, kBlock, 0, st>>>(a, m0, rawc, off, nseed, d_off,
+                                                SegOut{hs, hgs, ht, hq, tot32, ovf_list, tot32 + 1, ovf_flag});
+      mark(EV_WRITE1);
+      rt.check_launch("k_seed_segments");
+      uint32_t c2[2];
+      rt.read_scalars(tot32, c2, 2);
+      nhit = c2[0];
+      novf = c2[1];
+    } else {
+      k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 1);
+      novf = q.nseq;
+    }
+    if (novf > 0) {
+      if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
+      SeedPack pk{gbits, dbits, d_off};
+      SeedDecode dc{gbits, dbits, d_off};
+      uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", np);
+      uint32_t* off2 = ws.get<uint32_t>("g.off", np);
+      k_ovf_counts<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, ovf_flag, cnt, cnt2);
+      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, np, off2, tot64 + 1, false);
+      uint64_t t2 = 0;
+      rt.read_scalars(tot64 + 1, &t2, 1);
+      const uint32_t ng = static_cast<uint32_t>(t2);
+      uint64_t* skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
+      uint32_t* sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
+      k_seed_write<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, ovf_flag, m0, rawc, off2, skey, sval);
+      rt.check_launch("k_seed_write");
+      auto dump_seeds = [&](int slot) {
+        std::vector<uint64_t> hk;
+        std::vector<uint32_t> hv;
+        d2h(hk, skey, ng, st);
+        d2h(hv, sval, ng, st);
+        rt.sync();
+        auto& S = out.stage[slot];
+        for (int c = 0; c < 4; ++c) S[c].resize(ng);
+        for (uint32_t i = 0; i < ng; ++i) {
+          const uint64_t k = hk[i];
+          const uint32_t s = static_cast<uint32_t>(k >> (gbits + dbits));
+          const uint32_t gs = static_cast<uint32_t>((k >> dbits) & ((1ull << gbits) - 1ull));
+          const int d = static_cast<int>(k & ((1ull << dbits) - 1ull)) - d_off;
+          S[0][i] = ix.h_seq_ref[gs];
+          S[1][i] = d + static_cast<int>(hv[i]);
+          S[2][i] = q.h_qref[s];
+          S[3][i] = static_cast<int>(hv[i]);
+        }
+        out.stage_n[slot] = ng;
+      };
+      if (keep_stages) dump_seeds(CLAST_STAGE_SEEDS);
+      if (ng > 1) {
+        // the sort's slots grow before timing so no sync lands inside
+        (void)ws.get<uint64_t>("sort.k2", ng);
+        (void)ws.get<uint32_t>("sort.v2", ng);
+        (void)ws.get<uint32_t>("sort.counts", static_cast<uint64_t>(grid_for(ng, kSortTile)) * kRadixBins);
+        mark(EV_SORT0);
+        radix_sort_pairs(rt, skey, sval, ng, sbits + gbits + dbits);
+        mark(EV_SORT1);
+      }
+      if (keep_stages) dump_seeds(CLAST_STAGE_SORTED);
+      if (ng > 0) {
+        mark(EV_FILT0);
+        uint32_t* kidx = ws.get<uint32_t>("g.kidx", ng);
+        uint32_t* gidx = ws.get<uint32_t>("g.gidx", ng);
+        uint32_t* lastk = ws.get<uint32_t>("g.lastk", ng);
+        uint32_t* gstart = ws.get<uint32_t>("g.gstart", ng);
+        uint32_t* dflag = ws.get<uint32_t>("g.dflag", ng);
+        uint32_t* hoff = ws.get<uint32_t>("g.hoff", ng);
+        const uint32_t g = grid_for(ng, kBlock);
+        k_seed_marks<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, p.width, p.gap, ix.seq_win.ptr, q.qwin.ptr, q.nwin,
+                                           unit_first, kidx, gidx);
+        rt.check_launch("k_seed_marks");
+        device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, kidx, ng, lastk, static_cast<uint32_t*>(nullptr), true);
+        device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, gidx, ng, gstart, static_cast<uint32_t*>(nullptr), true);
+        k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
+        rt.check_launch("k_seed_dup");
+        device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, ng, hoff, tot32 + 2, false);
+        uint32_t nh2 = 0;
+        rt.read_scalars(tot32 + 2, &nh2, 1);
+        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, nhit, hs, hgs, ht, hq);
+        rt.check_launch("k_hits_scatter");
+        nhit += nh2;
+        mark(EV_FILT1);
+      }
     }
   }
   if (keep_stages) {
-    std::vector<int32_t> a, b, c, d;
-    d2h(a, hs, nhit, st); d2h(b, hgs, nhit, st); d2h(c, ht, nhit, st); d2h(d, hq, nhit, st);
+    std::vector<int32_t> va, vb, vc, vd;
+    d2h(va, hs, nhit, st); d2h(vb, hgs, nhit, st); d2h(vc, ht, nhit, st); d2h(vd, hq, nhit, st);
     rt.sync();
     auto& S = out.stage[CLAST_STAGE_HITS];
     for (int k = 0; k < 4; ++k) S[k].resize(nhit);
     for (uint32_t i = 0; i < nhit; ++i) {
-      S[0][i] = ix.h_seq_ref[b[i]]; S[1][i] = c[i]; S[2][i] = q.h_qref[a[i]]; S[3][i] = d[i];
+      S[0][i] = ix.h_seq_ref[vb[i]]; S[1][i] = vc[i]; S[2][i] = q.h_qref[va[i]]; S[3][i] = vd[i];
     }
     out.stage_n[CLAST_STAGE_HITS] = nhit;
   }
-  mark(EV_FILT1);
 
   // banded DP
   int32_t *ot = nullptr, *oq = nullptr, *thl = nullptr, *qhl = nullptr, *om = nullptr, *osc = nullptr;
@@ -520,11 +594,11 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       k_dp_rows_key<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, hs, q.qlen.ptr, q.max_qlen, dk, dporder);
       radix_sort_pairs(rt, dk, dporder, nhit, bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
     }
-    DPArgs a{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-             ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, p.lmer, ot, oq, thl, qhl, om, osc};
+    DPArgs da{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
+              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, p.lmer, ot, oq, thl, qhl, om, osc};
     mark(EV_DP0);
-    if (p.local) launch_dp_local(p.gap, a, st);
-    else launch_dp_global(p.gap, a, st);
+    if (p.local) launch_dp_local(p.gap, da, st);
+    else launch_dp_global(p.gap, da, st);
     mark(EV_DP1);
     rt.check_launch("k_dp");
   }
@@ -542,9 +616,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     }
     out.stage_n[CLAST_STAGE_ALIGNED] = nhit;
   }
-  mark(EV_RES0);
 
   // E-value, filters, result order, dedup
+  mark(EV_RES0);
   uint32_t nkeep = 0, nfinal = 0;
   uint32_t* order = nullptr;
   uint32_t* permB = nullptr;
@@ -560,10 +634,10 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, osc, qhl, q.qlen.ptr, q.smin.ptr, use_cut ? 1 : 0,
                                        p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev, kflag, srange);
     rt.check_launch("k_hit_filter");
-    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, kflag, nhit, koff, tot32 + 1, false);
-    hip_check(hipMemcpyAsync(tot32 + 2, srange, 8, hipMemcpyDeviceToDevice, st), "copy");
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, kflag, nhit, koff, tot32 + 3, false);
+    hip_check(hipMemcpyAsync(tot32 + 4, srange, 8, hipMemcpyDeviceToDevice, st), "copy");
     uint32_t sc3[3];
-    rt.read_scalars(tot32 + 1, sc3, 3);
+    rt.read_scalars(tot32 + 3, sc3, 3);
     nkeep = sc3[0];
     const int32_t smin_v = static_cast<int32_t>(sc3[1]), smax_v = static_cast<int32_t>(sc3[2]);
     if (nkeep) {
@@ -571,23 +645,24 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       const int pbits = bits_for((q.nseq + 1) / 2);
       const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
       const int qb = bits_for(static_cast<uint64_t>(q.max_qlen) + 1);
-      if (wbits + pbits + scb > 64) throw ClastError(CLAST_ERR_LIMIT, "order key over 64 bits");
+      if (wbits + pbits + scb > 64 || gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "order key over 64 bits");
       uint64_t* w0 = ws.get<uint64_t>("r.w0", nkeep);
       uint64_t* w1 = ws.get<uint64_t>("r.w1", nkeep);
+      uint64_t* w2 = ws.get<uint64_t>("r.w2", nkeep);
       uint32_t* rec = ws.get<uint32_t>("r.rec", nkeep);
       uint32_t* perm = ws.get<uint32_t>("r.perm", nkeep);
-      k_order_keys<<<g, kBlock, 0, st>>>(nhit, kflag, koff, hs, hgs, hq, osc, ix.seq_win.ptr, q.max_qlen, pbits, scb,
-                                         smax_v, w0, w1, rec);
+      k_order_keys<<<g, kBlock, 0, st>>>(nhit, kflag, koff, hs, hgs, ht, hq, osc, ix.seq_win.ptr, dbits, d_off,
+                                         q.max_qlen, pbits, scb, smax_v, w0, w1, w2, rec);
       rt.check_launch("k_order_keys");
-      const uint64_t* words[2] = {w0, w1};
-      const int wb[2] = {qb + 1, wbits + pbits + scb};
-      sort_perm(rt, nkeep, 2, words, wb, perm);
+      const uint64_t* words[3] = {w0, w1, w2};
+      const int wb[3] = {gbits + dbits, qb + 1, wbits + pbits + scb};
+      sort_perm(rt, nkeep, 3, words, wb, perm);
       uint32_t* dflag = ws.get<uint32_t>("r.dflag", nkeep);
       uint32_t* doff = ws.get<uint32_t>("r.doff", nkeep);
       const uint32_t gk = grid_for(nkeep, kBlock);
       k_dedup<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, hs, hgs, ot, oq, ix.seq_win.ptr, q.qwin.ptr, dflag);
-      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nkeep, doff, tot32 + 5, false);
-      rt.read_scalars(tot32 + 5, &nfinal, 1);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nkeep, doff, tot32 + 6, false);
+      rt.read_scalars(tot32 + 6, &nfinal, 1);
       order = ws.get<uint32_t>("r.order", nfinal);
       k_compact_perm<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, dflag, doff, order);
       rt.check_launch("k_dedup/compact");
@@ -627,7 +702,6 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   }
   out.times[7] = static_cast<float>(nseed);
 }
-
 }  // namespace
 
 // ---------------------------------------------------------------- C ABI
@@ -671,9 +745,8 @@ int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* 
     Runtime& rt = runtime();
     std::lock_guard<std::mutex> lk(rt.mutex());
     const uint64_t n = ix->nentries;
-    std::vector<int32_t> gs, ti;
-    d2h(gs, ix->egs.ptr, n, rt.stream());
-    d2h(ti, ix->etidx.ptr, n, rt.stream());
+    std::vector<int2> ep;
+    d2h(ep, ix->epos.ptr, n, rt.stream());
     std::vector<int64_t> k(n);
     if (ix->key64) {
       std::vector<uint64_t> kk;
@@ -688,9 +761,9 @@ int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* 
     }
     for (uint64_t i = 0; i < n; ++i) {
       if (key) key[i] = k[i];
-      if (window) window[i] = ix->h_seq_win[gs[i]];
-      if (tID) tID[i] = ix->h_seq_ref[gs[i]];
-      if (tIdx) tIdx[i] = ti[i];
+      if (window) window[i] = ix->h_seq_win[ep[i].x];
+      if (tID) tID[i] = ix->h_seq_ref[ep[i].x];
+      if (tIdx) tIdx[i] = ep[i].y;
     }
   });
 }

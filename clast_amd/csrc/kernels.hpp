@@ -161,50 +161,116 @@ static __global__ __launch_bounds__(kBlock) void k_index_keep(const uint32_t* __
   tmp_tidx[j] = static_cast<int32_t>(tidx);
 }
 
-template <class KeyT>
+// Kept entries in (key, window, position) order: full key (u64, for the
+// cell build; narrowed to u32 for lMer <= 16 afterwards) and (seq, tIdx).
 static __global__ __launch_bounds__(kBlock) void k_index_scatter(const uint64_t* __restrict__ keys,
-                                                          const uint32_t* __restrict__ keep,
-                                                          const uint32_t* __restrict__ off,
-                                                          const int32_t* __restrict__ tmp_gs,
-                                                          const int32_t* __restrict__ tmp_tidx, uint32_t nvalid,
-                                                          KeyT* __restrict__ ekey, int32_t* __restrict__ egs,
-                                                          int32_t* __restrict__ etidx) {
+                                                                 const uint32_t* __restrict__ keep,
+                                                                 const uint32_t* __restrict__ off,
+                                                                 const int32_t* __restrict__ tmp_gs,
+                                                                 const int32_t* __restrict__ tmp_tidx, uint32_t nvalid,
+                                                                 uint64_t* __restrict__ ekey, int2* __restrict__ epos) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= nvalid || !keep[j]) return;
   const uint32_t o = off[j];
-  ekey[o] = static_cast<KeyT>(keys[j]);
-  egs[o] = tmp_gs[j];
-  etidx[o] = tmp_tidx[j];
+  ekey[o] = keys[j];
+  epos[o] = make_int2(tmp_gs[j], tmp_tidx[j]);
 }
 
-// bucket[h] = first entry whose key prefix (key >> shift) is >= h; the
-// caller pre-fills every bucket with n (entries past the last prefix).
-template <class KeyT>
-static __global__ __launch_bounds__(kBlock) void k_bucket_heads(const KeyT* __restrict__ ekey, uint32_t n, int shift,
-                                                         uint32_t* __restrict__ bucket) {
+static __global__ __launch_bounds__(kBlock) void k_narrow_keys(const uint64_t* __restrict__ in, uint32_t n,
+                                                               uint32_t* __restrict__ out) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j < n) out[j] = static_cast<uint32_t>(in[j]);
+}
+
+// Presence bitmap over the top B key bits (B = min(2 lMer, 30)): bit h is set
+// when some entry's key prefix is h.  cell_off[r] is the first entry of the
+// r-th present prefix, rank[b] the number of set bits before 512-bit block b.
+// A query k-mer whose bit is clear costs one bitmap line, which stays in the
+// MI355X Infinity Cache (128 MiB at B = 30).
+static __global__ __launch_bounds__(kBlock) void k_cell_heads(const uint64_t* __restrict__ keys, uint32_t n,
+                                                              int shift, uint32_t* __restrict__ head) {
   const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= n) return;
-  const uint64_t h = static_cast<uint64_t>(ekey[e]) >> shift;
-  const int64_t hp = e == 0 ? -1 : static_cast<int64_t>(static_cast<uint64_t>(ekey[e - 1]) >> shift);
-  for (int64_t x = hp + 1; x <= static_cast<int64_t>(h); ++x) bucket[x] = e;
+  head[e] = (e == 0 || (keys[e] >> shift) != (keys[e - 1] >> shift)) ? 1u : 0u;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_cell_build(const uint64_t* __restrict__ keys, uint32_t n,
+                                                              int shift, const uint32_t* __restrict__ head,
+                                                              const uint32_t* __restrict__ cell,
+                                                              uint32_t* __restrict__ cell_off,
+                                                              unsigned long long* __restrict__ bitmap) {
+  const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  if (head[e]) {
+    const uint64_t h = keys[e] >> shift;
+    cell_off[cell[e]] = e;
+    atomicOr(&bitmap[h >> 6], 1ull << (h & 63));
+  }
+  if (e + 1 == n) cell_off[cell[e] + head[e]] = n;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_rank_count(const unsigned long long* __restrict__ bitmap,
+                                                              uint64_t nwords, uint64_t nblk,
+                                                              uint32_t* __restrict__ cnt) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (b >= nblk) return;
+  uint32_t c = 0;
+  for (uint64_t w = b * 8; w < b * 8 + 8 && w < nwords; ++w) c += __popcll(bitmap[w]);
+  cnt[b] = c;
+}
+
+struct Lookup {
+  const unsigned long long* bitmap;
+  const uint32_t* rank;
+  const uint32_t* cell_off;
+  const void* ekey;   // u32 or u64 keys; read only when the prefix is not the whole key
+  int shift;          // 2 lMer - B
+  int key64;
+};
+
+// Entry range [lo, hi) of a valid query k-mer (krnlBinarySearch.cu:3-33 +
+// the cell gathers of …createRawSeedList.cu:63-135): empty when absent.
+__device__ __forceinline__ void lookup(const Lookup& L, uint64_t key, uint32_t& lo, uint32_t& hi) {
+  const uint64_t h = key >> L.shift;
+  const uint64_t w = h >> 6;
+  const unsigned long long word = L.bitmap[w];
+  const int bit = static_cast<int>(h & 63);
+  lo = hi = 0;
+  if (!((word >> bit) & 1ull)) return;
+  uint32_t r = L.rank[w >> 3];
+  for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
+  r += static_cast<uint32_t>(__popcll(word & ((1ull << bit) - 1ull)));
+  lo = L.cell_off[r];
+  hi = L.cell_off[r + 1];
+  if (L.shift) {  // the prefix cell holds several keys: narrow to this one
+    if (L.key64) {
+      const uint64_t* k = static_cast<const uint64_t*>(L.ekey);
+      lo = lower_bound_u(k, lo, hi, key);
+      hi = upper_bound_u(k, lo, hi, key);
+    } else {
+      const uint32_t* k = static_cast<const uint32_t*>(L.ekey);
+      lo = lower_bound_u(k, lo, hi, static_cast<uint32_t>(key));
+      hi = upper_bound_u(k, lo, hi, static_cast<uint32_t>(key));
+    }
+  }
 }
 
 // ---------------------------------------------------------------- seeding
 
-template <class KeyT>
 struct SeedArgs {
   const uint8_t* qbase;
   const uint32_t* qslot;
   const int32_t* qlen;
+  const int32_t* qwin;      // query window of each strand
   const uint32_t* pos_off;  // nseq + 1
   uint32_t nseq, npos;
-  const KeyT* ekey;
-  const int32_t* egs;
-  const int32_t* etidx;
-  const uint32_t* bucket;
-  int shift;
-  const int32_t* tlen;  // per target sequence
-  int lmer, gap, local;
+  Lookup lk;
+  const int2* epos;         // (target sequence, intra-sequence index) per entry
+  const int32_t* tlen;      // per target sequence
+  const int32_t* twin;      // target window of each target sequence
+  int nqwin;
+  int lmer, gap, width, local;
+  uint32_t* unit_first;     // per (target window, query window): first strand with a seed
 };
 
 // Strand of flat k-mer position p; the workgroup narrows the search range
@@ -220,14 +286,20 @@ __device__ __forceinline__ uint32_t strand_of(const uint32_t* pos_off, uint32_t 
   return upper_bound_u(pos_off, lds2[0], lds2[1], p) - 1;
 }
 
-// Query k-mer lookup (krnlBinarySearch.cu + …createRawSeedList.cu:31-135):
-// the matching entry range of position p, and how many survive the corner
-// filter (…deleteSeedsOnSequenceBoundary.cu:129-188, global mode only).
-// Only positions with qIdx + lMer <= qLen exist here: the query-boundary
-// filter (:19-72) removes every other one.
-template <class KeyT>
-static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs<KeyT> a, uint32_t* __restrict__ m0,
-                                                       uint32_t* __restrict__ rawcnt, uint32_t* __restrict__ cnt) {
+// corner filter (…deleteSeedsOnSequenceBoundary.cuh:31-49), global mode only
+__device__ __forceinline__ bool corner_ok(const SeedArgs& a, int d, int qlen, int tlen) {
+  return a.local || !(d + a.gap < 0 || d + qlen - a.gap > tlen);
+}
+
+// Query k-mer lookup per position: matching entry range and the number of
+// seeds surviving the corner filter.  Only positions with qIdx + lMer <= qLen
+// exist here: the query-boundary filter (…deleteSeedsOnSequenceBoundary.cu:
+// 19-72) removes every other one.  Also records, per (target window, query
+// window) unit, the first strand holding a seed — the strand of the unit's
+// element 0, which the isolate filter keeps (…deleteIsolateSeeds.cu:28-76).
+static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint32_t* __restrict__ m0,
+                                                              uint32_t* __restrict__ rawcnt,
+                                                              uint32_t* __restrict__ cnt) {
   __shared__ uint32_t lds2[2];
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
@@ -244,17 +316,19 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs<KeyT> a, 
   }
   uint32_t lo = 0, hi = 0, c = 0;
   if (ok) {
-    const uint64_t h = key >> a.shift;
-    const uint32_t b0 = a.bucket[h], b1 = a.bucket[h + 1];
-    lo = lower_bound_u(a.ekey, b0, b1, static_cast<KeyT>(key));
-    hi = upper_bound_u(a.ekey, lo, b1, static_cast<KeyT>(key));
-    if (a.local) {
-      c = hi - lo;
-    } else {
-      for (uint32_t j = lo; j < hi; ++j) {
-        const int d = a.etidx[j] - qIdx;
-        const int tl = a.tlen[a.egs[j]];
-        c += (d + a.gap < 0 || d + qLen - a.gap > tl) ? 0u : 1u;
+    lookup(a.lk, key, lo, hi);
+    const int qw = a.qwin[s];
+    int last_tw = -1;
+    for (uint32_t j = lo; j < hi; ++j) {
+      const int2 e = a.epos[j];
+      const int d = e.y - qIdx;
+      if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
+      ++c;
+      const int tw = a.twin[e.x];
+      if (tw != last_tw) {
+        last_tw = tw;
+        uint32_t* uf = &a.unit_first[tw * a.nqwin + qw];
+        if (s < *uf) atomicMin(uf, s);  // values only decrease: a stale read only costs an atomic
       }
     }
   }
@@ -263,23 +337,210 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs<KeyT> a, 
   cnt[p] = c;
 }
 
+// hasNearPair core (…deleteDuplicateSeeds.cuh:29-38) for seeds of one
+// (qID, tID) group; (pt, pd) is the predecessor.
+__device__ __forceinline__ bool near_pair(int t, int d, int pt, int pd, int width, int gap) {
+  return width >= t - pt && width >= pt - t && gap >= pd - d;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class T>
+__device__ __forceinline__ T wave_max_scan_incl(T v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T o = __shfl_up(v, d, 64);
+    if (l >= d) v = o > v ? o : v;
+  }
+  return v;
+}
+
+constexpr int kSegCap = 1024;          // seeds of one strand sorted in LDS
+constexpr int kSegWaves = kBlock / 64;
+
+struct SegOut {
+  int32_t* hs;
+  int32_t* hgs;
+  int32_t* ht;
+  int32_t* hq;
+  uint32_t* nhit;         // atomic hit counter
+  uint32_t* ovf;          // strands with more than kSegCap seeds
+  uint32_t* novf;
+  uint8_t* ovf_flag;      // per strand
+};
+
+// One wavefront per query strand: its seeds (positions in qIdx order, cells
+// in index order) go to LDS, are bitonic-sorted by (target sequence,
+// diagonal, qIdx) — the …sortSeeds.cuh:14-55 order restricted to one qID —
+// and the isolate and duplicate filters run on the sorted list with wave
+// scans: K(i) = near(i) or i is its unit's element 0; D(i) = K(i) and not
+// near(i, previous K of the group).  Surviving seeds are appended as hits
+// (their order is fixed later by complete sort keys).  Strands with more
+// than kSegCap seeds are listed for the global path.
+static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint32_t* __restrict__ m0,
+                                                                 const uint32_t* __restrict__ rawcnt,
+                                                                 const uint32_t* __restrict__ off, uint32_t nseed,
+                                                                 int d_off, SegOut o) {
+  __shared__ uint64_t skey[kSegWaves][kSegCap];
+  __shared__ int32_t sq[kSegWaves][kSegCap];
+  const int w = wave_id(), lane = lane_id();
+  uint64_t* K = skey[w];
+  int32_t* Q = sq[w];
+  for (uint32_t s = blockIdx.x * kSegWaves + w; s < a.nseq; s += gridDim.x * kSegWaves) {
+    const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
+    if (p0 == p1) continue;
+    const uint32_t b0 = off[p0];
+    const uint32_t b1 = p1 < a.npos ? off[p1] : nseed;
+    const uint32_t n = b1 - b0;
+    if (n == 0) continue;
+    if (n > static_cast<uint32_t>(kSegCap)) {
+      if (lane == 0) {
+        o.ovf[atomicAdd(o.novf, 1u)] = s;
+        o.ovf_flag[s] = 1;
+      }
+      continue;
+    }
+    const int qLen = a.qlen[s];
+    // generate the strand's seeds into LDS at their scanned slots
+    for (uint32_t p = p0 + lane; p < p1; p += 64) {
+      const uint32_t rc = rawcnt[p];
+      if (!rc) continue;
+      const int qIdx = static_cast<int>(p - p0);
+      uint32_t at = off[p] - b0;
+      for (uint32_t j = m0[p]; j < m0[p] + rc; ++j) {
+        const int2 e = a.epos[j];
+        const int d = e.y - qIdx;
+        if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
+        K[at] = (static_cast<uint64_t>(e.x) << 32) | static_cast<uint32_t>(d + d_off);
+        Q[at] = qIdx;
+        ++at;
+      }
+    }
+    uint32_t np = 1;
+    while (np < n) np <<= 1;
+    for (uint32_t i = n + lane; i < np; i += 64) {
+      K[i] = ~0ull;
+      Q[i] = INT_MAX;
+    }
+    wave_lds_sync();
+    // bitonic sort of (key, qIdx) ascending
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = lane; i < np; i += 64) {
+          const uint32_t x = i ^ j;
+          if (x > i) {
+            const uint64_t ka = K[i], kb = K[x];
+            const int qa = Q[i], qb = Q[x];
+            const bool gt = ka > kb || (ka == kb && qa > qb);
+            if (gt == ((i & k) == 0)) {
+              K[i] = kb; K[x] = ka;
+              Q[i] = qb; Q[x] = qa;
+            }
+          }
+        }
+        wave_lds_sync();
+      }
+    }
+    // filters over the sorted list, 64 seeds at a time
+    const int qw = a.qwin[s];
+    int carry_k = -1, carry_g = -1;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+      const uint32_t i = c0 + lane;
+      const bool valid = i < n;
+      bool ghead = false, nr = false, kk = false;
+      int gs = 0, d = 0, t = 0, q = 0;
+      if (valid) {
+        const uint64_t key = K[i];
+        gs = static_cast<int>(key >> 32);
+        d = static_cast<int>(static_cast<uint32_t>(key)) - d_off;
+        q = Q[i];
+        t = d + q;
+        bool twhead = true;
+        ghead = true;
+        if (i > 0) {
+          const uint64_t pk = K[i - 1];
+          const int pgs = static_cast<int>(pk >> 32);
+          if (pgs == gs) {
+            ghead = false;
+            twhead = false;
+            const int pd = static_cast<int>(static_cast<uint32_t>(pk)) - d_off;
+            nr = near_pair(t, d, pd + Q[i - 1], pd, a.width, a.gap);
+          } else {
+            twhead = a.twin[pgs] != a.twin[gs];
+          }
+        }
+        const int tw = a.twin[gs];
+        const bool elem0 = twhead && a.unit_first[tw * a.nqwin + qw] == s;
+        kk = nr || elem0;
+      }
+      const int kin = wave_max_scan_incl<int>(kk ? static_cast<int>(i) : -1);
+      const int gin = wave_max_scan_incl<int>(ghead ? static_cast<int>(i) : -1);
+      int kex = __shfl_up(kin, 1, 64);
+      if (lane == 0) kex = -1;
+      const int prevk = kex > carry_k ? kex : carry_k;
+      const int gstart = gin > carry_g ? gin : carry_g;
+      bool dd = false;
+      if (valid && kk) {
+        dd = true;
+        if (prevk >= 0 && prevk >= gstart) {
+          const uint64_t pk = K[prevk];
+          const int pd = static_cast<int>(static_cast<uint32_t>(pk)) - d_off;
+          dd = !near_pair(t, d, pd + Q[prevk], pd, a.width, a.gap);
+        }
+      }
+      const unsigned long long m = __ballot(dd);
+      uint32_t base = 0;
+      if (lane == 0 && m) base = atomicAdd(o.nhit, static_cast<uint32_t>(__popcll(m)));
+      base = __shfl(base, 0, 64);
+      if (dd) {
+        const uint32_t at = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        o.hs[at] = static_cast<int32_t>(s);
+        o.hgs[at] = gs;
+        o.ht[at] = t;
+        o.hq[at] = q;
+      }
+      carry_k = __shfl(kin, 63, 64) > carry_k ? __shfl(kin, 63, 64) : carry_k;
+      carry_g = __shfl(gin, 63, 64) > carry_g ? __shfl(gin, 63, 64) : carry_g;
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------- global path
+// (strands with more than kSegCap seeds, and the stage-retaining mode)
+
 struct SeedPack {
   int gs_bits, d_bits;
   int d_off;
 };
 
-// Writes the surviving seeds of each position at its scanned offset, in
-// query-position then cell order (krnlWriteSeedList.cu:3-37 order).
-template <class KeyT>
-static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs<KeyT> a, SeedPack pk,
-                                                       const uint32_t* __restrict__ m0,
-                                                       const uint32_t* __restrict__ rawcnt,
-                                                       const uint32_t* __restrict__ off, uint64_t* __restrict__ skey,
-                                                       uint32_t* __restrict__ sval) {
+// Seed counts of the positions of listed strands only.
+static __global__ __launch_bounds__(kBlock) void k_ovf_counts(SeedArgs a, const uint8_t* __restrict__ ovf_flag,
+                                                              const uint32_t* __restrict__ cnt,
+                                                              uint32_t* __restrict__ cnt2) {
   __shared__ uint32_t lds2[2];
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
   if (p >= a.npos) return;
+  cnt2[p] = ovf_flag[s] ? cnt[p] : 0u;
+}
+
+// Writes the surviving seeds of each listed position at its scanned offset,
+// in query-position then cell order (krnlWriteSeedList.cu:3-37 order).
+static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk, const uint8_t* __restrict__ ovf_flag,
+                                                              const uint32_t* __restrict__ m0,
+                                                              const uint32_t* __restrict__ rawcnt,
+                                                              const uint32_t* __restrict__ off,
+                                                              uint64_t* __restrict__ skey,
+                                                              uint32_t* __restrict__ sval) {
+  __shared__ uint32_t lds2[2];
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
+  if (p >= a.npos || !ovf_flag[s]) return;
   const uint32_t n = rawcnt[p];
   if (n == 0) return;
   const int qIdx = static_cast<int>(p - a.pos_off[s]);
@@ -287,10 +548,10 @@ static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs<KeyT> a, 
   uint32_t o = off[p];
   const uint32_t lo = m0[p];
   for (uint32_t j = lo; j < lo + n; ++j) {
-    const int gs = a.egs[j];
-    const int d = a.etidx[j] - qIdx;
-    if (!a.local && (d + a.gap < 0 || d + qLen - a.gap > a.tlen[gs])) continue;
-    skey[o] = (static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits)) | (static_cast<uint64_t>(gs) << pk.d_bits) |
+    const int2 e = a.epos[j];
+    const int d = e.y - qIdx;
+    if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
+    skey[o] = (static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits)) | (static_cast<uint64_t>(e.x) << pk.d_bits) |
               static_cast<uint64_t>(d + pk.d_off);
     sval[o] = static_cast<uint32_t>(qIdx);
     ++o;
@@ -308,21 +569,17 @@ struct SeedDecode {
   }
 };
 
-// hasNearPair core (…deleteDuplicateSeeds.cuh:29-38) for seeds of one
-// (qID, tID) group; `post` is the predecessor.
-__device__ __forceinline__ bool near_pair(int t, int d, int pt, int pd, int width, int gap) {
-  return width >= t - pt && width >= pt - t && gap >= pd - d;
-}
-
 // Per sorted seed: near-predecessor flag, group head index (for the
-// group-start max-scan), and the first seed of every (target window, query
-// window) unit — the element 0 that …deleteIsolateSeeds.cu:28-76 keeps.
+// group-start max-scan) and the isolate keep flag K(i) = near(i) or i is its
+// unit's element 0 (the first seed of the unit's first strand in that target
+// window), stored as i+1 for the last-kept max-scan.
 static __global__ __launch_bounds__(kBlock) void k_seed_marks(const uint64_t* __restrict__ skey,
-                                                       const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
-                                                       int width, int gap, const int32_t* __restrict__ seq_win_t,
-                                                       const int32_t* __restrict__ seq_win_q, int nqwin,
-                                                       uint8_t* __restrict__ nearf, uint32_t* __restrict__ gidx,
-                                                       uint32_t* __restrict__ unit_first) {
+                                                              const uint32_t* __restrict__ sval, uint32_t ns,
+                                                              SeedDecode dc, int width, int gap,
+                                                              const int32_t* __restrict__ seq_win_t,
+                                                              const int32_t* __restrict__ seq_win_q, int nqwin,
+                                                              const uint32_t* __restrict__ unit_first,
+                                                              uint32_t* __restrict__ kidx, uint32_t* __restrict__ gidx) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ns) return;
   const uint64_t k = skey[i];
@@ -340,35 +597,21 @@ static __global__ __launch_bounds__(kBlock) void k_seed_marks(const uint64_t* __
     }
     uhead = dc.s(pk) != s || seq_win_t[dc.gs(pk)] != tw;
   }
-  if (uhead) atomicMin(&unit_first[tw * nqwin + seq_win_q[s]], i);
-  nearf[i] = nr ? 1 : 0;
+  const bool elem0 = uhead && unit_first[tw * nqwin + seq_win_q[s]] == s;
+  kidx[i] = (nr || elem0) ? i + 1 : 0u;
   gidx[i] = ghead ? i : 0u;
-}
-
-// Isolate filter keep flag K(i) = near(i) or unit element 0; stored as i+1
-// for the last-kept max-scan.
-static __global__ __launch_bounds__(kBlock) void k_seed_keep(const uint64_t* __restrict__ skey, uint32_t ns, SeedDecode dc,
-                                                      const int32_t* __restrict__ seq_win_t,
-                                                      const int32_t* __restrict__ seq_win_q, int nqwin,
-                                                      const uint8_t* __restrict__ nearf,
-                                                      const uint32_t* __restrict__ unit_first,
-                                                      uint32_t* __restrict__ kidx) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= ns) return;
-  const uint64_t k = skey[i];
-  const uint32_t u = seq_win_t[dc.gs(k)] * nqwin + seq_win_q[dc.s(k)];
-  const bool keep = nearf[i] || unit_first[u] == i;
-  kidx[i] = keep ? i + 1 : 0u;
 }
 
 // Duplicate filter (…deleteDuplicateSeeds.cu:15-71) on the isolate output:
 // a kept seed survives unless its previous kept seed in the same group is
 // near.  lastk = inclusive max-scan of kidx, gstart = of gidx.
 static __global__ __launch_bounds__(kBlock) void k_seed_dup(const uint64_t* __restrict__ skey,
-                                                     const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
-                                                     int width, int gap, const uint32_t* __restrict__ kidx,
-                                                     const uint32_t* __restrict__ lastk,
-                                                     const uint32_t* __restrict__ gstart, uint32_t* __restrict__ dflag) {
+                                                            const uint32_t* __restrict__ sval, uint32_t ns,
+                                                            SeedDecode dc, int width, int gap,
+                                                            const uint32_t* __restrict__ kidx,
+                                                            const uint32_t* __restrict__ lastk,
+                                                            const uint32_t* __restrict__ gstart,
+                                                            uint32_t* __restrict__ dflag) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ns) return;
   uint32_t f = 0;
@@ -386,15 +629,16 @@ static __global__ __launch_bounds__(kBlock) void k_seed_dup(const uint64_t* __re
   dflag[i] = f;
 }
 
+// Appends the surviving seeds of the global path after `base` hits.
 static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* __restrict__ skey,
-                                                         const uint32_t* __restrict__ sval, uint32_t ns, SeedDecode dc,
-                                                         const uint32_t* __restrict__ dflag,
-                                                         const uint32_t* __restrict__ off, int32_t* __restrict__ hs,
-                                                         int32_t* __restrict__ hgs, int32_t* __restrict__ ht,
-                                                         int32_t* __restrict__ hq) {
+                                                                const uint32_t* __restrict__ sval, uint32_t ns,
+                                                                SeedDecode dc, const uint32_t* __restrict__ dflag,
+                                                                const uint32_t* __restrict__ off, uint32_t base,
+                                                                int32_t* __restrict__ hs, int32_t* __restrict__ hgs,
+                                                                int32_t* __restrict__ ht, int32_t* __restrict__ hq) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ns || !dflag[i]) return;
-  const uint32_t o = off[i];
+  const uint32_t o = base + off[i];
   const uint64_t k = skey[i];
   const int q = static_cast<int>(sval[i]);
   hs[o] = static_cast<int32_t>(dc.s(k));
@@ -626,23 +870,30 @@ static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const 
   }
 }
 
-// Order keys: word0 = (maxq - seed qIdx, strand), word1 = (target window,
-// query pair, maxscore - score).  A stable sort by word0 then word1 over
-// hits in seed order yields every unit's CDeviceHitList order.
+// Order keys of the kept hits, least significant first: word0 = (target
+// sequence, seed diagonal), word1 = (maxq - seed qIdx, strand), word2 =
+// (target window, query pair, maxscore - score).  Sorting stably by word0,
+// then word1, then word2 gives each unit's CDeviceHitList order whatever
+// order the hits arrived in.
 static __global__ __launch_bounds__(kBlock) void k_order_keys(uint32_t n, const uint32_t* __restrict__ kept,
-                                                       const uint32_t* __restrict__ hoff,
-                                                       const int32_t* __restrict__ hs, const int32_t* __restrict__ hgs,
-                                                       const int32_t* __restrict__ hq,
-                                                       const int32_t* __restrict__ score,
-                                                       const int32_t* __restrict__ seq_win_t, int maxq, int pair_bits,
-                                                       int score_bits, int smax, uint64_t* __restrict__ w0,
-                                                       uint64_t* __restrict__ w1, uint32_t* __restrict__ rec) {
+                                                              const uint32_t* __restrict__ hoff,
+                                                              const int32_t* __restrict__ hs,
+                                                              const int32_t* __restrict__ hgs,
+                                                              const int32_t* __restrict__ ht,
+                                                              const int32_t* __restrict__ hq,
+                                                              const int32_t* __restrict__ score,
+                                                              const int32_t* __restrict__ seq_win_t, int d_bits,
+                                                              int d_off, int maxq, int pair_bits, int score_bits,
+                                                              int smax, uint64_t* __restrict__ w0,
+                                                              uint64_t* __restrict__ w1, uint64_t* __restrict__ w2,
+                                                              uint32_t* __restrict__ rec) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || !kept[i]) return;
   const uint32_t o = hoff[i];
   const uint32_t s = static_cast<uint32_t>(hs[i]);
-  w0[o] = (static_cast<uint64_t>(maxq - hq[i]) << 1) | (s & 1u);
-  w1[o] = (static_cast<uint64_t>(seq_win_t[hgs[i]]) << (pair_bits + score_bits)) |
+  w0[o] = (static_cast<uint64_t>(hgs[i]) << d_bits) | static_cast<uint64_t>(ht[i] - hq[i] + d_off);
+  w1[o] = (static_cast<uint64_t>(maxq - hq[i]) << 1) | (s & 1u);
+  w2[o] = (static_cast<uint64_t>(seq_win_t[hgs[i]]) << (pair_bits + score_bits)) |
           (static_cast<uint64_t>(s >> 1) << score_bits) | static_cast<uint64_t>(smax - score[i]);
   rec[o] = i;
 }

@@ -58,13 +58,15 @@ def test_index_entries(lmer, stride, cut):
     np.testing.assert_array_equal(got["tIdx"], exp[:, 3])
 
 
-def run_unit(p, t, r, tw=None, qw=None):
+def run_unit(p, t, r, tw=None, qw=None, keep=True):
+    """keep=True retains the stages (every strand on the global path);
+    keep=False runs the per-strand LDS path."""
     tl, ql = lists(t, r, p.lmer)
     tw = tw or (0, tl.n_seq)
     qw = qw or (0, ql.n_seq // 2)
     ix = native.Index(p, tl, [tw])
     q = native.Query(p, ql, [qw])
-    hits = native.search_batch(p, ix, q, keep_stages=True)
+    hits = native.search_batch(p, ix, q, keep_stages=keep)
     o = oracle.search(p, tl, [tw], ql, [qw], dump=True)
     return hits, o
 
@@ -95,6 +97,8 @@ def test_unit_stages(kw):
     got = hits.arrays()
     assert_rows_equal(got, o["final"], what="final")
     assert len(o["final"]["tID"]) > 0
+    fast, _ = run_unit(p, t, r, keep=False)
+    assert_rows_equal(fast.arrays(), o["final"], what="final, per-strand path")
 
 
 def test_plain_search_matches_batch_unit():
@@ -138,7 +142,7 @@ def test_uneven_lengths_and_local():
     for local in (0, 1):
         p = native.Params(total_db_size=db_size(t), local=local)
         print(f"uneven local={local}", flush=True)
-        hits, o = run_unit(p, t, r)
+        hits, o = run_unit(p, t, r, keep=False)
         assert_rows_equal(hits.arrays(), o["final"], what=f"uneven local={local}")
 
 
@@ -151,8 +155,9 @@ def test_edge_inputs():
     off = np.concatenate([[0], np.cumsum([len(s) for s in seqs])]).astype(np.int64)
     r = gen.Reads(bases, off)
     p = native.Params(total_db_size=db_size(t))
-    hits, o = run_unit(p, t, r)
-    assert_rows_equal(hits.arrays(), o["final"], what="edge")
+    for keep in (True, False):
+        hits, o = run_unit(p, t, r, keep=keep)
+        assert_rows_equal(hits.arrays(), o["final"], what=f"edge keep={keep}")
     # a query set with no k-mer position at all
     short = gen.Reads(np.frombuffer(b"ACGTACGT", np.uint8).copy(), np.array([0, 8], np.int64))
     hits, o = run_unit(p, t, short)
@@ -171,3 +176,25 @@ def test_result_sort():
     native.result_sort(rows)
     for k in exp:
         np.testing.assert_array_equal(rows[k], exp[k])
+
+
+def test_long_reads_mix_fast_and_global_paths():
+    # 6-10 kbp reads: their true strands carry > 1024 seeds and take the
+    # global path; short reads in the same query set take the per-strand path
+    t, _ = workload(n_t=3, t_len=60_000, n_r=1)
+    long_r = gen.make_reads(gen.ReadSpec(21, 12, len_mode=gen.LEN_LOGUNIFORM, len_a=6000, len_b=10_000,
+                                         sub_rate=0.01, indel_rate=0.002), t)
+    short_r = gen.make_reads(gen.ReadSpec(22, 80, sub_rate=0.03, indel_rate=0.01), t)
+    bases = np.concatenate([long_r.bases, short_r.bases])
+    off = np.concatenate([long_r.offsets, short_r.offsets[1:] + long_r.offsets[-1]]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    for local in (0, 1):
+        p = native.Params(total_db_size=db_size(t), local=local, stride=2)
+        tl, ql = lists(t, r, p.lmer)
+        ix = native.Index(p, tl, [(0, 3)])
+        q = native.Query(p, ql, [(0, 46), (46, 92)])
+        print(f"long reads local={local}", flush=True)
+        got = native.search_batch(p, ix, q)
+        assert got.times()[7] > 3000  # seeds
+        o = oracle.search(p, tl, [(0, 3)], ql, [(0, 46), (46, 92)])
+        assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
