@@ -243,6 +243,8 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
     if (wins[w].n_slots)
       hip_check(hipMemcpyAsync(ix.tbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
                 "upload target bases");
+  if (win_slot[nwin])
+    k_classify<<<grid_for(win_slot[nwin], kBlock), kBlock, 0, st>>>(ix.tbase.ptr, win_slot[nwin]);
   ix.win_slot.upload(win_slot.data(), win_slot.size(), st);
   ix.seq_slot.upload(seq_slot.data(), seq_slot.size(), st);
   ix.seq_len.upload(ix.h_seq_len.data(), ix.h_seq_len.size(), st);
@@ -336,6 +338,8 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
     if (wins[w].n_slots)
       hip_check(hipMemcpyAsync(q.qbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
                 "upload query bases");
+  if (win_slot[nwin])
+    k_classify<<<grid_for(win_slot[nwin], kBlock), kBlock, 0, st>>>(q.qbase.ptr, win_slot[nwin]);
   q.qslot.upload(qslot.data(), qslot.size(), st);
   q.pos_off.upload(pos_off.data(), pos_off.size(), st);
   q.qlen.upload(q.h_qlen.data(), q.h_qlen.size(), st);
@@ -468,7 +472,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     nseed = static_cast<uint32_t>(total);
   }
 
-  // per-strand sort + isolate/duplicate, global path for the rest
+  // per-strand sort + isolate/duplicate, global path for the rest; hits are
+  // first written into their strand's seed range, then compacted in strand order
   uint32_t nhit = 0;
   const uint64_t cap = std::max<uint64_t>(nseed, 1);
   int32_t* hs = ws.get<int32_t>("h.s", cap);
@@ -476,24 +481,30 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   int32_t* ht = ws.get<int32_t>("h.t", cap);
   int32_t* hq = ws.get<int32_t>("h.q", cap);
   if (nseed > 0) {
+    int32_t* ss = ws.get<int32_t>("h.ss", cap);
+    int32_t* sgs = ws.get<int32_t>("h.sgs", cap);
+    int32_t* stt = ws.get<int32_t>("h.st", cap);
+    int32_t* sq = ws.get<int32_t>("h.sq", cap);
+    uint32_t* hcnt = ws.get<uint32_t>("h.cnt", q.nseq);
+    uint32_t* hbase = ws.get<uint32_t>("h.base", q.nseq);
     uint32_t* ovf_list = ws.get<uint32_t>("s.ovf", q.nseq);
     uint8_t* ovf_flag = ws.get<uint8_t>("s.ovfflag", q.nseq);
     uint32_t novf = 0;
+    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
     if (!keep_stages) {
-      k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
       const uint32_t grid = std::min<uint32_t>(grid_for(q.nseq, kSegWaves), 1u << 20);
       mark(EV_WRITE0);
       k_seed_segments<<<grid, kBlock, 0, st>>>(a, m0, rawc, off, nseed, d_off,
-                                                SegOut{hs, hgs, ht, hq, tot32, ovf_list, tot32 + 1, ovf_flag});
+                                                SegOut{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag});
       mark(EV_WRITE1);
       rt.check_launch("k_seed_segments");
-      uint32_t c2[2];
-      rt.read_scalars(tot32, c2, 2);
-      nhit = c2[0];
-      novf = c2[1];
+      rt.read_scalars(tot32 + 1, &novf, 1);
     } else {
+      // every strand on the global path: list them all
+      k_iota_u32<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_list, q.nseq);
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 1);
+      k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(hcnt, q.nseq, 0u);
       novf = q.nseq;
     }
     if (novf > 0) {
@@ -559,14 +570,26 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
         rt.check_launch("k_seed_dup");
         device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, ng, hoff, tot32 + 2, false);
+        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, q.pos_off.ptr, off, off2, ss, sgs, stt,
+                                             sq);
+        rt.check_launch("k_hits_scatter");
+        // per-strand hit counts of the listed strands (nh2 read on device)
         uint32_t nh2 = 0;
         rt.read_scalars(tot32 + 2, &nh2, 1);
-        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, nhit, hs, hgs, ht, hq);
-        rt.check_launch("k_hits_scatter");
-        nhit += nh2;
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.pos_off.ptr, np, off2, ng, hoff, nh2,
+                                                               hcnt);
         mark(EV_FILT1);
+      } else {
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.pos_off.ptr, np, off2, 0, hcnt, 0,
+                                                               hcnt);
       }
     }
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
+    rt.read_scalars(tot32 + 3, &nhit, 1);
+    if (nhit)
+      k_hits_compact<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(q.nseq, np, nseed, q.pos_off.ptr, off, hcnt, hbase,
+                                                                   ss, sgs, stt, sq, hs, hgs, ht, hq);
+    rt.check_launch("k_hits_compact");
   }
   if (keep_stages) {
     std::vector<int32_t> va, vb, vc, vd;
@@ -586,13 +609,15 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
     thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
     om = ws.get<int32_t>("d.m", nhit); osc = ws.get<int32_t>("d.s", nhit);
-    // length buckets: when query lengths differ, process hits longest first
+    // buckets by (read length, seed qIdx): converged lanes in both DP loops
     uint32_t* dporder = nullptr;
-    if (q.min_qlen != q.max_qlen && nhit > 1) {
+    if (nhit > 1) {
       uint64_t* dk = ws.get<uint64_t>("d.key", nhit);
       dporder = ws.get<uint32_t>("d.order", nhit);
-      k_dp_rows_key<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, hs, q.qlen.ptr, q.max_qlen, dk, dporder);
-      radix_sort_pairs(rt, dk, dporder, nhit, bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
+      const int qb = bits_for(static_cast<uint64_t>(q.max_qlen) + 1);
+      k_dp_rows_key<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, q.max_qlen, qb, dk, dporder);
+      radix_sort_pairs(rt, dk, dporder, nhit,
+                       qb + bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
     }
     DPArgs da{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
               ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, p.lmer, ot, oq, thl, qhl, om, osc};

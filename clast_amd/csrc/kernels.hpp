@@ -22,28 +22,39 @@ namespace clast {
 
 // ---------------------------------------------------------------- bases
 
-// 2-bit code of A/C/G/T (A0 C1 G2 T3, SRead2bit.cuh:12-18); 4 for any other byte.
-__device__ __forceinline__ uint32_t base_code(uint32_t c) {
-  const bool ok = c == 'A' || c == 'C' || c == 'G' || c == 'T';
-  const uint32_t v = ((c >> 1) & 3u) ^ ((c >> 2) & 1u);
-  return ok ? v : 4u;
+// Base classes.  Every base byte is stored on the device as a class code:
+// A/C/G/T one-hot in bits 0-3, any other letter 0x10 (never matches, N vs N
+// included), E F I J L O P Q Z 0x20 (scores -1), X 0x40 (bad area) — the
+// 26x26 POINT_MATRIX / MATCH_COUNTER of krnlMatrix.cuh:11-67 as a rule.
+constexpr uint32_t kClassX = 0x40u;
+constexpr uint32_t kClassStar = 0x20u;
+constexpr uint32_t kClassOther = 0x10u;
+
+__device__ __forceinline__ uint32_t dp_class(uint32_t c) {
+  const uint32_t idx = c - 'A';
+  constexpr uint32_t star = (1u << 4) | (1u << 5) | (1u << 8) | (1u << 9) | (1u << 11) | (1u << 14) | (1u << 15) |
+                            (1u << 16) | (1u << 25);  // E F I J L O P Q Z
+  uint32_t v = (idx < 26u && ((star >> idx) & 1u)) ? kClassStar : kClassOther;
+  v = c == 'X' ? kClassX : v;
+  v = c == 'A' ? 1u : v;
+  v = c == 'C' ? 2u : v;
+  v = c == 'G' ? 4u : v;
+  v = c == 'T' ? 8u : v;
+  return v;
 }
 
-// Substitution class bits for the DP (krnlMatrix.cuh:11-67 as a rule):
-// A/C/G/T one-hot in bits 0-3, any other letter 0x10 (never matches, N vs N
-// included), E F I J L O P Q Z 0x20 (-1), X 0x40 (bad area).
-__device__ __forceinline__ uint32_t dp_class(uint32_t c) {
-  switch (c) {
-    case 'A': return 1u;
-    case 'C': return 2u;
-    case 'G': return 4u;
-    case 'T': return 8u;
-    case 'X': return 0x40u;
-    case 'E': case 'F': case 'I': case 'J': case 'L': case 'O': case 'P': case 'Q': case 'Z': return 0x20u;
-    default: return 0x10u;
-  }
+// 2-bit code of a class (A0 C1 G2 T3, SRead2bit.cuh:12-18); 4 for anything
+// that is not A/C/G/T (SRepelHasOddBase.cuh:6-14 makes the k-mer -1).
+__device__ __forceinline__ uint32_t base_code(uint32_t cls) {
+  const uint32_t b = cls & 0xFu;
+  return b ? static_cast<uint32_t>(__builtin_ctz(b)) : 4u;
 }
-constexpr uint32_t kClassX = 0x40u;
+
+// Bytes -> class codes, in place (once per upload).
+static __global__ __launch_bounds__(kBlock) void k_classify(uint8_t* __restrict__ p, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = static_cast<uint8_t>(dp_class(p[i]));
+}
 
 template <class T>
 __device__ __forceinline__ uint32_t upper_bound_u(const T* a, uint32_t lo, uint32_t hi, T v) {
@@ -363,11 +374,11 @@ constexpr int kSegCap = 1024;          // seeds of one strand sorted in LDS
 constexpr int kSegWaves = kBlock / 64;
 
 struct SegOut {
-  int32_t* hs;
-  int32_t* hgs;
-  int32_t* ht;
+  int32_t* hs;            // hits of strand s go to [off[pos_off[s]], ...), the
+  int32_t* hgs;           // strand's own seed range (a strand has no more
+  int32_t* ht;            // hits than seeds)
   int32_t* hq;
-  uint32_t* nhit;         // atomic hit counter
+  uint32_t* hcnt;         // hits per strand
   uint32_t* ovf;          // strands with more than kSegCap seeds
   uint32_t* novf;
   uint8_t* ovf_flag;      // per strand
@@ -392,11 +403,17 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
   int32_t* Q = sq[w];
   for (uint32_t s = blockIdx.x * kSegWaves + w; s < a.nseq; s += gridDim.x * kSegWaves) {
     const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
-    if (p0 == p1) continue;
+    if (p0 == p1) {
+      if (lane == 0) o.hcnt[s] = 0;
+      continue;
+    }
     const uint32_t b0 = off[p0];
     const uint32_t b1 = p1 < a.npos ? off[p1] : nseed;
     const uint32_t n = b1 - b0;
-    if (n == 0) continue;
+    if (n == 0) {
+      if (lane == 0) o.hcnt[s] = 0;
+      continue;
+    }
     if (n > static_cast<uint32_t>(kSegCap)) {
       if (lane == 0) {
         o.ovf[atomicAdd(o.novf, 1u)] = s;
@@ -448,6 +465,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     // filters over the sorted list, 64 seeds at a time
     const int qw = a.qwin[s];
     int carry_k = -1, carry_g = -1;
+    uint32_t nh = 0;
     for (uint32_t c0 = 0; c0 < n; c0 += 64) {
       const uint32_t i = c0 + lane;
       const bool valid = i < n;
@@ -493,19 +511,18 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         }
       }
       const unsigned long long m = __ballot(dd);
-      uint32_t base = 0;
-      if (lane == 0 && m) base = atomicAdd(o.nhit, static_cast<uint32_t>(__popcll(m)));
-      base = __shfl(base, 0, 64);
       if (dd) {
-        const uint32_t at = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        const uint32_t at = b0 + nh + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
         o.hs[at] = static_cast<int32_t>(s);
         o.hgs[at] = gs;
         o.ht[at] = t;
         o.hq[at] = q;
       }
+      nh += static_cast<uint32_t>(__popcll(m));
       carry_k = __shfl(kin, 63, 64) > carry_k ? __shfl(kin, 63, 64) : carry_k;
       carry_g = __shfl(gin, 63, 64) > carry_g ? __shfl(gin, 63, 64) : carry_g;
     }
+    if (lane == 0) o.hcnt[s] = nh;
     wave_lds_sync();
   }
 }
@@ -629,22 +646,73 @@ static __global__ __launch_bounds__(kBlock) void k_seed_dup(const uint64_t* __re
   dflag[i] = f;
 }
 
-// Appends the surviving seeds of the global path after `base` hits.
+// Surviving seeds of the global path go to their strand's own seed range
+// (like k_seed_segments): slot = start of the strand's range + rank among
+// the strand's surviving seeds.  seg_off: the main seed offsets per position;
+// goff: the global path's offsets; hoff: exclusive scan of dflag.
 static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* __restrict__ skey,
                                                                 const uint32_t* __restrict__ sval, uint32_t ns,
                                                                 SeedDecode dc, const uint32_t* __restrict__ dflag,
-                                                                const uint32_t* __restrict__ off, uint32_t base,
+                                                                const uint32_t* __restrict__ hoff,
+                                                                const uint32_t* __restrict__ pos_off,
+                                                                const uint32_t* __restrict__ seg_off,
+                                                                const uint32_t* __restrict__ goff,
                                                                 int32_t* __restrict__ hs, int32_t* __restrict__ hgs,
                                                                 int32_t* __restrict__ ht, int32_t* __restrict__ hq) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ns || !dflag[i]) return;
-  const uint32_t o = base + off[i];
   const uint64_t k = skey[i];
+  const uint32_t s = dc.s(k);
+  const uint32_t o = seg_off[pos_off[s]] + (hoff[i] - hoff[goff[pos_off[s]]]);
   const int q = static_cast<int>(sval[i]);
-  hs[o] = static_cast<int32_t>(dc.s(k));
+  hs[o] = static_cast<int32_t>(s);
   hgs[o] = static_cast<int32_t>(dc.gs(k));
   ht[o] = dc.d(k) + q;
   hq[o] = q;
+}
+
+// Hit counts of the global-path strands: surviving seeds between the
+// strand's first global-path seed and the next strand's.
+static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __restrict__ ovf, uint32_t novf,
+                                                            const uint32_t* __restrict__ pos_off, uint32_t npos,
+                                                            const uint32_t* __restrict__ goff, uint32_t ng,
+                                                            const uint32_t* __restrict__ hoff, uint32_t nh,
+                                                            uint32_t* __restrict__ hcnt) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= novf) return;
+  const uint32_t s = ovf[k];
+  const uint32_t g0 = goff[pos_off[s]];
+  const uint32_t g1 = pos_off[s + 1] < npos ? goff[pos_off[s + 1]] : ng;
+  const uint32_t h0 = g0 < ng ? hoff[g0] : nh;
+  const uint32_t h1 = g1 < ng ? hoff[g1] : nh;
+  hcnt[s] = h1 - h0;
+}
+
+// Dense hits in strand order: strand s's hcnt[s] hits move from the start of
+// its seed range to hbase[s] (exclusive scan of hcnt).
+static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, uint32_t npos, uint32_t nseed,
+                                                                const uint32_t* __restrict__ pos_off,
+                                                                const uint32_t* __restrict__ seg_off,
+                                                                const uint32_t* __restrict__ hcnt,
+                                                                const uint32_t* __restrict__ hbase,
+                                                                const int32_t* __restrict__ hs,
+                                                                const int32_t* __restrict__ hgs,
+                                                                const int32_t* __restrict__ ht,
+                                                                const int32_t* __restrict__ hq,
+                                                                int32_t* __restrict__ ds, int32_t* __restrict__ dgs,
+                                                                int32_t* __restrict__ dt, int32_t* __restrict__ dq) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= nseq) return;
+  const uint32_t n = hcnt[s];
+  if (!n) return;
+  const uint32_t src = pos_off[s] < npos ? seg_off[pos_off[s]] : nseed;
+  const uint32_t dst = hbase[s];
+  for (uint32_t j = 0; j < n; ++j) {
+    ds[dst + j] = hs[src + j];
+    dgs[dst + j] = hgs[src + j];
+    dt[dst + j] = ht[src + j];
+    dq[dst + j] = hq[src + j];
+  }
 }
 
 // ---------------------------------------------------------------- banded DP
@@ -675,33 +743,36 @@ struct DPArgs {
 
 constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
 
-// Target base class at intra-sequence index ti.  The backward direction
-// guards ti >= tLen and the forward direction ti < 0, exactly like
-// krnlAlignment.cu:121-127,160-165 / 234-240,271-276; a read that leaves the
-// window's slot array is 'X'.
-__device__ __forceinline__ uint32_t t_class(const uint8_t* tbase, uint64_t seq_slot, int ti, int tlen, bool backward,
-                                            uint64_t wlo, uint64_t whi) {
-  if (backward ? ti >= tlen : ti < 0) return kClassX;
-  const int64_t at = static_cast<int64_t>(seq_slot) + ti;
-  if (at < static_cast<int64_t>(wlo) || at >= static_cast<int64_t>(whi)) return kClassX;
-  return dp_class(tbase[at]);
+// Target base class at intra-sequence index ti of a sequence whose class
+// bytes start at `seq`.  [lo, hi) is the valid range: the window's slots
+// relative to the sequence, cut by the direction's one-sided guard —
+// backward ti < tLen, forward ti >= 0 (krnlAlignment.cu:121-127,160-165 /
+// 234-240,271-276).  Anything outside is 'X'.  The load always stays in the
+// window (an invalid index reads the window's first slot and is discarded).
+__device__ __forceinline__ uint32_t t_class(const uint8_t* seq, int ti, int lo, int hi) {
+  const bool ok = ti >= lo && ti < hi;
+  const uint32_t v = seq[ok ? ti : lo];
+  return ok ? v : kClassX;
 }
 
 // One query row of the band (calcutlateCell, krnlAlignment.cu:42-77, for
-// bands 1..W in order; bands 0 and W+1 are the constant margins).
+// bands 1..W in order; bands 0 and W+1 are the constant margins).  The
+// horizontal value of band b-1 is only read by band b of the same row, so it
+// is carried in a register (starting from band 0's constant) instead of an
+// array.
 template <int W, bool LOCAL>
-__device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&H)[W + 2], int (&M)[W + 2],
-                                       const uint32_t (&T)[W], uint32_t qc, int r, int& bestS, int& bestM, int& bestR,
-                                       int& bestB) {
+__device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&M)[W + 2], const uint32_t (&T)[W],
+                                       uint32_t qc, int r, int& bestS, int& bestM, int& bestR, int& bestB) {
+  int hp = -30000;  // band 0's horizontal value (initTempNodeArray margin)
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
     const int v = max(S[b + 1] + (kGapOpen + kGapExt), V[b + 1] + kGapExt);
-    const int h = max(S[b - 1] + (kGapOpen + kGapExt), H[b - 1] + kGapExt);
+    const int h = max(S[b - 1] + (kGapOpen + kGapExt), hp + kGapExt);
     const uint32_t tc = T[b - 1];
     const uint32_t u = tc | qc;
     const bool m = (tc & qc & 0xFu) != 0u;
-    const int pm = (u & kClassX) ? kBad : ((u & 0x20u) ? -1 : (m ? kMatch : kMismatch));
-    const int mc = (u & kClassX) ? kBad : ((u & 0x20u) ? -1 : (m ? 1 : 0));
+    const int pm = (u & kClassX) ? kBad : ((u & kClassStar) ? -1 : (m ? kMatch : kMismatch));
+    const int mc = (u & kClassX) ? kBad : ((u & kClassStar) ? -1 : (m ? 1 : 0));
     const int pt = S[b] + pm;
     int ns, nm;
     if (pt > max(v, h)) {
@@ -716,8 +787,8 @@ __device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&H
     }
     S[b] = ns;
     V[b] = v;
-    H[b] = h;
     M[b] = nm;
+    hp = h;
     if (LOCAL) {
       if (bestS < ns) {
         bestS = ns;
@@ -731,18 +802,17 @@ __device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&H
 
 // initTempNodeArray (krnlAlignment.cu:6-38) for one hit's band.
 template <int G>
-__device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3], int (&H)[2 * G + 3],
-                                        int (&M)[2 * G + 3]) {
+__device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3], int (&M)[2 * G + 3]) {
   constexpr int half = G + 1;
 #pragma unroll
   for (int b = 0; b < 2 * G + 3; ++b) {
     if (b < half) {
-      S[b] = V[b] = H[b] = M[b] = -30000;
+      S[b] = V[b] = M[b] = -30000;
     } else if (b == half) {
-      S[b] = 0; V[b] = kGapOpen; H[b] = kGapOpen; M[b] = 0;
+      S[b] = 0; V[b] = kGapOpen; M[b] = 0;
     } else {
       const int sc = (b - half) * kGapExt + kGapOpen;
-      S[b] = sc; V[b] = sc + kGapOpen; H[b] = sc; M[b] = 0;
+      S[b] = sc; V[b] = sc + kGapOpen; M[b] = 0;
     }
   }
 }
@@ -751,7 +821,10 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
 // ({global,local}AlignBackward) then the left one ({global,local}AlignForward).
 // The two directions are independent (the forward pass reads the seed's
 // tIdx/qIdx, which the backward pass does not change), so one thread runs
-// both and sums the results.
+// both and sums the results.  Hits arrive ordered by (read length, seed
+// qIdx), so the lanes of a wavefront run near-equal row counts in both
+// loops.  The next row's query and target bytes are loaded before the
+// current row is computed.
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
@@ -765,8 +838,10 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
   const int tLen = a.tlen[gs];
   const uint64_t tslot = a.tslot[gs];
   const int w = a.twin[gs];
-  const uint64_t wlo = a.wslot[w], whi = a.wslot[w + 1];
-  int S[W + 2], V[W + 2], H[W + 2], M[W + 2];
+  const uint8_t* tseq = a.tbase + tslot;
+  const int wlo = static_cast<int>(static_cast<int64_t>(a.wslot[w]) - static_cast<int64_t>(tslot));
+  const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - static_cast<int64_t>(tslot));
+  int S[W + 2], V[W + 2], M[W + 2];
   uint32_t T[W];
   int oT = tIdx, oQ = qIdx, tHL = L, qHL = L, mm = L, sc = L * kMatch;
 
@@ -775,21 +850,31 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
     const bool bwd = dir == 0;
     const int rows = bwd ? qLen - (qIdx + L) : qIdx;
     if (rows == 0) continue;
-    dp_init<G>(S, V, H, M);
+    dp_init<G>(S, V, M);
+    // ring slot k holds target index tb + ts*k; row r adds tn0 + ts*r
+    const int ts = bwd ? 1 : -1;
+    const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+    const int tn0 = bwd ? tIdx + L + G + 1 : tIdx - G - 2;
+    const int lo = bwd ? wlo : max(wlo, 0);
+    const int hi = bwd ? min(whi, tLen) : whi;
 #pragma unroll
-    for (int k = 0; k < W; ++k)
-      T[k] = bwd ? t_class(a.tbase, tslot, tIdx + L - G + k, tLen, true, wlo, whi)
-                 : t_class(a.tbase, tslot, tIdx + G - k - 1, tLen, false, wlo, whi);
+    for (int k = 0; k < W; ++k) T[k] = t_class(tseq, tb + ts * k, lo, hi);
     int bestS = 0, bestM = 0, bestR = -1, bestB = G;
     const int q0 = bwd ? qIdx + L : qIdx - 1;
-    const int qstep = bwd ? 1 : -1;
-    const int tnext0 = bwd ? tIdx + L + G + 1 : tIdx - G - 2;
+    uint32_t qc = qb[q0];
+    uint32_t tn = t_class(tseq, tn0, lo, hi);
+#pragma unroll 1
     for (int r = 0; r < rows; ++r) {
-      const uint32_t qc = dp_class(qb[q0 + qstep * r]);
-      dp_row<W, LOCAL>(S, V, H, M, T, qc, r, bestS, bestM, bestR, bestB);
+      // next row's bytes first (the last row re-reads its own, harmlessly)
+      const int rn = r + 1 < rows ? r + 1 : r;
+      const uint32_t qc_n = qb[q0 + ts * rn];
+      const uint32_t tn_n = t_class(tseq, tn0 + ts * rn, lo, hi);
+      dp_row<W, LOCAL>(S, V, M, T, qc, r, bestS, bestM, bestR, bestB);
 #pragma unroll
       for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
-      T[W - 1] = t_class(a.tbase, tslot, tnext0 + qstep * r, tLen, bwd, wlo, whi);
+      T[W - 1] = tn;
+      qc = qc_n;
+      tn = tn_n;
     }
     int move, endp;
     if (LOCAL) {
@@ -832,15 +917,17 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
 
-// Length-bucket key of a hit: its DP row count (both directions together
-// always total qLen - lMer) complemented, so a stable ascending sort puts the
-// longest hits first.
+// DP bucket key of a hit: (maxq - read length, seed qIdx).  Both directions
+// together always run qLen - lMer rows; sorting by length (longest first)
+// and then by qIdx gives the lanes of a wavefront near-equal backward and
+// forward row counts.
 static __global__ __launch_bounds__(kBlock) void k_dp_rows_key(uint32_t n, const int32_t* __restrict__ hs,
-                                                               const int32_t* __restrict__ qlen, int maxq,
+                                                               const int32_t* __restrict__ hq,
+                                                               const int32_t* __restrict__ qlen, int maxq, int qbits,
                                                                uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  key[i] = static_cast<uint64_t>(maxq - qlen[hs[i]]);
+  key[i] = (static_cast<uint64_t>(maxq - qlen[hs[i]]) << qbits) | static_cast<uint64_t>(hq[i]);
   idx[i] = i;
 }
 
