@@ -106,7 +106,8 @@ struct clast_index_s {
   DevArray<uint64_t> ekey64;
   DevArray<int2> epos;
   DevArray<unsigned long long> bitmap;
-  DevArray<uint32_t> rank, cell_off;
+  DevArray<uint32_t> rank;
+  DevArray<uint4> cells;
   std::vector<int32_t> h_seq_len, h_seq_win, h_seq_ref, h_id_begin;
   std::vector<uint64_t> h_win_slot;
 };
@@ -178,9 +179,10 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
     k_cell_heads<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head);
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, head, nent, cell, scal, false);
     rt.read_scalars(scal, &ncell, 1);
-    ix.cell_off.alloc(static_cast<uint64_t>(ncell) + 1);
-    k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, ix.cell_off.ptr,
-                                                             ix.bitmap.ptr);
+    uint32_t* cell_off = ws.get<uint32_t>("ix.coff", static_cast<uint64_t>(ncell) + 1);
+    k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, cell_off, ix.bitmap.ptr);
+    ix.cells.alloc(ncell);
+    k_cell_records<<<grid_for(ncell, kBlock), kBlock, 0, st>>>(cell_off, ncell, ix.epos.ptr, ix.cells.ptr);
     rt.check_launch("k_cell_build");
     if (ix.key64) {
       ix.ekey64.alloc(nent);
@@ -190,8 +192,8 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
       k_narrow_keys<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.ekey32.ptr);
     }
   } else {
-    ix.cell_off.alloc(1);
-    k_fill<uint32_t><<<1, 64, 0, st>>>(ix.cell_off.ptr, 1, 0u);
+    ix.cells.alloc(1);
+    k_fill<uint4><<<1, 64, 0, st>>>(ix.cells.ptr, 1, make_uint4(0, 0, 0, 0));
     ix.ekey32.alloc(1);
     ix.ekey64.alloc(1);
   }
@@ -392,7 +394,9 @@ struct StageEvents {
 };
 
 // The CDeviceHitList chain (CDeviceHitList.cu:228-423) for every unit of
-// idx x q.  final_order: apply the fixResult order across units.
+// idx x q, rows in fixResult order (CHostResultHolder.cu:75-128): query
+// pair, score descending, target window, then each unit's own order.  For a
+// single unit that is the unit's CDeviceHitList order.
 //
 // Seeds of a query strand are contiguous after the count scan, so the sort
 // by (qID, tID, diagonal, qIdx) and the isolate / duplicate filters run per
@@ -401,7 +405,7 @@ struct StageEvents {
 // sequence, diagonal) keys and scan-based filters.  CLAST_KEEP_STAGES sends
 // every strand through the global path so its intermediate arrays can be
 // read back.
-void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int flags, bool final_order,
+void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int flags,
                 clast_hits_s& out) {
   if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
     throw ClastError(CLAST_ERR_ARG, "params differ from those the index/query were built with");
@@ -440,9 +444,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   a.pos_off = q.pos_off.ptr;
   a.nseq = q.nseq;
   a.npos = np;
-  a.lk = Lookup{ix.bitmap.ptr, ix.rank.ptr, ix.cell_off.ptr,
+  a.lk = Lookup{ix.bitmap.ptr, ix.rank.ptr, ix.cells.ptr,
                 ix.key64 ? static_cast<const void*>(ix.ekey64.ptr) : static_cast<const void*>(ix.ekey32.ptr),
-                ix.shift, ix.key64 ? 1 : 0};
+                ix.epos.ptr, ix.shift, ix.key64 ? 1 : 0};
   a.epos = ix.epos.ptr;
   a.tlen = ix.seq_len.ptr;
   a.twin = ix.seq_win.ptr;
@@ -455,14 +459,14 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
 
   // lookup + count
   uint32_t nseed = 0;
-  uint32_t *m0 = nullptr, *rawc = nullptr, *cnt = nullptr, *off = nullptr;
+  uint4* cellp = nullptr;
+  uint32_t *cnt = nullptr, *off = nullptr;
   if (np > 0 && ix.nentries > 0) {
-    m0 = ws.get<uint32_t>("s.m0", np);
-    rawc = ws.get<uint32_t>("s.raw", np);
+    cellp = ws.get<uint4>("s.cell", np);
     cnt = ws.get<uint32_t>("s.cnt", np);
     off = ws.get<uint32_t>("s.off", np);
     mark(EV_COUNT0);
-    k_seed_count<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, m0, rawc, cnt);
+    k_seed_count<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, cellp, cnt);
     mark(EV_COUNT1);
     rt.check_launch("k_seed_count");
     device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt, np, off, tot64, false);
@@ -495,8 +499,13 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
       const uint32_t grid = std::min<uint32_t>(grid_for(q.nseq, kSegWaves), 1u << 20);
       mark(EV_WRITE0);
-      k_seed_segments<<<grid, kBlock, 0, st>>>(a, m0, rawc, off, nseed, d_off,
-                                                SegOut{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag});
+      const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
+      const SegPack sp{d_off, qbits, dbits};
+      const SegOut so{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag};
+      if (gbits + dbits + qbits <= 64)
+        k_seed_segments<true><<<grid, kBlock, 0, st>>>(a, cellp, off, nseed, sp, so);
+      else
+        k_seed_segments<false><<<grid, kBlock, 0, st>>>(a, cellp, off, nseed, sp, so);
       mark(EV_WRITE1);
       rt.check_launch("k_seed_segments");
       rt.read_scalars(tot32 + 1, &novf, 1);
@@ -520,7 +529,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       const uint32_t ng = static_cast<uint32_t>(t2);
       uint64_t* skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
       uint32_t* sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
-      k_seed_write<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, ovf_flag, m0, rawc, off2, skey, sval);
+      k_seed_write<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, ovf_flag, cellp, off2, skey, sval);
       rt.check_launch("k_seed_write");
       auto dump_seeds = [&](int slot) {
         std::vector<uint64_t> hk;
@@ -642,64 +651,99 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.stage_n[CLAST_STAGE_ALIGNED] = nhit;
   }
 
-  // E-value, filters, result order, dedup
+  // E-value, filters, dedup and row order, pair by pair (hits are dense in
+  // strand order, so a pair's hits are contiguous)
   mark(EV_RES0);
-  uint32_t nkeep = 0, nfinal = 0;
+  uint32_t nfinal = 0;
   uint32_t* order = nullptr;
-  uint32_t* permB = nullptr;
   double* hev = nullptr;
   if (nhit) {
     hev = ws.get<double>("r.ev", nhit);
     uint32_t* kflag = ws.get<uint32_t>("r.kflag", nhit);
-    uint32_t* koff = ws.get<uint32_t>("r.koff", nhit);
-    int32_t* srange = ws.get<int32_t>("r.srange", 4);
-    k_fill<int32_t><<<1, 64, 0, st>>>(srange, 1, INT32_MAX);
-    k_fill<int32_t><<<1, 64, 0, st>>>(srange + 1, 1, INT32_MIN);
+    RowKey* rk = ws.get<RowKey>("r.key", nhit);
+    uint32_t* dflag = ws.get<uint32_t>("r.dflag", nhit);
+    uint32_t* gflag = ws.get<uint32_t>("r.gflag", nhit);
+    uint32_t* doff = ws.get<uint32_t>("r.doff", nhit);
+    uint32_t* hbase = ws.get<uint32_t>("h.base", q.nseq);
     const uint32_t g = grid_for(nhit, kBlock);
-    k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, osc, qhl, q.qlen.ptr, q.smin.ptr, use_cut ? 1 : 0,
-                                       p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev, kflag, srange);
+    k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, hgs, ht, hq, osc, qhl, q.qlen.ptr, ix.seq_win.ptr, q.smin.ptr,
+                                       use_cut ? 1 : 0, p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev,
+                                       kflag, rk);
     rt.check_launch("k_hit_filter");
-    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, kflag, nhit, koff, tot32 + 3, false);
-    hip_check(hipMemcpyAsync(tot32 + 4, srange, 8, hipMemcpyDeviceToDevice, st), "copy");
-    uint32_t sc3[3];
-    rt.read_scalars(tot32 + 3, sc3, 3);
-    nkeep = sc3[0];
-    const int32_t smin_v = static_cast<int32_t>(sc3[1]), smax_v = static_cast<int32_t>(sc3[2]);
-    if (nkeep) {
+    PairArgs pa{nhit, q.nseq, hbase, hs, hgs, ot, oq, kflag, rk};
+    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 4, 1, 0u);
+    k_pair_dedup<<<g, kBlock, 0, st>>>(pa, dflag, gflag, tot32 + 4);
+    rt.check_launch("k_pair_dedup");
+    uint32_t nbig = 0;
+    rt.read_scalars(tot32 + 4, &nbig, 1);
+    uint32_t* grow = nullptr;
+    uint32_t* gfirst = nullptr;
+    uint32_t ngrow = 0;
+    if (nbig) {
+      // pairs over kPairSmall hits: complete order keys, radix sorts, dedup
+      uint32_t* goff = ws.get<uint32_t>("r.goff", nhit);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, gflag, nhit, goff, tot32 + 5, false);
+      int32_t* srange = ws.get<int32_t>("r.srange", 2);
+      k_fill<int32_t><<<1, 64, 0, st>>>(srange, 1, INT32_MAX);
+      k_fill<int32_t><<<1, 64, 0, st>>>(srange + 1, 1, INT32_MIN);
+      k_score_range<<<g, kBlock, 0, st>>>(nhit, gflag, osc, srange);
+      hip_check(hipMemcpyAsync(tot32 + 6, srange, 8, hipMemcpyDeviceToDevice, st), "copy");
+      uint32_t sc3[3];
+      rt.read_scalars(tot32 + 5, sc3, 3);
+      const uint32_t nk = sc3[0];
+      const int32_t smin_v = static_cast<int32_t>(sc3[1]), smax_v = static_cast<int32_t>(sc3[2]);
       const int scb = bits_for(static_cast<uint64_t>(static_cast<int64_t>(smax_v) - smin_v + 1));
       const int pbits = bits_for((q.nseq + 1) / 2);
       const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
       const int qb = bits_for(static_cast<uint64_t>(q.max_qlen) + 1);
       if (wbits + pbits + scb > 64 || gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "order key over 64 bits");
-      uint64_t* w0 = ws.get<uint64_t>("r.w0", nkeep);
-      uint64_t* w1 = ws.get<uint64_t>("r.w1", nkeep);
-      uint64_t* w2 = ws.get<uint64_t>("r.w2", nkeep);
-      uint32_t* rec = ws.get<uint32_t>("r.rec", nkeep);
-      uint32_t* perm = ws.get<uint32_t>("r.perm", nkeep);
-      k_order_keys<<<g, kBlock, 0, st>>>(nhit, kflag, koff, hs, hgs, ht, hq, osc, ix.seq_win.ptr, dbits, d_off,
+      uint64_t* w0 = ws.get<uint64_t>("r.w0", nk);
+      uint64_t* w1 = ws.get<uint64_t>("r.w1", nk);
+      uint64_t* w2 = ws.get<uint64_t>("r.w2", nk);
+      uint32_t* rec = ws.get<uint32_t>("r.rec", nk);
+      uint32_t* perm = ws.get<uint32_t>("r.perm", nk);
+      k_order_keys<<<g, kBlock, 0, st>>>(nhit, gflag, goff, hs, hgs, ht, hq, osc, ix.seq_win.ptr, dbits, d_off,
                                          q.max_qlen, pbits, scb, smax_v, w0, w1, w2, rec);
       rt.check_launch("k_order_keys");
       const uint64_t* words[3] = {w0, w1, w2};
       const int wb[3] = {gbits + dbits, qb + 1, wbits + pbits + scb};
-      sort_perm(rt, nkeep, 3, words, wb, perm);
-      uint32_t* dflag = ws.get<uint32_t>("r.dflag", nkeep);
-      uint32_t* doff = ws.get<uint32_t>("r.doff", nkeep);
-      const uint32_t gk = grid_for(nkeep, kBlock);
-      k_dedup<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, hs, hgs, ot, oq, ix.seq_win.ptr, q.qwin.ptr, dflag);
-      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nkeep, doff, tot32 + 6, false);
-      rt.read_scalars(tot32 + 6, &nfinal, 1);
-      order = ws.get<uint32_t>("r.order", nfinal);
-      k_compact_perm<<<gk, kBlock, 0, st>>>(nkeep, perm, rec, dflag, doff, order);
+      sort_perm(rt, nk, 3, words, wb, perm);
+      uint32_t* bflag = ws.get<uint32_t>("r.bflag", nk);
+      uint32_t* boff = ws.get<uint32_t>("r.boff", nk);
+      const uint32_t gk = grid_for(nk, kBlock);
+      k_dedup<<<gk, kBlock, 0, st>>>(nk, perm, rec, hs, hgs, ot, oq, ix.seq_win.ptr, q.qwin.ptr, bflag);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, bflag, nk, boff, tot32 + 7, false);
+      rt.read_scalars(tot32 + 7, &ngrow, 1);
+      grow = ws.get<uint32_t>("r.grow", ngrow);
+      k_compact_perm<<<gk, kBlock, 0, st>>>(nk, perm, rec, bflag, boff, grow);
       rt.check_launch("k_dedup/compact");
-      if (final_order && (ix.nwin > 1 || q.nwin > 1) && nfinal > 1) {
-        uint64_t* wf = ws.get<uint64_t>("r.wf", nfinal);
-        permB = ws.get<uint32_t>("r.permB", nfinal);
-        k_final_keys<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(nfinal, order, hs, hgs, osc, ix.seq_win.ptr, scb,
-                                                                  wbits, smax_v, wf);
+      if (ix.nwin > 1 && ngrow > 1) {
+        // unit order is window-major: rows of a pair to (score desc, window)
+        uint64_t* wf = ws.get<uint64_t>("r.wf", ngrow);
+        uint32_t* pf = ws.get<uint32_t>("r.permB", ngrow);
+        uint32_t* g2 = ws.get<uint32_t>("r.grow2", ngrow);
+        k_final_keys<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hgs, osc, ix.seq_win.ptr, scb,
+                                                                 wbits, smax_v, wf);
         const uint64_t* fw[1] = {wf};
         const int fb[1] = {pbits + scb + wbits};
-        sort_perm(rt, nfinal, 1, fw, fb, permB);
+        sort_perm(rt, ngrow, 1, fw, fb, pf);
+        k_gather<uint32_t><<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(grow, pf, g2, ngrow);
+        grow = g2;
       }
+      if (ngrow) {
+        gfirst = ws.get<uint32_t>("r.gfirst", nhit);
+        k_big_counts<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hbase, dflag, gfirst);
+        rt.check_launch("k_big_counts");
+      }
+    }
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nhit, doff, tot32 + 3, false);
+    rt.read_scalars(tot32 + 3, &nfinal, 1);
+    if (nfinal) {
+      order = ws.get<uint32_t>("r.order", nfinal);
+      k_pair_place<<<g, kBlock, 0, st>>>(pa, dflag, doff, order);
+      if (ngrow)
+        k_big_place<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hbase, gfirst, doff, order);
+      rt.check_launch("k_pair_place");
     }
   }
   // gather + D2H
@@ -708,7 +752,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   if (nfinal) {
     int32_t* o32 = ws.get<int32_t>("o.i32", static_cast<uint64_t>(nfinal) * 10);
     double* oev = ws.get<double>("o.ev", nfinal);
-    OutArgs oa{nfinal, order, permB, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
+    OutArgs oa{nfinal, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
                ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
     k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
     rt.check_launch("k_gather_out");
@@ -832,7 +876,7 @@ int clast_search(const clast_params* p, clast_index idx, clast_query q, int32_t 
     if (idx->h_id_begin[0] != t_begin || q->h_id_begin[0] != q_begin)
       throw ClastError(CLAST_ERR_ARG, "t_begin/q_begin differ from the windows' id_begin");
     auto h = std::make_unique<clast_hits_s>();
-    run_search(*p, *idx, *q, 0, false, *h);
+    run_search(*p, *idx, *q, 0, *h);
     *out = h.release();
   });
 }
@@ -842,7 +886,7 @@ int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, in
     check_params(p);
     if (!idx || !q || !out) throw ClastError(CLAST_ERR_ARG, "null handle");
     auto h = std::make_unique<clast_hits_s>();
-    run_search(*p, *idx, *q, flags, true, *h);
+    run_search(*p, *idx, *q, flags, *h);
     *out = h.release();
   });
 }

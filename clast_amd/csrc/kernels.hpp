@@ -194,10 +194,12 @@ static __global__ __launch_bounds__(kBlock) void k_narrow_keys(const uint64_t* _
 }
 
 // Presence bitmap over the top B key bits (B = min(2 lMer, 30)): bit h is set
-// when some entry's key prefix is h.  cell_off[r] is the first entry of the
-// r-th present prefix, rank[b] the number of set bits before 512-bit block b.
-// A query k-mer whose bit is clear costs one bitmap line, which stays in the
-// MI355X Infinity Cache (128 MiB at B = 30).
+// when some entry's key prefix is h; rank[b] is the number of set bits before
+// 512-bit block b.  The r-th present prefix has a 16-byte cell record
+// {first entry, entry count, target sequence and tIdx of the first entry}:
+// a query k-mer whose bit is clear costs one bitmap line (the bitmap stays in
+// the MI355X Infinity Cache: 128 MiB at B = 30), a present one a single
+// record line more when its cell holds one entry, the common case.
 static __global__ __launch_bounds__(kBlock) void k_cell_heads(const uint64_t* __restrict__ keys, uint32_t n,
                                                               int shift, uint32_t* __restrict__ head) {
   const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
@@ -220,6 +222,16 @@ static __global__ __launch_bounds__(kBlock) void k_cell_build(const uint64_t* __
   if (e + 1 == n) cell_off[cell[e] + head[e]] = n;
 }
 
+static __global__ __launch_bounds__(kBlock) void k_cell_records(const uint32_t* __restrict__ cell_off, uint32_t ncell,
+                                                                const int2* __restrict__ epos,
+                                                                uint4* __restrict__ rec) {
+  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= ncell) return;
+  const uint32_t lo = cell_off[r];
+  const int2 e = epos[lo];
+  rec[r] = make_uint4(lo, cell_off[r + 1] - lo, static_cast<uint32_t>(e.x), static_cast<uint32_t>(e.y));
+}
+
 static __global__ __launch_bounds__(kBlock) void k_rank_count(const unsigned long long* __restrict__ bitmap,
                                                               uint64_t nwords, uint64_t nblk,
                                                               uint32_t* __restrict__ cnt) {
@@ -233,27 +245,33 @@ static __global__ __launch_bounds__(kBlock) void k_rank_count(const unsigned lon
 struct Lookup {
   const unsigned long long* bitmap;
   const uint32_t* rank;
-  const uint32_t* cell_off;
+  const uint4* cells;
   const void* ekey;   // u32 or u64 keys; read only when the prefix is not the whole key
+  const int2* epos;
   int shift;          // 2 lMer - B
   int key64;
 };
 
-// Entry range [lo, hi) of a valid query k-mer (krnlBinarySearch.cu:3-33 +
-// the cell gathers of …createRawSeedList.cu:63-135): empty when absent.
-__device__ __forceinline__ void lookup(const Lookup& L, uint64_t key, uint32_t& lo, uint32_t& hi) {
+// Entries of a valid query k-mer (krnlBinarySearch.cu:3-33 + the cell
+// gathers of …createRawSeedList.cu:63-135): returns their count n; when
+// n > 0, [lo, lo + n) is their range and first the first entry.
+__device__ __forceinline__ uint32_t lookup(const Lookup& L, uint64_t key, uint32_t& lo, int2& first) {
   const uint64_t h = key >> L.shift;
   const uint64_t w = h >> 6;
   const unsigned long long word = L.bitmap[w];
   const int bit = static_cast<int>(h & 63);
-  lo = hi = 0;
-  if (!((word >> bit) & 1ull)) return;
+  lo = 0;
+  first = make_int2(0, 0);
+  if (!((word >> bit) & 1ull)) return 0;
   uint32_t r = L.rank[w >> 3];
   for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
   r += static_cast<uint32_t>(__popcll(word & ((1ull << bit) - 1ull)));
-  lo = L.cell_off[r];
-  hi = L.cell_off[r + 1];
+  const uint4 c = L.cells[r];
+  lo = c.x;
+  uint32_t n = c.y;
+  first = make_int2(static_cast<int>(c.z), static_cast<int>(c.w));
   if (L.shift) {  // the prefix cell holds several keys: narrow to this one
+    uint32_t hi = lo + n;
     if (L.key64) {
       const uint64_t* k = static_cast<const uint64_t*>(L.ekey);
       lo = lower_bound_u(k, lo, hi, key);
@@ -263,7 +281,10 @@ __device__ __forceinline__ void lookup(const Lookup& L, uint64_t key, uint32_t& 
       lo = lower_bound_u(k, lo, hi, static_cast<uint32_t>(key));
       hi = upper_bound_u(k, lo, hi, static_cast<uint32_t>(key));
     }
+    n = hi - lo;
+    if (n) first = L.epos[lo];
   }
+  return n;
 }
 
 // ---------------------------------------------------------------- seeding
@@ -302,14 +323,14 @@ __device__ __forceinline__ bool corner_ok(const SeedArgs& a, int d, int qlen, in
   return a.local || !(d + a.gap < 0 || d + qlen - a.gap > tlen);
 }
 
-// Query k-mer lookup per position: matching entry range and the number of
-// seeds surviving the corner filter.  Only positions with qIdx + lMer <= qLen
-// exist here: the query-boundary filter (…deleteSeedsOnSequenceBoundary.cu:
-// 19-72) removes every other one.  Also records, per (target window, query
-// window) unit, the first strand holding a seed — the strand of the unit's
-// element 0, which the isolate filter keeps (…deleteIsolateSeeds.cu:28-76).
-static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint32_t* __restrict__ m0,
-                                                              uint32_t* __restrict__ rawcnt,
+// Query k-mer lookup per position: its cell {first entry, entry count, first
+// entry's (seq, tIdx)} and the number of seeds surviving the corner filter.
+// Only positions with qIdx + lMer <= qLen exist here: the query-boundary
+// filter (…deleteSeedsOnSequenceBoundary.cu:19-72) removes every other one.
+// Also records, per (target window, query window) unit, the first strand
+// holding a seed — the strand of the unit's element 0, which the isolate
+// filter keeps (…deleteIsolateSeeds.cu:28-76).
+static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint4* __restrict__ cellp,
                                                               uint32_t* __restrict__ cnt) {
   __shared__ uint32_t lds2[2];
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -325,13 +346,14 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint32
     ok = ok && c < 4u;
     key = (key << 2) | (c & 3u);
   }
-  uint32_t lo = 0, hi = 0, c = 0;
+  uint32_t lo = 0, n = 0, c = 0;
+  int2 first = make_int2(0, 0);
   if (ok) {
-    lookup(a.lk, key, lo, hi);
+    n = lookup(a.lk, key, lo, first);
     const int qw = a.qwin[s];
     int last_tw = -1;
-    for (uint32_t j = lo; j < hi; ++j) {
-      const int2 e = a.epos[j];
+    for (uint32_t j = 0; j < n; ++j) {
+      const int2 e = j == 0 ? first : a.epos[lo + j];
       const int d = e.y - qIdx;
       if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
       ++c;
@@ -343,8 +365,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint32
       }
     }
   }
-  m0[p] = lo;
-  rawcnt[p] = hi - lo;
+  cellp[p] = make_uint4(lo, n, static_cast<uint32_t>(first.x), static_cast<uint32_t>(first.y));
   cnt[p] = c;
 }
 
@@ -384,23 +405,68 @@ struct SegOut {
   uint8_t* ovf_flag;      // per strand
 };
 
+// LDS image of one strand's seeds.  Packed: one u64 (target sequence, diagonal
+// + d_off, qIdx) when the three fit in 64 bits; otherwise (target sequence,
+// diagonal + d_off) and qIdx apart.  Either sorts as (seq, diagonal, qIdx).
+template <bool PACKED>
+struct SegLds;
+template <>
+struct SegLds<true> {
+  uint64_t k[kSegWaves][kSegCap];
+};
+template <>
+struct SegLds<false> {
+  uint64_t k[kSegWaves][kSegCap];
+  int32_t q[kSegWaves][kSegCap];
+};
+
+struct SegPack {
+  int d_off;
+  int qbits, dbits;  // packed layout only
+};
+
+template <bool PACKED>
+__device__ __forceinline__ void seg_put(SegLds<PACKED>& L, int w, uint32_t at, const SegPack& pk, int gs, int d,
+                                        int q) {
+  if constexpr (PACKED) {
+    L.k[w][at] = (static_cast<uint64_t>(gs) << (pk.dbits + pk.qbits)) |
+                 (static_cast<uint64_t>(d + pk.d_off) << pk.qbits) | static_cast<uint64_t>(q);
+  } else {
+    L.k[w][at] = (static_cast<uint64_t>(gs) << 32) | static_cast<uint32_t>(d + pk.d_off);
+    L.q[w][at] = q;
+  }
+}
+
+template <bool PACKED>
+__device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t i, const SegPack& pk, int& gs, int& d,
+                                        int& q) {
+  const uint64_t k = L.k[w][i];
+  if constexpr (PACKED) {
+    gs = static_cast<int>(k >> (pk.dbits + pk.qbits));
+    d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
+    q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
+  } else {
+    gs = static_cast<int>(k >> 32);
+    d = static_cast<int>(static_cast<uint32_t>(k)) - pk.d_off;
+    q = L.q[w][i];
+  }
+}
+
 // One wavefront per query strand: its seeds (positions in qIdx order, cells
 // in index order) go to LDS, are bitonic-sorted by (target sequence,
 // diagonal, qIdx) — the …sortSeeds.cuh:14-55 order restricted to one qID —
 // and the isolate and duplicate filters run on the sorted list with wave
 // scans: K(i) = near(i) or i is its unit's element 0; D(i) = K(i) and not
-// near(i, previous K of the group).  Surviving seeds are appended as hits
-// (their order is fixed later by complete sort keys).  Strands with more
-// than kSegCap seeds are listed for the global path.
-static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint32_t* __restrict__ m0,
-                                                                 const uint32_t* __restrict__ rawcnt,
+// near(i, previous K of the group).  Surviving seeds are written as hits at
+// the start of the strand's own seed range (their order is fixed later by
+// complete sort keys).  Strands with more than kSegCap seeds are listed for
+// the global path.
+template <bool PACKED>
+static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint4* __restrict__ cellp,
                                                                  const uint32_t* __restrict__ off, uint32_t nseed,
-                                                                 int d_off, SegOut o) {
-  __shared__ uint64_t skey[kSegWaves][kSegCap];
-  __shared__ int32_t sq[kSegWaves][kSegCap];
+                                                                 SegPack pk, SegOut o) {
+  __shared__ SegLds<PACKED> L;
   const int w = wave_id(), lane = lane_id();
-  uint64_t* K = skey[w];
-  int32_t* Q = sq[w];
   for (uint32_t s = blockIdx.x * kSegWaves + w; s < a.nseq; s += gridDim.x * kSegWaves) {
     const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
     if (p0 == p1) {
@@ -424,38 +490,46 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     const int qLen = a.qlen[s];
     // generate the strand's seeds into LDS at their scanned slots
     for (uint32_t p = p0 + lane; p < p1; p += 64) {
-      const uint32_t rc = rawcnt[p];
-      if (!rc) continue;
+      const uint4 cp = cellp[p];
+      if (!cp.y) continue;
       const int qIdx = static_cast<int>(p - p0);
       uint32_t at = off[p] - b0;
-      for (uint32_t j = m0[p]; j < m0[p] + rc; ++j) {
-        const int2 e = a.epos[j];
+      for (uint32_t j = 0; j < cp.y; ++j) {
+        const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
         const int d = e.y - qIdx;
         if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
-        K[at] = (static_cast<uint64_t>(e.x) << 32) | static_cast<uint32_t>(d + d_off);
-        Q[at] = qIdx;
+        seg_put<PACKED>(L, w, at, pk, e.x, d, qIdx);
         ++at;
       }
     }
     uint32_t np = 1;
     while (np < n) np <<= 1;
     for (uint32_t i = n + lane; i < np; i += 64) {
-      K[i] = ~0ull;
-      Q[i] = INT_MAX;
+      L.k[w][i] = ~0ull;
+      if constexpr (!PACKED) L.q[w][i] = INT_MAX;
     }
     wave_lds_sync();
-    // bitonic sort of (key, qIdx) ascending
+    // bitonic sort ascending
     for (uint32_t k = 2; k <= np; k <<= 1) {
       for (uint32_t j = k >> 1; j > 0; j >>= 1) {
         for (uint32_t i = lane; i < np; i += 64) {
           const uint32_t x = i ^ j;
           if (x > i) {
-            const uint64_t ka = K[i], kb = K[x];
-            const int qa = Q[i], qb = Q[x];
-            const bool gt = ka > kb || (ka == kb && qa > qb);
+            const uint64_t ka = L.k[w][i], kb = L.k[w][x];
+            bool gt;
+            if constexpr (PACKED) {
+              gt = ka > kb;
+            } else {
+              gt = ka > kb || (ka == kb && L.q[w][i] > L.q[w][x]);
+            }
             if (gt == ((i & k) == 0)) {
-              K[i] = kb; K[x] = ka;
-              Q[i] = qb; Q[x] = qa;
+              L.k[w][i] = kb;
+              L.k[w][x] = ka;
+              if constexpr (!PACKED) {
+                const int qa = L.q[w][i];
+                L.q[w][i] = L.q[w][x];
+                L.q[w][x] = qa;
+              }
             }
           }
         }
@@ -472,21 +546,17 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       bool ghead = false, nr = false, kk = false;
       int gs = 0, d = 0, t = 0, q = 0;
       if (valid) {
-        const uint64_t key = K[i];
-        gs = static_cast<int>(key >> 32);
-        d = static_cast<int>(static_cast<uint32_t>(key)) - d_off;
-        q = Q[i];
+        seg_get<PACKED>(L, w, i, pk, gs, d, q);
         t = d + q;
         bool twhead = true;
         ghead = true;
         if (i > 0) {
-          const uint64_t pk = K[i - 1];
-          const int pgs = static_cast<int>(pk >> 32);
+          int pgs, pd, pq;
+          seg_get<PACKED>(L, w, i - 1, pk, pgs, pd, pq);
           if (pgs == gs) {
             ghead = false;
             twhead = false;
-            const int pd = static_cast<int>(static_cast<uint32_t>(pk)) - d_off;
-            nr = near_pair(t, d, pd + Q[i - 1], pd, a.width, a.gap);
+            nr = near_pair(t, d, pd + pq, pd, a.width, a.gap);
           } else {
             twhead = a.twin[pgs] != a.twin[gs];
           }
@@ -505,9 +575,9 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       if (valid && kk) {
         dd = true;
         if (prevk >= 0 && prevk >= gstart) {
-          const uint64_t pk = K[prevk];
-          const int pd = static_cast<int>(static_cast<uint32_t>(pk)) - d_off;
-          dd = !near_pair(t, d, pd + Q[prevk], pd, a.width, a.gap);
+          int kgs, kd, kq;
+          seg_get<PACKED>(L, w, static_cast<uint32_t>(prevk), pk, kgs, kd, kq);
+          dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
         }
       }
       const unsigned long long m = __ballot(dd);
@@ -519,8 +589,9 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         o.hq[at] = q;
       }
       nh += static_cast<uint32_t>(__popcll(m));
-      carry_k = __shfl(kin, 63, 64) > carry_k ? __shfl(kin, 63, 64) : carry_k;
-      carry_g = __shfl(gin, 63, 64) > carry_g ? __shfl(gin, 63, 64) : carry_g;
+      const int lk = __shfl(kin, 63, 64), lg = __shfl(gin, 63, 64);
+      carry_k = lk > carry_k ? lk : carry_k;
+      carry_g = lg > carry_g ? lg : carry_g;
     }
     if (lane == 0) o.hcnt[s] = nh;
     wave_lds_sync();
@@ -549,8 +620,7 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_counts(SeedArgs a, const 
 // Writes the surviving seeds of each listed position at its scanned offset,
 // in query-position then cell order (krnlWriteSeedList.cu:3-37 order).
 static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk, const uint8_t* __restrict__ ovf_flag,
-                                                              const uint32_t* __restrict__ m0,
-                                                              const uint32_t* __restrict__ rawcnt,
+                                                              const uint4* __restrict__ cellp,
                                                               const uint32_t* __restrict__ off,
                                                               uint64_t* __restrict__ skey,
                                                               uint32_t* __restrict__ sval) {
@@ -558,14 +628,13 @@ static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPa
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
   if (p >= a.npos || !ovf_flag[s]) return;
-  const uint32_t n = rawcnt[p];
-  if (n == 0) return;
+  const uint4 cp = cellp[p];
+  if (cp.y == 0) return;
   const int qIdx = static_cast<int>(p - a.pos_off[s]);
   const int qLen = a.qlen[s];
   uint32_t o = off[p];
-  const uint32_t lo = m0[p];
-  for (uint32_t j = lo; j < lo + n; ++j) {
-    const int2 e = a.epos[j];
+  for (uint32_t j = 0; j < cp.y; ++j) {
+    const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
     const int d = e.y - qIdx;
     if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
     skey[o] = (static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits)) | (static_cast<uint64_t>(e.x) << pk.d_bits) |
@@ -933,17 +1002,43 @@ static __global__ __launch_bounds__(kBlock) void k_dp_rows_key(uint32_t n, const
 
 // ---------------------------------------------------------------- results
 
+// Order key of one hit, compared lexicographically (a, b, c):
+//   a = (target window << 32) | ~score order     (score descending)
+//   b = ((0x7FFFFFFF - seed qIdx) << 1) | strand bit
+//   c = (target sequence << 32) | diagonal order
+// (a, b, c) is the CDeviceHitList order of one unit; (a.low, a.high, b, c)
+// is the row order fixResult leaves among the pair's rows.
+struct RowKey {
+  uint64_t a, b, c;
+};
+
+__device__ __forceinline__ bool unit_less(const RowKey& x, const RowKey& y) {
+  return x.a < y.a || (x.a == y.a && (x.b < y.b || (x.b == y.b && x.c < y.c)));
+}
+
+__device__ __forceinline__ uint64_t swap_halves(uint64_t v) { return (v << 32) | (v >> 32); }
+
+__device__ __forceinline__ bool row_less(const RowKey& x, const RowKey& y) {
+  const uint64_t xa = swap_halves(x.a), ya = swap_halves(y.a);
+  return xa < ya || (xa == ya && (x.b < y.b || (x.b == y.b && x.c < y.c)));
+}
+
 // calculateEvalue (krnlCalculateEvalue.cu:18-21) in FP64, the E-value cut
 // as an integer score threshold per strand (score >= smin[s] <=> E <= cutOff,
-// threshold computed on the host with the same formula), and the too-short
-// cut (…deleteBadHits.cuh:41-48).  Kept hits also widen the score range.
+// threshold computed on the host with the same formula), the too-short cut
+// (…deleteBadHits.cuh:41-48), and the hit's order key.
 static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const int32_t* __restrict__ hs,
-                                                       const int32_t* __restrict__ score,
-                                                       const int32_t* __restrict__ qhl,
-                                                       const int32_t* __restrict__ qlen,
-                                                       const int32_t* __restrict__ smin, int use_cut, int min_qhl,
-                                                       double kd, double lambda, double* __restrict__ ev,
-                                                       uint32_t* __restrict__ flag, int32_t* __restrict__ srange) {
+                                                              const int32_t* __restrict__ hgs,
+                                                              const int32_t* __restrict__ ht,
+                                                              const int32_t* __restrict__ hq,
+                                                              const int32_t* __restrict__ score,
+                                                              const int32_t* __restrict__ qhl,
+                                                              const int32_t* __restrict__ qlen,
+                                                              const int32_t* __restrict__ seq_win_t,
+                                                              const int32_t* __restrict__ smin, int use_cut,
+                                                              int min_qhl, double kd, double lambda,
+                                                              double* __restrict__ ev, uint32_t* __restrict__ flag,
+                                                              RowKey* __restrict__ rk) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int s = hs[i];
@@ -951,10 +1046,14 @@ static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const 
   ev[i] = kd * static_cast<double>(qlen[s]) * exp(-lambda * static_cast<double>(sc));
   const bool keep = (!use_cut || sc >= smin[s]) && !(min_qhl > qhl[i]);
   flag[i] = keep ? 1u : 0u;
-  if (keep) {
-    atomicMin(&srange[0], sc);
-    atomicMax(&srange[1], sc);
-  }
+  const int gs = hgs[i], q = hq[i];
+  RowKey k;
+  k.a = (static_cast<uint64_t>(static_cast<uint32_t>(seq_win_t[gs])) << 32) |
+        static_cast<uint64_t>(0x7FFFFFFFu - static_cast<uint32_t>(sc));
+  k.b = (static_cast<uint64_t>(0x7FFFFFFFu - static_cast<uint32_t>(q)) << 1) | static_cast<uint64_t>(s & 1);
+  k.c = (static_cast<uint64_t>(static_cast<uint32_t>(gs)) << 32) |
+        static_cast<uint64_t>(static_cast<uint32_t>(ht[i] - q) ^ 0x80000000u);
+  rk[i] = k;
 }
 
 // Order keys of the kept hits, least significant first: word0 = (target
@@ -1065,6 +1164,131 @@ static __global__ __launch_bounds__(kBlock) void k_gather_out(OutArgs a) {
   o[8 * n + i] = a.twin[gs];
   o[9 * n + i] = a.qwin[s];
   a.out_ev[i] = a.ev[r];
+}
+
+// ---------------------------------------------------------------- per-pair results
+
+// Pairs with up to kPairSmall hits are ordered by thread-per-hit ranking
+// inside the pair; larger pairs take the sort path.
+constexpr uint32_t kPairSmall = 256;
+
+// Hits of query pair k ('+' strand 2k, '-' strand 2k+1) are [hbase[2k],
+// pair_end) in the dense strand-ordered hit list.
+__device__ __forceinline__ uint32_t pair_end(const uint32_t* hbase, uint32_t nseq, uint32_t nhit, uint32_t k) {
+  return 2 * k + 2 < nseq ? hbase[2 * k + 2] : nhit;
+}
+
+struct PairArgs {
+  uint32_t nhit, nseq;
+  const uint32_t* hbase;   // per strand: first dense hit
+  const int32_t* hs;
+  const int32_t* hgs;
+  const int32_t* ot;       // extended tIdx / qIdx
+  const int32_t* oq;
+  const uint32_t* kflag;   // passed the E-value and length cuts
+  const RowKey* rk;
+};
+
+// deleteHits_duplicateResult (CDeviceHitList.cu:32-128) within one pair: a
+// kept hit is dropped when its predecessor in unit order is in the same
+// target window and equal in (tID, tIdx, qID, qIdx).  A row of another pair
+// never precedes it within a unit (pairs are contiguous in unit order) and
+// could not be equal.  Hits of pairs over kPairSmall go to the sort path
+// (gflag) and count in nbig once per pair.
+static __global__ __launch_bounds__(kBlock) void k_pair_dedup(PairArgs a, uint32_t* __restrict__ dflag,
+                                                              uint32_t* __restrict__ gflag,
+                                                              uint32_t* __restrict__ nbig) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nhit) return;
+  const uint32_t k = static_cast<uint32_t>(a.hs[i]) >> 1;
+  const uint32_t lo = a.hbase[2 * k], hi = pair_end(a.hbase, a.nseq, a.nhit, k);
+  const bool big = hi - lo > kPairSmall;
+  if (big && i == lo) atomicAdd(nbig, 1u);
+  const bool kept = a.kflag[i] != 0;
+  uint32_t f = 0;
+  if (kept && !big) {
+    const RowKey me = a.rk[i];
+    uint32_t pred = 0xFFFFFFFFu;
+    RowKey best{};
+    for (uint32_t j = lo; j < hi; ++j) {
+      if (!a.kflag[j]) continue;
+      const RowKey o = a.rk[j];
+      if (unit_less(o, me) && (pred == 0xFFFFFFFFu || unit_less(best, o))) {
+        best = o;
+        pred = j;
+      }
+    }
+    f = 1;
+    if (pred != 0xFFFFFFFFu && (best.a >> 32) == (me.a >> 32) && a.hgs[pred] == a.hgs[i] && a.ot[pred] == a.ot[i] &&
+        a.hs[pred] == a.hs[i] && a.oq[pred] == a.oq[i])
+      f = 0;
+  }
+  dflag[i] = f;
+  gflag[i] = kept && big ? 1u : 0u;
+}
+
+// Row slot of each surviving hit of a small pair: the pair's first output
+// row (exclusive scan of the per-hit row counts at the pair's first hit)
+// plus the hit's rank among the pair's survivors in row order.
+static __global__ __launch_bounds__(kBlock) void k_pair_place(PairArgs a, const uint32_t* __restrict__ dflag,
+                                                              const uint32_t* __restrict__ doff,
+                                                              uint32_t* __restrict__ order) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nhit) return;
+  const uint32_t k = static_cast<uint32_t>(a.hs[i]) >> 1;
+  const uint32_t lo = a.hbase[2 * k], hi = pair_end(a.hbase, a.nseq, a.nhit, k);
+  if (hi - lo > kPairSmall || !dflag[i]) return;
+  const RowKey me = a.rk[i];
+  uint32_t rank = 0;
+  for (uint32_t j = lo; j < hi; ++j)
+    if (dflag[j] && row_less(a.rk[j], me)) ++rank;
+  order[doff[lo] + rank] = i;
+}
+
+// Rows of the large pairs from the sort path (pair-major, row order within
+// each pair): each pair's row count lands on its first hit's slot of the
+// per-hit count array, and its first row index in gfirst at the same slot.
+static __global__ __launch_bounds__(kBlock) void k_big_counts(uint32_t n, const uint32_t* __restrict__ grow,
+                                                              const int32_t* __restrict__ hs,
+                                                              const uint32_t* __restrict__ hbase,
+                                                              uint32_t* __restrict__ rows, uint32_t* __restrict__ gfirst) {
+  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t k = static_cast<uint32_t>(hs[grow[r]]) >> 1;
+  const uint32_t lo = hbase[2 * k];
+  if (r == 0 || (static_cast<uint32_t>(hs[grow[r - 1]]) >> 1) != k) gfirst[lo] = r;
+  atomicAdd(&rows[lo], 1u);
+}
+
+static __global__ __launch_bounds__(kBlock) void k_big_place(uint32_t n, const uint32_t* __restrict__ grow,
+                                                             const int32_t* __restrict__ hs,
+                                                             const uint32_t* __restrict__ hbase,
+                                                             const uint32_t* __restrict__ gfirst,
+                                                             const uint32_t* __restrict__ doff,
+                                                             uint32_t* __restrict__ order) {
+  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t lo = hbase[static_cast<uint32_t>(hs[grow[r]]) & ~1u];
+  order[doff[lo] + (r - gfirst[lo])] = grow[r];
+}
+
+// Score range of the hits sent to the sort path (its key width).
+static __global__ __launch_bounds__(kBlock) void k_score_range(uint32_t n, const uint32_t* __restrict__ flag,
+                                                               const int32_t* __restrict__ score,
+                                                               int32_t* __restrict__ srange) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  int lo = INT32_MAX, hi = INT32_MIN;
+  if (i < n && flag[i]) lo = hi = score[i];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int a = __shfl_xor(lo, d, 64), b = __shfl_xor(hi, d, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (lane_id() == 0 && lo <= hi) {
+    atomicMin(&srange[0], lo);
+    atomicMax(&srange[1], hi);
+  }
 }
 
 // resultSorting key for clast_result_sort: (qID/2, -score) biased to u64.

@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         L.oracle_copy.restype = C.c_int32
         L.oracle_copy.argtypes = [vp, C.c_int32] + [vp] * 10
         L.oracle_free.argtypes = [vp]
+        L.oracle_times.argtypes = [vp, vp]
         L.oracle_index_build.restype = vp
         L.oracle_index_build.argtypes = [C.POINTER(OParams), C.POINTER(OList), C.c_int32, C.c_int32]
         L.oracle_index_cells.restype = C.c_int64
@@ -91,6 +92,9 @@ def search(params, tlist, twins, qlist, qwins, dump: bool = False) -> dict:
                         qw.ctypes.data, len(qwins), int(dump))
     try:
         out = {"final": _copy(L, h, FINAL, hits=True)}
+        tm = np.zeros(2, np.float64)
+        L.oracle_times(h, tm.ctypes.data)
+        out["index_s"], out["search_s"] = float(tm[0]), float(tm[1])
         if dump:
             for name, st in (("raw", RAW), ("boundary", BOUNDARY), ("sorted", SORTED), ("isolate", ISOLATE),
                              ("duplicate", DUPLICATE)):

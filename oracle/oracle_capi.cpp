@@ -2,6 +2,7 @@
 // CPU-baseline leg of bench.py — TEST INFRASTRUCTURE ONLY, never linked by
 // the product.
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -36,6 +37,8 @@ struct oracle_result {
   Hits hits;               // holder order (fixResult) over all units
   std::vector<int> twin;   // target window index of each final hit
   StageDump dump;          // single-unit runs only
+  double index_s = 0;      // wall time building the target windows' tables
+  double search_s = 0;     // wall time of the unit searches
 };
 
 static Params to_params(const oracle_params* p) {
@@ -75,8 +78,11 @@ void* oracle_search(const oracle_params* pp, const oracle_list* tl, const int32_
   const SeqList t = to_list(tl), q = to_list(ql);
   auto* r = new oracle_result();
   std::vector<int> order_tw;
+  using clk = std::chrono::steady_clock;
   for (int a = 0; a < ntw; ++a) {
+    const auto t0 = clk::now();
     HashTable h = build_hash_table(p, t, twin[2 * a], twin[2 * a + 1]);
+    const auto t1 = clk::now();
     std::vector<Hits> res(nqw);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int b = 0; b < nqw; ++b) {
@@ -84,6 +90,8 @@ void* oracle_search(const oracle_params* pp, const oracle_list* tl, const int32_
       res[b] = search(p, h, qw, twin[2 * a], qwin[2 * b] * 2,
                       (dump && ntw == 1 && nqw == 1) ? &r->dump : nullptr);
     }
+    r->index_s += std::chrono::duration<double>(t1 - t0).count();
+    r->search_s += std::chrono::duration<double>(clk::now() - t1).count();
     for (int b = 0; b < nqw; ++b) {
       r->hits.append(res[b]);
       order_tw.insert(order_tw.end(), res[b].size(), a);
@@ -160,6 +168,12 @@ int32_t oracle_copy(void* h, int32_t stage, int32_t* tID, int32_t* tIdx, int32_t
 }
 
 void oracle_free(void* h) { delete static_cast<oracle_result*>(h); }
+
+// Wall seconds of oracle_search: [0] table builds, [1] unit searches.
+void oracle_times(void* h, double* out) {
+  out[0] = static_cast<oracle_result*>(h)->index_s;
+  out[1] = static_cast<oracle_result*>(h)->search_s;
+}
 
 // The reference index of one target window (CDeviceHashTable members).
 struct oracle_index {
