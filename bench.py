@@ -122,7 +122,19 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--index", choices=("merged", "per-chunk"), default="merged",
                     help="one resident index over every target window, or one per target chunk")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only copy MiB of device memory once (the known-size kernel that scales the HBM "
+                         "traffic counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} MiB")
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (the search shards by query "
               "with no exchange; see DESIGN.md).", file=sys.stderr)

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -34,6 +35,13 @@ Runtime& runtime() {
   Runtime* rt = new Runtime(dev);
   g_rt[dev] = rt;
   return *rt;
+}
+
+// Byte arrays the DP streams read as 16-byte blocks: whole blocks, at least one.
+static uint64_t padded_blocks(uint64_t bytes) {
+  const uint64_t b = (bytes + 15) / 16;
+  if (b >= (1ull << 31)) throw ClastError(CLAST_ERR_LIMIT, "sequence set over 32 GiB");
+  return b ? b : 1;
 }
 
 static int bits_for(uint64_t count) {  // bits to hold values 0 .. count-1
@@ -96,6 +104,8 @@ struct clast_index_s {
   int nwin = 0, nseq = 0, max_tlen = 0;
   bool key64 = false;
   int pbits = 0, shift = 0;
+  int32_t bmax = 0;  // last 16-byte block of tbase
+  bool direct = false;  // cell lookup form (see use_direct)
   uint32_t nentries = 0;
   DevArray<uint8_t> tbase;
   DevArray<uint64_t> win_slot;
@@ -118,6 +128,7 @@ struct clast_query_s {
   int nwin = 0;
   uint32_t nseq = 0, npos = 0;
   int max_qlen = 0, min_qlen = 0;
+  int32_t bmax = 0;  // last 16-byte block of qbase
   DevArray<uint8_t> qbase;
   DevArray<uint32_t> qslot, pos_off;
   DevArray<int32_t> qlen, qref, qwin, smin;
@@ -159,18 +170,40 @@ void set_device(int dev) { hip_check(hipSetDevice(dev), "hipSetDevice"); }
 
 // ---------------------------------------------------------------- index
 
-// Presence bitmap, rank directory and cell offsets over the top B key bits
-// of the sorted entries (keys in ek, u64); keeps the keys as u32 or u64.
+// Form of a new index's cell lookup: CLAST_INDEX_FORM=direct|bitmap forces
+// one, otherwise the direct table is used when it is at most 16 GiB and the
+// index has at least one entry per 16 prefixes.
+static bool use_direct(int B, uint32_t nent) {
+  const uint64_t prefixes = 1ull << B;
+  if (const char* f = std::getenv("CLAST_INDEX_FORM")) {
+    if (!std::strcmp(f, "direct")) return true;
+    if (!std::strcmp(f, "bitmap")) return false;
+    if (std::strcmp(f, "auto")) throw ClastError(CLAST_ERR_ARG, "CLAST_INDEX_FORM must be direct, bitmap or auto");
+  }
+  return prefixes * sizeof(uint4) <= (16ull << 30) && static_cast<uint64_t>(nent) * 16 >= prefixes;
+}
+
+// Cell lookup over the top B key bits of the sorted entries (keys in ek,
+// u64): presence bitmap + rank directory + a record per present prefix, or a
+// direct record table; keeps the keys as u32 or u64.
 void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t nent, int keybits) {
   hipStream_t st = rt.stream();
   Workspace& ws = rt.ws();
   const int B = std::min(keybits, 30);
   ix.pbits = B;
   ix.shift = keybits - B;
+  ix.direct = use_direct(B, nent);
   const uint64_t nwords = std::max<uint64_t>(1, (1ull << B) / 64);
   const uint64_t nblk = (nwords + 7) / 8;
-  ix.bitmap.alloc(nwords);
-  k_fill<unsigned long long><<<grid_for(nwords, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, 0ull);
+  if (ix.direct) {
+    ix.bitmap.alloc(1);
+    ix.rank.alloc(1);
+    ix.cells.alloc(1ull << B);
+    k_fill<uint4><<<grid_for(1ull << B, kBlock), kBlock, 0, st>>>(ix.cells.ptr, 1ull << B, make_uint4(0, 0, 0, 0));
+  } else {
+    ix.bitmap.alloc(nwords);
+    k_fill<unsigned long long><<<grid_for(nwords, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, 0ull);
+  }
   uint32_t ncell = 0;
   if (nent > 0) {
     uint32_t* head = ws.get<uint32_t>("ix.chead", nent);
@@ -180,9 +213,11 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, head, nent, cell, scal, false);
     rt.read_scalars(scal, &ncell, 1);
     uint32_t* cell_off = ws.get<uint32_t>("ix.coff", static_cast<uint64_t>(ncell) + 1);
-    k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, cell_off, ix.bitmap.ptr);
-    ix.cells.alloc(ncell);
-    k_cell_records<<<grid_for(ncell, kBlock), kBlock, 0, st>>>(cell_off, ncell, ix.epos.ptr, ix.cells.ptr);
+    k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, cell_off,
+                                                             ix.direct ? nullptr : ix.bitmap.ptr);
+    if (!ix.direct) ix.cells.alloc(ncell);
+    k_cell_records<<<grid_for(ncell, kBlock), kBlock, 0, st>>>(cell_off, ncell, ix.epos.ptr, ek, ix.shift,
+                                                                ix.direct ? 1 : 0, ix.cells.ptr);
     rt.check_launch("k_cell_build");
     if (ix.key64) {
       ix.ekey64.alloc(nent);
@@ -192,16 +227,21 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
       k_narrow_keys<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.ekey32.ptr);
     }
   } else {
-    ix.cells.alloc(1);
-    k_fill<uint4><<<1, 64, 0, st>>>(ix.cells.ptr, 1, make_uint4(0, 0, 0, 0));
+    if (!ix.direct) {
+      ix.cells.alloc(1);
+      k_fill<uint4><<<1, 64, 0, st>>>(ix.cells.ptr, 1, make_uint4(0, 0, 0, 0));
+    }
     ix.ekey32.alloc(1);
     ix.ekey64.alloc(1);
   }
   ix.ncell = ncell;
-  uint32_t* bc = ws.get<uint32_t>("ix.bcnt", nblk);
-  ix.rank.alloc(nblk);
-  k_rank_count<<<grid_for(nblk, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, nblk, bc);
-  device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, bc, nblk, ix.rank.ptr, static_cast<uint32_t*>(nullptr), false);
+  if (!ix.direct) {
+    uint32_t* bc = ws.get<uint32_t>("ix.bcnt", nblk);
+    ix.rank.alloc(nblk);
+    k_rank_count<<<grid_for(nblk, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, nblk, bc);
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, bc, nblk, ix.rank.ptr, static_cast<uint32_t*>(nullptr),
+                                                     false);
+  }
   rt.check_launch("rank");
 }
 
@@ -240,7 +280,8 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
   ix.nseq = static_cast<int>(seq_slot.size());
   ix.h_win_slot = win_slot;
   // a sequence's slot is never past the next sequence's; windows are in order
-  ix.tbase.alloc(win_slot[nwin]);
+  ix.tbase.alloc(padded_blocks(win_slot[nwin]) * 16);
+  ix.bmax = static_cast<int32_t>(padded_blocks(win_slot[nwin]) - 1);
   for (int w = 0; w < nwin; ++w)
     if (wins[w].n_slots)
       hip_check(hipMemcpyAsync(ix.tbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
@@ -335,7 +376,8 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
   q.nseq = static_cast<uint32_t>(qslot.size());
   q.npos = static_cast<uint32_t>(npos);
   q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
-  q.qbase.alloc(win_slot[nwin]);
+  q.qbase.alloc(padded_blocks(win_slot[nwin]) * 16);
+  q.bmax = static_cast<int32_t>(padded_blocks(win_slot[nwin]) - 1);
   for (int w = 0; w < nwin; ++w)
     if (wins[w].n_slots)
       hip_check(hipMemcpyAsync(q.qbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
@@ -446,7 +488,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   a.npos = np;
   a.lk = Lookup{ix.bitmap.ptr, ix.rank.ptr, ix.cells.ptr,
                 ix.key64 ? static_cast<const void*>(ix.ekey64.ptr) : static_cast<const void*>(ix.ekey32.ptr),
-                ix.epos.ptr, ix.shift, ix.key64 ? 1 : 0};
+                ix.epos.ptr, ix.shift, ix.key64 ? 1 : 0, ix.direct ? 1 : 0};
   a.epos = ix.epos.ptr;
   a.tlen = ix.seq_len.ptr;
   a.twin = ix.seq_win.ptr;
@@ -629,7 +671,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
                        qb + bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
     }
     DPArgs da{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, p.lmer, ot, oq, thl, qhl, om, osc};
+              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, q.bmax, ix.bmax, p.lmer, ot, oq, thl, qhl,
+              om, osc};
     mark(EV_DP0);
     if (p.local) launch_dp_local(p.gap, da, st);
     else launch_dp_global(p.gap, da, st);

@@ -217,19 +217,24 @@ static __global__ __launch_bounds__(kBlock) void k_cell_build(const uint64_t* __
   if (head[e]) {
     const uint64_t h = keys[e] >> shift;
     cell_off[cell[e]] = e;
-    atomicOr(&bitmap[h >> 6], 1ull << (h & 63));
+    if (bitmap) atomicOr(&bitmap[h >> 6], 1ull << (h & 63));
   }
   if (e + 1 == n) cell_off[cell[e] + head[e]] = n;
 }
 
+// Record of the r-th present prefix, stored at r (bitmap form) or at the
+// prefix value itself (direct form: a table over every prefix, zero count
+// where absent, so a lookup is one record read).
 static __global__ __launch_bounds__(kBlock) void k_cell_records(const uint32_t* __restrict__ cell_off, uint32_t ncell,
                                                                 const int2* __restrict__ epos,
-                                                                uint4* __restrict__ rec) {
+                                                                const uint64_t* __restrict__ keys, int shift,
+                                                                int direct, uint4* __restrict__ rec) {
   const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= ncell) return;
   const uint32_t lo = cell_off[r];
   const int2 e = epos[lo];
-  rec[r] = make_uint4(lo, cell_off[r + 1] - lo, static_cast<uint32_t>(e.x), static_cast<uint32_t>(e.y));
+  const uint64_t at = direct ? keys[lo] >> shift : r;
+  rec[at] = make_uint4(lo, cell_off[r + 1] - lo, static_cast<uint32_t>(e.x), static_cast<uint32_t>(e.y));
 }
 
 static __global__ __launch_bounds__(kBlock) void k_rank_count(const unsigned long long* __restrict__ bitmap,
@@ -250,6 +255,7 @@ struct Lookup {
   const int2* epos;
   int shift;          // 2 lMer - B
   int key64;
+  int direct;         // cells is indexed by the prefix itself (no bitmap)
 };
 
 // Entries of a valid query k-mer (krnlBinarySearch.cu:3-33 + the cell
@@ -257,16 +263,22 @@ struct Lookup {
 // n > 0, [lo, lo + n) is their range and first the first entry.
 __device__ __forceinline__ uint32_t lookup(const Lookup& L, uint64_t key, uint32_t& lo, int2& first) {
   const uint64_t h = key >> L.shift;
-  const uint64_t w = h >> 6;
-  const unsigned long long word = L.bitmap[w];
-  const int bit = static_cast<int>(h & 63);
   lo = 0;
   first = make_int2(0, 0);
-  if (!((word >> bit) & 1ull)) return 0;
-  uint32_t r = L.rank[w >> 3];
-  for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
-  r += static_cast<uint32_t>(__popcll(word & ((1ull << bit) - 1ull)));
-  const uint4 c = L.cells[r];
+  uint4 c;
+  if (L.direct) {
+    c = L.cells[h];
+    if (c.y == 0) return 0;
+  } else {
+    const uint64_t w = h >> 6;
+    const unsigned long long word = L.bitmap[w];
+    const int bit = static_cast<int>(h & 63);
+    if (!((word >> bit) & 1ull)) return 0;
+    uint32_t r = L.rank[w >> 3];
+    for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
+    r += static_cast<uint32_t>(__popcll(word & ((1ull << bit) - 1ull)));
+    c = L.cells[r];
+  }
   lo = c.x;
   uint32_t n = c.y;
   first = make_int2(static_cast<int>(c.z), static_cast<int>(c.w));
@@ -801,6 +813,7 @@ struct DPArgs {
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
   const uint32_t* order;   // optional: hit processed by thread i (length buckets)
+  int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
   int lmer;
   int32_t* o_tidx;
   int32_t* o_qidx;
@@ -812,16 +825,50 @@ struct DPArgs {
 
 constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
 
-// Target base class at intra-sequence index ti of a sequence whose class
-// bytes start at `seq`.  [lo, hi) is the valid range: the window's slots
-// relative to the sequence, cut by the direction's one-sided guard —
-// backward ti < tLen, forward ti >= 0 (krnlAlignment.cu:121-127,160-165 /
-// 234-240,271-276).  Anything outside is 'X'.  The load always stays in the
-// window (an invalid index reads the window's first slot and is discarded).
-__device__ __forceinline__ uint32_t t_class(const uint8_t* seq, int ti, int lo, int hi) {
-  const bool ok = ti >= lo && ti < hi;
-  const uint32_t v = seq[ok ? ti : lo];
-  return ok ? v : kClassX;
+// Sequential reader of one DP lane's bytes: indexes g, g + step, g + 2 step,
+// ... of a device byte array padded to whole 16-byte blocks, read a block
+// at a time.  Entering a block requests the one after it, so a load is in
+// flight 16 rows ahead, and one load serves 16 rows: the 64 scattered
+// target and query lines of a wavefront are fetched once each instead of
+// once per row (per-row byte loads thrash L2 once enough wavefronts are
+// resident).  Blocks outside the array are clamped; their bytes are never
+// used.
+struct ByteStream {
+  const uint4* blk;
+  int64_t g;
+  int32_t cb, bmax, step;
+  uint4 cur, nxt;
+
+  __device__ __forceinline__ void init(const uint8_t* base, int32_t last_block, int64_t g0, int st) {
+    blk = reinterpret_cast<const uint4*>(base);
+    bmax = last_block;
+    step = st;
+    g = g0;
+    cb = static_cast<int32_t>(g0 >> 4);
+    cur = blk[min(max(cb, 0), bmax)];
+    nxt = blk[min(max(cb + step, 0), bmax)];
+  }
+  __device__ __forceinline__ uint32_t next() {
+    const int32_t b = static_cast<int32_t>(g >> 4);
+    if (b != cb) {
+      cb = b;
+      cur = nxt;
+      nxt = blk[min(max(b + step, 0), bmax)];
+    }
+    const uint32_t sel = static_cast<uint32_t>(g >> 2) & 3u;
+    const uint32_t word = sel == 0 ? cur.x : sel == 1 ? cur.y : sel == 2 ? cur.z : cur.w;
+    const uint32_t v = (word >> ((static_cast<uint32_t>(g) & 3u) * 8u)) & 0xFFu;
+    g += step;
+    return v;
+  }
+};
+
+// Target base class at intra-sequence index ti: [lo, hi) is the valid range,
+// the window's slots relative to the sequence cut by the direction's
+// one-sided guard — backward ti < tLen, forward ti >= 0 (krnlAlignment.cu:
+// 121-127,160-165 / 234-240,271-276).  Anything outside is 'X'.
+__device__ __forceinline__ uint32_t t_mask(uint32_t v, int ti, int lo, int hi) {
+  return ti >= lo && ti < hi ? v : kClassX;
 }
 
 // One query row of the band (calcutlateCell, krnlAlignment.cu:42-77, for
@@ -892,8 +939,7 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
 // tIdx/qIdx, which the backward pass does not change), so one thread runs
 // both and sums the results.  Hits arrive ordered by (read length, seed
 // qIdx), so the lanes of a wavefront run near-equal row counts in both
-// loops.  The next row's query and target bytes are loaded before the
-// current row is computed.
+// loops.  Query and target bytes arrive through 16-byte block streams.
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
@@ -903,16 +949,15 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
   const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
   const int L = a.lmer;
   const int qLen = a.qlen[s];
-  const uint8_t* qb = a.qbase + a.qslot[s];
   const int tLen = a.tlen[gs];
   const uint64_t tslot = a.tslot[gs];
   const int w = a.twin[gs];
-  const uint8_t* tseq = a.tbase + tslot;
   const int wlo = static_cast<int>(static_cast<int64_t>(a.wslot[w]) - static_cast<int64_t>(tslot));
   const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - static_cast<int64_t>(tslot));
   int S[W + 2], V[W + 2], M[W + 2];
   uint32_t T[W];
   int oT = tIdx, oQ = qIdx, tHL = L, qHL = L, mm = L, sc = L * kMatch;
+  const int64_t qg = static_cast<int64_t>(a.qslot[s]);
 
 #pragma unroll 1
   for (int dir = 0; dir < 2; ++dir) {
@@ -920,30 +965,27 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
     const int rows = bwd ? qLen - (qIdx + L) : qIdx;
     if (rows == 0) continue;
     dp_init<G>(S, V, M);
-    // ring slot k holds target index tb + ts*k; row r adds tn0 + ts*r
+    // target index tb + ts*j feeds ring slot j (j < W) and then row j - W
     const int ts = bwd ? 1 : -1;
     const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
-    const int tn0 = bwd ? tIdx + L + G + 1 : tIdx - G - 2;
     const int lo = bwd ? wlo : max(wlo, 0);
     const int hi = bwd ? min(whi, tLen) : whi;
+    ByteStream tst, qst;
+    tst.init(a.tbase, a.t_bmax, static_cast<int64_t>(tslot) + tb, ts);
+    qst.init(a.qbase, a.q_bmax, qg + (bwd ? qIdx + L : qIdx - 1), ts);
 #pragma unroll
-    for (int k = 0; k < W; ++k) T[k] = t_class(tseq, tb + ts * k, lo, hi);
+    for (int k = 0; k < W; ++k) T[k] = t_mask(tst.next(), tb + ts * k, lo, hi);
     int bestS = 0, bestM = 0, bestR = -1, bestB = G;
-    const int q0 = bwd ? qIdx + L : qIdx - 1;
-    uint32_t qc = qb[q0];
-    uint32_t tn = t_class(tseq, tn0, lo, hi);
+    int tn_i = tb + ts * W;
 #pragma unroll 1
     for (int r = 0; r < rows; ++r) {
-      // next row's bytes first (the last row re-reads its own, harmlessly)
-      const int rn = r + 1 < rows ? r + 1 : r;
-      const uint32_t qc_n = qb[q0 + ts * rn];
-      const uint32_t tn_n = t_class(tseq, tn0 + ts * rn, lo, hi);
+      const uint32_t qc = qst.next();
+      const uint32_t tn = t_mask(tst.next(), tn_i, lo, hi);
+      tn_i += ts;
       dp_row<W, LOCAL>(S, V, M, T, qc, r, bestS, bestM, bestR, bestB);
 #pragma unroll
       for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
       T[W - 1] = tn;
-      qc = qc_n;
-      tn = tn_n;
     }
     int move, endp;
     if (LOCAL) {

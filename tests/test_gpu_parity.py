@@ -198,3 +198,25 @@ def test_long_reads_mix_fast_and_global_paths():
         assert got.times()[7] > 3000  # seeds
         o = oracle.search(p, tl, [(0, 3)], ql, [(0, 46), (46, 92)])
         assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
+
+
+@pytest.mark.parametrize("form", ["direct", "bitmap"])
+@pytest.mark.parametrize("lmer", [12, 15, 17])
+def test_index_forms(form, lmer, monkeypatch):
+    # both cell-lookup forms (direct prefix table / bitmap + rank), with and
+    # without prefix narrowing (lMer 17 keeps 30 of 34 key bits)
+    monkeypatch.setenv("CLAST_INDEX_FORM", form)
+    t, r = workload(n_t=6, t_len=20_000, n_r=300, repeat_frac=0.15, indel=0.02)
+    p = native.Params(total_db_size=db_size(t), lmer=lmer)
+    tl, ql = lists(t, r, lmer)
+    twins = native.windows(tl.gateway, 50_000, query=False)
+    qwins = native.windows(ql.gateway, 30_000, query=True)
+    print(f"index form {form} lmer={lmer}: {len(twins)}x{len(qwins)} windows", flush=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    got = native.search_batch(p, ix, q).arrays()
+    ix.close()
+    q.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(got, o["final"], what=f"form {form} lmer {lmer}")
