@@ -26,6 +26,11 @@ namespace clast {
 static std::mutex g_rt_mu;
 static std::map<int, Runtime*> g_rt;
 
+PinnedPool& pinned_pool() {
+  static PinnedPool* pool = new PinnedPool();
+  return *pool;
+}
+
 Runtime& runtime() {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -130,7 +135,7 @@ struct clast_query_s {
   int max_qlen = 0, min_qlen = 0;
   int32_t bmax = 0;  // last 16-byte block of qbase
   DevArray<uint8_t> qbase;
-  DevArray<uint32_t> qslot, pos_off;
+  DevArray<uint32_t> qslot, pos_off, pos_strand;
   DevArray<int32_t> qlen, qref, qwin, smin;
   std::vector<int32_t> h_qlen, h_qref, h_qwin, h_id_begin;
   bool smin_valid = false;
@@ -138,8 +143,15 @@ struct clast_query_s {
 };
 
 struct clast_hits_s {
-  std::vector<int32_t> col[10];  // tID tIdx qID qIdx tHL qHL match score twin qwin
-  std::vector<double> ev;
+  // n rows in page-locked memory: 10 int32 planes (tID tIdx qID qIdx tHL
+  // qHL match score twin qwin), then n E-values
+  clast::PinnedPool::Block rows;
+  int64_t n = 0;
+  const int32_t* col(int c) const { return static_cast<const int32_t*>(rows.ptr) + static_cast<int64_t>(c) * n; }
+  const double* ev() const {
+    return reinterpret_cast<const double*>(static_cast<const int32_t*>(rows.ptr) + 10 * n);
+  }
+  ~clast_hits_s() { clast::pinned_pool().release(rows); }
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -386,6 +398,8 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
     k_classify<<<grid_for(win_slot[nwin], kBlock), kBlock, 0, st>>>(q.qbase.ptr, win_slot[nwin]);
   q.qslot.upload(qslot.data(), qslot.size(), st);
   q.pos_off.upload(pos_off.data(), pos_off.size(), st);
+  q.pos_strand.alloc(std::max<uint64_t>(npos, 1));
+  if (q.nseq) k_pos_strand<<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.pos_off.ptr, q.nseq, q.pos_strand.ptr);
   q.qlen.upload(q.h_qlen.data(), q.h_qlen.size(), st);
   q.qref.upload(q.h_qref.data(), q.h_qref.size(), st);
   q.qwin.upload(q.h_qwin.data(), q.h_qwin.size(), st);
@@ -470,7 +484,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
 
   const uint32_t np = q.npos;
   const int sbits = bits_for(q.nseq), gbits = bits_for(static_cast<uint64_t>(ix.nseq));
-  const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 1);
+  // diagonal + d_off in [0, max_tlen + max_qlen]; one code above stays unused
+  const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 2);
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
   uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 2);
@@ -484,6 +499,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   a.qlen = q.qlen.ptr;
   a.qwin = q.qwin.ptr;
   a.pos_off = q.pos_off.ptr;
+  a.pos_strand = q.pos_strand.ptr;
   a.nseq = q.nseq;
   a.npos = np;
   a.lk = Lookup{ix.bitmap.ptr, ix.rank.ptr, ix.cells.ptr,
@@ -499,27 +515,28 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   a.local = p.local;
   a.unit_first = unit_first;
 
-  // lookup + count
+  // lookup + count: raw seed count per strand, strands' seed ranges
   uint32_t nseed = 0;
   uint4* cellp = nullptr;
-  uint32_t *cnt = nullptr, *off = nullptr;
+  uint32_t* scnt = ws.get<uint32_t>("s.scnt", q.nseq);
+  uint32_t* soff = ws.get<uint32_t>("s.soff", q.nseq);
   if (np > 0 && ix.nentries > 0) {
     cellp = ws.get<uint4>("s.cell", np);
-    cnt = ws.get<uint32_t>("s.cnt", np);
-    off = ws.get<uint32_t>("s.off", np);
+    k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(scnt, q.nseq, 0u);
     mark(EV_COUNT0);
-    k_seed_count<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, cellp, cnt);
+    k_seed_count<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, cellp, scnt);
     mark(EV_COUNT1);
     rt.check_launch("k_seed_count");
-    device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt, np, off, tot64, false);
+    device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, scnt, q.nseq, soff, tot64, false);
     uint64_t total = 0;
     rt.read_scalars(tot64, &total, 1);
     if (total >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
     nseed = static_cast<uint32_t>(total);
   }
 
-  // per-strand sort + isolate/duplicate, global path for the rest; hits are
-  // first written into their strand's seed range, then compacted in strand order
+  // per-strand sort + isolate/duplicate in LDS, the global path for strands
+  // over kSegCap seeds; hits are first written into their strand's seed
+  // range, then compacted in strand order
   uint32_t nhit = 0;
   const uint64_t cap = std::max<uint64_t>(nseed, 1);
   int32_t* hs = ws.get<int32_t>("h.s", cap);
@@ -537,19 +554,20 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint8_t* ovf_flag = ws.get<uint8_t>("s.ovfflag", q.nseq);
     uint32_t novf = 0;
     k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
+    const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
+    const bool packed = gbits + dbits + qbits <= 63;
+    const SegPack sp{d_off, qbits, dbits};
+    const SegOut so{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag};
+    auto segments = [&](uint32_t nwaves, const uint32_t* list, const uint32_t* nlist) {
+      const uint32_t grid = std::min<uint32_t>(grid_for(nwaves, kSegWaves), 1u << 20);
+      if (packed) k_seed_segments<true><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist);
+      else k_seed_segments<false><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist);
+      rt.check_launch("k_seed_segments");
+    };
+    mark(EV_WRITE0);
     if (!keep_stages) {
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
-      const uint32_t grid = std::min<uint32_t>(grid_for(q.nseq, kSegWaves), 1u << 20);
-      mark(EV_WRITE0);
-      const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
-      const SegPack sp{d_off, qbits, dbits};
-      const SegOut so{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag};
-      if (gbits + dbits + qbits <= 64)
-        k_seed_segments<true><<<grid, kBlock, 0, st>>>(a, cellp, off, nseed, sp, so);
-      else
-        k_seed_segments<false><<<grid, kBlock, 0, st>>>(a, cellp, off, nseed, sp, so);
-      mark(EV_WRITE1);
-      rt.check_launch("k_seed_segments");
+      segments(q.nseq, nullptr, nullptr);
       rt.read_scalars(tot32 + 1, &novf, 1);
     } else {
       // every strand on the global path: list them all
@@ -558,21 +576,37 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(hcnt, q.nseq, 0u);
       novf = q.nseq;
     }
+    SeedPack pk{gbits, dbits, d_off};
+    SeedDecode dc{gbits, dbits, d_off};
+    uint32_t ng = 0;
+    uint32_t* off2 = nullptr;
+    uint64_t* skey = nullptr;
+    uint32_t* sval = nullptr;
     if (novf > 0) {
       if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
-      SeedPack pk{gbits, dbits, d_off};
-      SeedDecode dc{gbits, dbits, d_off};
       uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", np);
-      uint32_t* off2 = ws.get<uint32_t>("g.off", np);
-      k_ovf_counts<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, ovf_flag, cnt, cnt2);
+      off2 = ws.get<uint32_t>("g.off", np);
+      k_ovf_counts<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, ovf_flag, cellp, cnt2);
       device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, np, off2, tot64 + 1, false);
       uint64_t t2 = 0;
       rt.read_scalars(tot64 + 1, &t2, 1);
-      const uint32_t ng = static_cast<uint32_t>(t2);
-      uint64_t* skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
-      uint32_t* sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
+      ng = static_cast<uint32_t>(t2);
+      skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
+      sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
       k_seed_write<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, ovf_flag, cellp, off2, skey, sval);
       rt.check_launch("k_seed_write");
+    }
+    if (!keep_stages) {
+      // unit_first is final: run each unit's first strand again with its element 0
+      uint32_t* fmark = ws.get<uint32_t>("s.fmark", q.nseq);
+      uint32_t* flist = ws.get<uint32_t>("s.flist", nunits);
+      k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(fmark, q.nseq, 0u);
+      k_fix_list<<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, static_cast<uint32_t>(nunits), ovf_flag,
+                                                              fmark, flist, tot32 + 5);
+      segments(static_cast<uint32_t>(nunits), flist, tot32 + 5);
+    }
+    mark(EV_WRITE1);
+    if (novf > 0) {
       auto dump_seeds = [&](int slot) {
         std::vector<uint64_t> hk;
         std::vector<uint32_t> hv;
@@ -580,18 +614,20 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         d2h(hv, sval, ng, st);
         rt.sync();
         auto& S = out.stage[slot];
-        for (int c = 0; c < 4; ++c) S[c].resize(ng);
+        for (int c = 0; c < 4; ++c) S[c].clear();
+        const uint64_t low = (1ull << (gbits + dbits)) - 1ull;
         for (uint32_t i = 0; i < ng; ++i) {
           const uint64_t k = hk[i];
+          if ((k & low) == low) continue;  // removed by the corner filter
           const uint32_t s = static_cast<uint32_t>(k >> (gbits + dbits));
           const uint32_t gs = static_cast<uint32_t>((k >> dbits) & ((1ull << gbits) - 1ull));
           const int d = static_cast<int>(k & ((1ull << dbits) - 1ull)) - d_off;
-          S[0][i] = ix.h_seq_ref[gs];
-          S[1][i] = d + static_cast<int>(hv[i]);
-          S[2][i] = q.h_qref[s];
-          S[3][i] = static_cast<int>(hv[i]);
+          S[0].push_back(ix.h_seq_ref[gs]);
+          S[1].push_back(d + static_cast<int>(hv[i]));
+          S[2].push_back(q.h_qref[s]);
+          S[3].push_back(static_cast<int>(hv[i]));
         }
-        out.stage_n[slot] = ng;
+        out.stage_n[slot] = static_cast<int64_t>(S[0].size());
       };
       if (keep_stages) dump_seeds(CLAST_STAGE_SEEDS);
       if (ng > 1) {
@@ -621,8 +657,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
         rt.check_launch("k_seed_dup");
         device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, ng, hoff, tot32 + 2, false);
-        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, q.pos_off.ptr, off, off2, ss, sgs, stt,
-                                             sq);
+        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, q.pos_off.ptr, soff, off2, ss, sgs,
+                                             stt, sq);
         rt.check_launch("k_hits_scatter");
         // per-strand hit counts of the listed strands (nh2 read on device)
         uint32_t nh2 = 0;
@@ -638,8 +674,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
     rt.read_scalars(tot32 + 3, &nhit, 1);
     if (nhit)
-      k_hits_compact<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(q.nseq, np, nseed, q.pos_off.ptr, off, hcnt, hbase,
-                                                                   ss, sgs, stt, sq, hs, hgs, ht, hq);
+      k_hits_compact<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(q.nseq, soff, hcnt, hbase, ss, sgs, stt, sq, hs,
+                                                                   hgs, ht, hq);
     rt.check_launch("k_hits_compact");
   }
   if (keep_stages) {
@@ -790,8 +826,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     }
   }
   // gather + D2H
-  for (auto& c : out.col) c.clear();
-  out.ev.clear();
+  clast::pinned_pool().release(out.rows);
+  out.rows = {};
+  out.n = 0;
   if (nfinal) {
     int32_t* o32 = ws.get<int32_t>("o.i32", static_cast<uint64_t>(nfinal) * 10);
     double* oev = ws.get<double>("o.ev", nfinal);
@@ -799,8 +836,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
                ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
     k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
     rt.check_launch("k_gather_out");
-    for (int c = 0; c < 10; ++c) d2h(out.col[c], o32 + static_cast<uint64_t>(c) * nfinal, nfinal, st);
-    d2h(out.ev, oev, nfinal, st);
+    out.rows = clast::pinned_pool().acquire(static_cast<uint64_t>(nfinal) * 48);
+    out.n = nfinal;
+    hip_check(hipMemcpyAsync(out.rows.ptr, o32, sizeof(int32_t) * 10 * static_cast<uint64_t>(nfinal),
+                             hipMemcpyDeviceToHost, st), "D2H rows");
+    hip_check(hipMemcpyAsync(const_cast<double*>(out.ev()), oev, sizeof(double) * nfinal, hipMemcpyDeviceToHost, st),
+              "D2H E-values");
   }
   mark(EV_RES1);
   rt.sync();
@@ -934,24 +975,26 @@ int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, in
   });
 }
 
-int64_t clast_hits_count(clast_hits h) { return h ? static_cast<int64_t>(h->ev.size()) : -1; }
+int64_t clast_hits_count(clast_hits h) { return h ? h->n : -1; }
 
 int clast_hits_copy(clast_hits h, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
                     int32_t* qHL, int32_t* match, int32_t* score, double* evalue) {
   return guard([&] {
     if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
     int32_t* dst[8] = {tID, tIdx, qID, qIdx, tHL, qHL, match, score};
+    if (!h->n) return;
     for (int c = 0; c < 8; ++c)
-      if (dst[c] && !h->col[c].empty()) std::memcpy(dst[c], h->col[c].data(), h->col[c].size() * sizeof(int32_t));
-    if (evalue && !h->ev.empty()) std::memcpy(evalue, h->ev.data(), h->ev.size() * sizeof(double));
+      if (dst[c]) std::memcpy(dst[c], h->col(c), static_cast<size_t>(h->n) * sizeof(int32_t));
+    if (evalue) std::memcpy(evalue, h->ev(), static_cast<size_t>(h->n) * sizeof(double));
   });
 }
 
 int clast_hits_copy_windows(clast_hits h, int32_t* twin, int32_t* qwin) {
   return guard([&] {
     if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
-    if (twin && !h->col[8].empty()) std::memcpy(twin, h->col[8].data(), h->col[8].size() * sizeof(int32_t));
-    if (qwin && !h->col[9].empty()) std::memcpy(qwin, h->col[9].data(), h->col[9].size() * sizeof(int32_t));
+    if (!h->n) return;
+    if (twin) std::memcpy(twin, h->col(8), static_cast<size_t>(h->n) * sizeof(int32_t));
+    if (qwin) std::memcpy(qwin, h->col(9), static_cast<size_t>(h->n) * sizeof(int32_t));
   });
 }
 

@@ -305,29 +305,26 @@ struct SeedArgs {
   const uint8_t* qbase;
   const uint32_t* qslot;
   const int32_t* qlen;
-  const int32_t* qwin;      // query window of each strand
-  const uint32_t* pos_off;  // nseq + 1
+  const int32_t* qwin;        // query window of each strand
+  const uint32_t* pos_off;    // nseq + 1
+  const uint32_t* pos_strand; // strand of each k-mer position
   uint32_t nseq, npos;
   Lookup lk;
-  const int2* epos;         // (target sequence, intra-sequence index) per entry
-  const int32_t* tlen;      // per target sequence
-  const int32_t* twin;      // target window of each target sequence
+  const int2* epos;           // (target sequence, intra-sequence index) per entry
+  const int32_t* tlen;        // per target sequence
+  const int32_t* twin;        // target window of each target sequence
   int nqwin;
   int lmer, gap, width, local;
-  uint32_t* unit_first;     // per (target window, query window): first strand with a seed
+  uint32_t* unit_first;       // per (target window, query window): first strand with a seed
 };
 
-// Strand of flat k-mer position p; the workgroup narrows the search range
-// to the strands its positions span.
-__device__ __forceinline__ uint32_t strand_of(const uint32_t* pos_off, uint32_t nseq, uint32_t npos, uint32_t p,
-                                              uint32_t* lds2) {
-  const uint32_t first = blockIdx.x * kBlock;
-  uint32_t last = first + kBlock - 1;
-  if (last >= npos) last = npos - 1;
-  if (threadIdx.x == 0) lds2[0] = upper_bound_u(pos_off, 0u, nseq + 1, first) - 1;
-  if (threadIdx.x == 1) lds2[1] = upper_bound_u(pos_off, 0u, nseq + 1, last);
-  __syncthreads();
-  return upper_bound_u(pos_off, lds2[0], lds2[1], p) - 1;
+// Strand of every k-mer position of a query set, written once when the set
+// is uploaded: one wavefront per strand.
+static __global__ __launch_bounds__(kBlock) void k_pos_strand(const uint32_t* __restrict__ pos_off, uint32_t nseq,
+                                                              uint32_t* __restrict__ pos_strand) {
+  const uint32_t s = blockIdx.x * (kBlock / 64) + wave_id();
+  if (s >= nseq) return;
+  for (uint32_t p = pos_off[s] + lane_id(); p < pos_off[s + 1]; p += 64) pos_strand[p] = s;
 }
 
 // corner filter (…deleteSeedsOnSequenceBoundary.cuh:31-49), global mode only
@@ -335,50 +332,56 @@ __device__ __forceinline__ bool corner_ok(const SeedArgs& a, int d, int qlen, in
   return a.local || !(d + a.gap < 0 || d + qlen - a.gap > tlen);
 }
 
-// Query k-mer lookup per position: its cell {first entry, entry count, first
-// entry's (seq, tIdx)} and the number of seeds surviving the corner filter.
-// Only positions with qIdx + lMer <= qLen exist here: the query-boundary
-// filter (…deleteSeedsOnSequenceBoundary.cu:19-72) removes every other one.
-// Also records, per (target window, query window) unit, the first strand
-// holding a seed — the strand of the unit's element 0, which the isolate
-// filter keeps (…deleteIsolateSeeds.cu:28-76).
+// Records strand s as a candidate first strand of unit (target window tw,
+// query window qw) — the strand of the unit's element 0, which the isolate
+// filter keeps (…deleteIsolateSeeds.cu:28-76).  Values only decrease, so a
+// stale read only costs an atomic.
+__device__ __forceinline__ void note_unit_first(const SeedArgs& a, int tw, int qw, uint32_t s) {
+  uint32_t* uf = &a.unit_first[tw * a.nqwin + qw];
+  if (s < *uf) atomicMin(uf, s);
+}
+
+// Query k-mer lookup per position (krnlBinarySearch.cu:3-33 + the cell
+// gathers of …createRawSeedList.cu:63-135): the cell record {first entry,
+// entry count, first entry's (seq, tIdx)} of its k-mer, and per strand the
+// number of matching entries (its raw seed count, before the corner
+// filter).  Only positions with qIdx + lMer <= qLen exist here: the
+// query-boundary filter (…deleteSeedsOnSequenceBoundary.cu:19-72) removes
+// every other one.
 static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint4* __restrict__ cellp,
-                                                              uint32_t* __restrict__ cnt) {
-  __shared__ uint32_t lds2[2];
+                                                              uint32_t* __restrict__ scnt) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
-  if (p >= a.npos) return;
-  const int qIdx = static_cast<int>(p - a.pos_off[s]);
-  const int qLen = a.qlen[s];
-  const uint8_t* b = a.qbase + a.qslot[s] + qIdx;
-  uint64_t key = 0;
-  bool ok = true;
-  for (int j = 0; j < a.lmer; ++j) {
-    const uint32_t c = base_code(b[j]);
-    ok = ok && c < 4u;
-    key = (key << 2) | (c & 3u);
-  }
-  uint32_t lo = 0, n = 0, c = 0;
-  int2 first = make_int2(0, 0);
-  if (ok) {
-    n = lookup(a.lk, key, lo, first);
-    const int qw = a.qwin[s];
-    int last_tw = -1;
-    for (uint32_t j = 0; j < n; ++j) {
-      const int2 e = j == 0 ? first : a.epos[lo + j];
-      const int d = e.y - qIdx;
-      if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
-      ++c;
-      const int tw = a.twin[e.x];
-      if (tw != last_tw) {
-        last_tw = tw;
-        uint32_t* uf = &a.unit_first[tw * a.nqwin + qw];
-        if (s < *uf) atomicMin(uf, s);  // values only decrease: a stale read only costs an atomic
-      }
+  const bool valid = p < a.npos;
+  const uint32_t s = valid ? a.pos_strand[p] : 0xFFFFFFFFu;
+  uint32_t n = 0;
+  if (valid) {
+    const int qIdx = static_cast<int>(p - a.pos_off[s]);
+    const uint8_t* b = a.qbase + a.qslot[s] + qIdx;
+    uint64_t key = 0;
+    bool ok = true;
+    for (int j = 0; j < a.lmer; ++j) {
+      const uint32_t c = base_code(b[j]);
+      ok = ok && c < 4u;
+      key = (key << 2) | (c & 3u);
     }
+    uint32_t lo = 0;
+    int2 first = make_int2(0, 0);
+    if (ok) n = lookup(a.lk, key, lo, first);
+    cellp[p] = make_uint4(lo, n, static_cast<uint32_t>(first.x), static_cast<uint32_t>(first.y));
   }
-  cellp[p] = make_uint4(lo, n, static_cast<uint32_t>(first.x), static_cast<uint32_t>(first.y));
-  cnt[p] = c;
+  // strand totals: positions are strand-ordered, so each strand's lanes of a
+  // wavefront are contiguous; segmented inclusive scan, the segment's last
+  // lane adds the sum
+  const int lane = lane_id();
+  uint32_t v = n;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t up = __shfl_up(v, d, 64);
+    const uint32_t su = __shfl_up(s, d, 64);
+    if (lane >= d && su == s) v += up;
+  }
+  const uint32_t sn = __shfl_down(s, 1, 64);
+  if (valid && v && (lane == 63 || sn != s)) atomicAdd(&scnt[s], v);
 }
 
 // hasNearPair core (…deleteDuplicateSeeds.cuh:29-38) for seeds of one
@@ -403,11 +406,21 @@ __device__ __forceinline__ T wave_max_scan_incl(T v) {
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_incl(uint32_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (l >= d) v += o;
+  }
+  return v;
+}
+
 constexpr int kSegCap = 1024;          // seeds of one strand sorted in LDS
 constexpr int kSegWaves = kBlock / 64;
 
 struct SegOut {
-  int32_t* hs;            // hits of strand s go to [off[pos_off[s]], ...), the
+  int32_t* hs;            // hits of strand s go to [soff[s], ...), the
   int32_t* hgs;           // strand's own seed range (a strand has no more
   int32_t* ht;            // hits than seeds)
   int32_t* hq;
@@ -418,8 +431,9 @@ struct SegOut {
 };
 
 // LDS image of one strand's seeds.  Packed: one u64 (target sequence, diagonal
-// + d_off, qIdx) when the three fit in 64 bits; otherwise (target sequence,
-// diagonal + d_off) and qIdx apart.  Either sorts as (seq, diagonal, qIdx).
+// + d_off, qIdx) when the three fit in 63 bits; otherwise (target sequence,
+// diagonal + d_off) and qIdx apart.  Either sorts as (seq, diagonal, qIdx);
+// seeds the corner filter removes are ~0 and sort last.
 template <bool PACKED>
 struct SegLds;
 template <>
@@ -450,6 +464,12 @@ __device__ __forceinline__ void seg_put(SegLds<PACKED>& L, int w, uint32_t at, c
 }
 
 template <bool PACKED>
+__device__ __forceinline__ void seg_none(SegLds<PACKED>& L, int w, uint32_t at) {
+  L.k[w][at] = ~0ull;
+  if constexpr (!PACKED) L.q[w][at] = INT_MAX;
+}
+
+template <bool PACKED>
 __device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t i, const SegPack& pk, int& gs, int& d,
                                         int& q) {
   const uint64_t k = L.k[w][i];
@@ -465,29 +485,33 @@ __device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t
 }
 
 // One wavefront per query strand: its seeds (positions in qIdx order, cells
-// in index order) go to LDS, are bitonic-sorted by (target sequence,
-// diagonal, qIdx) — the …sortSeeds.cuh:14-55 order restricted to one qID —
-// and the isolate and duplicate filters run on the sorted list with wave
-// scans: K(i) = near(i) or i is its unit's element 0; D(i) = K(i) and not
-// near(i, previous K of the group).  Surviving seeds are written as hits at
-// the start of the strand's own seed range (their order is fixed later by
-// complete sort keys).  Strands with more than kSegCap seeds are listed for
-// the global path.
+// in index order; the corner filter applied) go to LDS, are bitonic-sorted
+// by (target sequence, diagonal, qIdx) — the …sortSeeds.cuh:14-55 order
+// restricted to one qID — and the isolate and duplicate filters run on the
+// sorted list with wave scans: K(i) = near(i) or i is its unit's element 0;
+// D(i) = K(i) and not near(i, previous K of the group).  Surviving seeds are
+// written as hits at the start of the strand's own seed range (their order
+// is fixed later by complete sort keys).  Strands with more than kSegCap
+// seeds are listed for the global path.
+//
+// Element 0 of a unit is the first seed of the unit's first strand with a
+// seed in that target window.  The pass over every strand (list == null)
+// records each strand as a candidate (unit_first) and takes no seed as
+// element 0; the strands that end up first in some unit are then run again
+// (list, *nlist of them) with unit_first final.
 template <bool PACKED>
 static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint4* __restrict__ cellp,
-                                                                 const uint32_t* __restrict__ off, uint32_t nseed,
-                                                                 SegPack pk, SegOut o) {
+                                                                 const uint32_t* __restrict__ scnt,
+                                                                 const uint32_t* __restrict__ soff, SegPack pk,
+                                                                 SegOut o, const uint32_t* __restrict__ list,
+                                                                 const uint32_t* __restrict__ nlist) {
   __shared__ SegLds<PACKED> L;
   const int w = wave_id(), lane = lane_id();
-  for (uint32_t s = blockIdx.x * kSegWaves + w; s < a.nseq; s += gridDim.x * kSegWaves) {
-    const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
-    if (p0 == p1) {
-      if (lane == 0) o.hcnt[s] = 0;
-      continue;
-    }
-    const uint32_t b0 = off[p0];
-    const uint32_t b1 = p1 < a.npos ? off[p1] : nseed;
-    const uint32_t n = b1 - b0;
+  const bool rerun = list != nullptr;
+  const uint32_t count = rerun ? *nlist : a.nseq;
+  for (uint32_t k = blockIdx.x * kSegWaves + w; k < count; k += gridDim.x * kSegWaves) {
+    const uint32_t s = rerun ? list[k] : k;
+    const uint32_t n = scnt[s];
     if (n == 0) {
       if (lane == 0) o.hcnt[s] = 0;
       continue;
@@ -499,31 +523,38 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       }
       continue;
     }
+    const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
     const int qLen = a.qlen[s];
-    // generate the strand's seeds into LDS at their scanned slots
-    for (uint32_t p = p0 + lane; p < p1; p += 64) {
-      const uint4 cp = cellp[p];
-      if (!cp.y) continue;
+    // generate the strand's seeds into LDS: slots from a wave scan of the
+    // positions' entry counts
+    uint32_t base = 0, mine = 0;
+    for (uint32_t c0 = p0; c0 < p1; c0 += 64) {
+      const uint32_t p = c0 + lane;
+      uint4 cp = make_uint4(0, 0, 0, 0);
+      if (p < p1) cp = cellp[p];
+      const uint32_t inc = wave_sum_incl(cp.y);
+      uint32_t at = base + inc - cp.y;
+      base += __shfl(inc, 63, 64);
       const int qIdx = static_cast<int>(p - p0);
-      uint32_t at = off[p] - b0;
-      for (uint32_t j = 0; j < cp.y; ++j) {
+      for (uint32_t j = 0; j < cp.y; ++j, ++at) {
         const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
         const int d = e.y - qIdx;
-        if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
-        seg_put<PACKED>(L, w, at, pk, e.x, d, qIdx);
-        ++at;
+        if (corner_ok(a, d, qLen, a.tlen[e.x])) {
+          seg_put<PACKED>(L, w, at, pk, e.x, d, qIdx);
+          ++mine;
+        } else {
+          seg_none<PACKED>(L, w, at);
+        }
       }
     }
+    const uint32_t m = __shfl(wave_sum_incl(mine), 63, 64);  // seeds past the corner filter
     uint32_t np = 1;
     while (np < n) np <<= 1;
-    for (uint32_t i = n + lane; i < np; i += 64) {
-      L.k[w][i] = ~0ull;
-      if constexpr (!PACKED) L.q[w][i] = INT_MAX;
-    }
+    for (uint32_t i = n + lane; i < np; i += 64) seg_none<PACKED>(L, w, i);
     wave_lds_sync();
     // bitonic sort ascending
-    for (uint32_t k = 2; k <= np; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+    for (uint32_t kk = 2; kk <= np; kk <<= 1) {
+      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
         for (uint32_t i = lane; i < np; i += 64) {
           const uint32_t x = i ^ j;
           if (x > i) {
@@ -534,7 +565,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
             } else {
               gt = ka > kb || (ka == kb && L.q[w][i] > L.q[w][x]);
             }
-            if (gt == ((i & k) == 0)) {
+            if (gt == ((i & kk) == 0)) {
               L.k[w][i] = kb;
               L.k[w][x] = ka;
               if constexpr (!PACKED) {
@@ -548,13 +579,13 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         wave_lds_sync();
       }
     }
-    // filters over the sorted list, 64 seeds at a time
+    // filters over the sorted seeds, 64 at a time
     const int qw = a.qwin[s];
     int carry_k = -1, carry_g = -1;
     uint32_t nh = 0;
-    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+    for (uint32_t c0 = 0; c0 < m; c0 += 64) {
       const uint32_t i = c0 + lane;
-      const bool valid = i < n;
+      const bool valid = i < m;
       bool ghead = false, nr = false, kk = false;
       int gs = 0, d = 0, t = 0, q = 0;
       if (valid) {
@@ -574,7 +605,11 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
           }
         }
         const int tw = a.twin[gs];
-        const bool elem0 = twhead && a.unit_first[tw * a.nqwin + qw] == s;
+        bool elem0 = false;
+        if (twhead) {
+          if (rerun) elem0 = a.unit_first[tw * a.nqwin + qw] == s;
+          else note_unit_first(a, tw, qw, s);
+        }
         kk = nr || elem0;
       }
       const int kin = wave_max_scan_incl<int>(kk ? static_cast<int>(i) : -1);
@@ -592,15 +627,15 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
           dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
         }
       }
-      const unsigned long long m = __ballot(dd);
+      const unsigned long long mk = __ballot(dd);
       if (dd) {
-        const uint32_t at = b0 + nh + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        const uint32_t at = soff[s] + nh + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
         o.hs[at] = static_cast<int32_t>(s);
         o.hgs[at] = gs;
         o.ht[at] = t;
         o.hq[at] = q;
       }
-      nh += static_cast<uint32_t>(__popcll(m));
+      nh += static_cast<uint32_t>(__popcll(mk));
       const int lk = __shfl(kin, 63, 64), lg = __shfl(gin, 63, 64);
       carry_k = lk > carry_k ? lk : carry_k;
       carry_g = lg > carry_g ? lg : carry_g;
@@ -610,51 +645,31 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
   }
 }
 
+// Strands to run again once unit_first is final: each unit's first strand,
+// unless it took the global path (whose filters read unit_first directly).
+static __global__ __launch_bounds__(kBlock) void k_fix_list(const uint32_t* __restrict__ unit_first, uint32_t nunits,
+                                                            const uint8_t* __restrict__ ovf_flag,
+                                                            uint32_t* __restrict__ mark, uint32_t* __restrict__ list,
+                                                            uint32_t* __restrict__ nlist) {
+  const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= nunits) return;
+  const uint32_t s = unit_first[u];
+  if (s == 0xFFFFFFFFu || ovf_flag[s]) return;
+  if (atomicExch(&mark[s], 1u) == 0u) list[atomicAdd(nlist, 1u)] = s;
+}
+
 // ---------------------------------------------------------------- global path
 // (strands with more than kSegCap seeds, and the stage-retaining mode)
+//
+// Seed key (strand, target sequence, diagonal + d_off); a seed the corner
+// filter removes keeps its slot with every bit below the strand set, which
+// no seed reaches (d_off leaves the top diagonal code unused), so it sorts
+// after its strand's seeds and is skipped by the filters.
 
 struct SeedPack {
   int gs_bits, d_bits;
   int d_off;
 };
-
-// Seed counts of the positions of listed strands only.
-static __global__ __launch_bounds__(kBlock) void k_ovf_counts(SeedArgs a, const uint8_t* __restrict__ ovf_flag,
-                                                              const uint32_t* __restrict__ cnt,
-                                                              uint32_t* __restrict__ cnt2) {
-  __shared__ uint32_t lds2[2];
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
-  if (p >= a.npos) return;
-  cnt2[p] = ovf_flag[s] ? cnt[p] : 0u;
-}
-
-// Writes the surviving seeds of each listed position at its scanned offset,
-// in query-position then cell order (krnlWriteSeedList.cu:3-37 order).
-static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk, const uint8_t* __restrict__ ovf_flag,
-                                                              const uint4* __restrict__ cellp,
-                                                              const uint32_t* __restrict__ off,
-                                                              uint64_t* __restrict__ skey,
-                                                              uint32_t* __restrict__ sval) {
-  __shared__ uint32_t lds2[2];
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t s = strand_of(a.pos_off, a.nseq, a.npos, p < a.npos ? p : a.npos - 1, lds2);
-  if (p >= a.npos || !ovf_flag[s]) return;
-  const uint4 cp = cellp[p];
-  if (cp.y == 0) return;
-  const int qIdx = static_cast<int>(p - a.pos_off[s]);
-  const int qLen = a.qlen[s];
-  uint32_t o = off[p];
-  for (uint32_t j = 0; j < cp.y; ++j) {
-    const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
-    const int d = e.y - qIdx;
-    if (!corner_ok(a, d, qLen, a.tlen[e.x])) continue;
-    skey[o] = (static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits)) | (static_cast<uint64_t>(e.x) << pk.d_bits) |
-              static_cast<uint64_t>(d + pk.d_off);
-    sval[o] = static_cast<uint32_t>(qIdx);
-    ++o;
-  }
-}
 
 struct SeedDecode {
   int gs_bits, d_bits, d_off;
@@ -665,7 +680,52 @@ struct SeedDecode {
   __device__ __forceinline__ int d(uint64_t k) const {
     return static_cast<int>(k & ((1ull << d_bits) - 1ull)) - d_off;
   }
+  __device__ __forceinline__ bool none(uint64_t k) const {
+    const uint64_t low = (1ull << (gs_bits + d_bits)) - 1ull;
+    return (k & low) == low;
+  }
 };
+
+// Entry counts of the positions of listed strands only.
+static __global__ __launch_bounds__(kBlock) void k_ovf_counts(SeedArgs a, const uint8_t* __restrict__ ovf_flag,
+                                                              const uint4* __restrict__ cellp,
+                                                              uint32_t* __restrict__ cnt2) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= a.npos) return;
+  cnt2[p] = ovf_flag[a.pos_strand[p]] ? cellp[p].y : 0u;
+}
+
+// Writes the seeds of each listed position at its scanned offset, in
+// query-position then cell order (krnlWriteSeedList.cu:3-37 order), and
+// records the strands as unit candidates like k_seed_segments.
+static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk, const uint8_t* __restrict__ ovf_flag,
+                                                              const uint4* __restrict__ cellp,
+                                                              const uint32_t* __restrict__ off,
+                                                              uint64_t* __restrict__ skey,
+                                                              uint32_t* __restrict__ sval) {
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= a.npos) return;
+  const uint32_t s = a.pos_strand[p];
+  if (!ovf_flag[s]) return;
+  const uint4 cp = cellp[p];
+  if (cp.y == 0) return;
+  const int qIdx = static_cast<int>(p - a.pos_off[s]);
+  const int qLen = a.qlen[s];
+  const int qw = a.qwin[s];
+  const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
+  uint32_t o = off[p];
+  for (uint32_t j = 0; j < cp.y; ++j, ++o) {
+    const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
+    const int d = e.y - qIdx;
+    if (corner_ok(a, d, qLen, a.tlen[e.x])) {
+      skey[o] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
+      note_unit_first(a, a.twin[e.x], qw, s);
+    } else {
+      skey[o] = sk | ((1ull << (pk.gs_bits + pk.d_bits)) - 1ull);
+    }
+    sval[o] = static_cast<uint32_t>(qIdx);
+  }
+}
 
 // Per sorted seed: near-predecessor flag, group head index (for the
 // group-start max-scan) and the isolate keep flag K(i) = near(i) or i is its
@@ -681,12 +741,18 @@ static __global__ __launch_bounds__(kBlock) void k_seed_marks(const uint64_t* __
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ns) return;
   const uint64_t k = skey[i];
+  if (dc.none(k)) {
+    kidx[i] = 0u;
+    gidx[i] = i;
+    return;
+  }
   const uint64_t grp = k >> dc.d_bits;
   const uint32_t s = dc.s(k), gs = dc.gs(k);
   const int d = dc.d(k), q = static_cast<int>(sval[i]), t = d + q;
   bool nr = false, ghead = true, uhead = true;
   const int tw = seq_win_t[gs];
   if (i > 0) {
+    // a removed seed is never the predecessor of one of its strand's seeds
     const uint64_t pk = skey[i - 1];
     if ((pk >> dc.d_bits) == grp) {
       ghead = false;
@@ -728,15 +794,15 @@ static __global__ __launch_bounds__(kBlock) void k_seed_dup(const uint64_t* __re
 }
 
 // Surviving seeds of the global path go to their strand's own seed range
-// (like k_seed_segments): slot = start of the strand's range + rank among
-// the strand's surviving seeds.  seg_off: the main seed offsets per position;
-// goff: the global path's offsets; hoff: exclusive scan of dflag.
+// (like k_seed_segments): slot = soff[s] + rank among the strand's
+// surviving seeds.  goff: the global path's offsets per position; hoff:
+// exclusive scan of dflag.
 static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* __restrict__ skey,
                                                                 const uint32_t* __restrict__ sval, uint32_t ns,
                                                                 SeedDecode dc, const uint32_t* __restrict__ dflag,
                                                                 const uint32_t* __restrict__ hoff,
                                                                 const uint32_t* __restrict__ pos_off,
-                                                                const uint32_t* __restrict__ seg_off,
+                                                                const uint32_t* __restrict__ soff,
                                                                 const uint32_t* __restrict__ goff,
                                                                 int32_t* __restrict__ hs, int32_t* __restrict__ hgs,
                                                                 int32_t* __restrict__ ht, int32_t* __restrict__ hq) {
@@ -744,7 +810,7 @@ static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* 
   if (i >= ns || !dflag[i]) return;
   const uint64_t k = skey[i];
   const uint32_t s = dc.s(k);
-  const uint32_t o = seg_off[pos_off[s]] + (hoff[i] - hoff[goff[pos_off[s]]]);
+  const uint32_t o = soff[s] + (hoff[i] - hoff[goff[pos_off[s]]]);
   const int q = static_cast<int>(sval[i]);
   hs[o] = static_cast<int32_t>(s);
   hgs[o] = static_cast<int32_t>(dc.gs(k));
@@ -762,6 +828,10 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __re
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= novf) return;
   const uint32_t s = ovf[k];
+  if (pos_off[s] == pos_off[s + 1]) {  // no k-mer position (the list may hold every strand)
+    hcnt[s] = 0;
+    return;
+  }
   const uint32_t g0 = goff[pos_off[s]];
   const uint32_t g1 = pos_off[s + 1] < npos ? goff[pos_off[s + 1]] : ng;
   const uint32_t h0 = g0 < ng ? hoff[g0] : nh;
@@ -770,10 +840,8 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __re
 }
 
 // Dense hits in strand order: strand s's hcnt[s] hits move from the start of
-// its seed range to hbase[s] (exclusive scan of hcnt).
-static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, uint32_t npos, uint32_t nseed,
-                                                                const uint32_t* __restrict__ pos_off,
-                                                                const uint32_t* __restrict__ seg_off,
+// its seed range (soff[s]) to hbase[s] (exclusive scan of hcnt).
+static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, const uint32_t* __restrict__ soff,
                                                                 const uint32_t* __restrict__ hcnt,
                                                                 const uint32_t* __restrict__ hbase,
                                                                 const int32_t* __restrict__ hs,
@@ -786,7 +854,7 @@ static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, u
   if (s >= nseq) return;
   const uint32_t n = hcnt[s];
   if (!n) return;
-  const uint32_t src = pos_off[s] < npos ? seg_off[pos_off[s]] : nseed;
+  const uint32_t src = soff[s];
   const uint32_t dst = hbase[s];
   for (uint32_t j = 0; j < n; ++j) {
     ds[dst + j] = hs[src + j];

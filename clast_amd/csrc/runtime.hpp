@@ -4,11 +4,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace clast {
 
@@ -127,6 +129,52 @@ class Runtime {
   Workspace* ws_ = nullptr;
   std::mutex mu_;
 };
+
+// Page-locked host blocks reused across calls: result rows are copied into
+// them at full link speed (a pageable destination goes through a driver
+// staging copy).  A block goes back to the pool when its holder releases it.
+class PinnedPool {
+ public:
+  struct Block {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+  };
+  Block acquire(uint64_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    size_t best = free_.size();
+    for (size_t i = 0; i < free_.size(); ++i)
+      if (free_[i].bytes >= bytes && (best == free_.size() || free_[i].bytes < free_[best].bytes)) best = i;
+    if (best < free_.size()) {
+      const Block b = free_[best];
+      free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(best));
+      return b;
+    }
+    Block b;
+    b.bytes = std::max<uint64_t>(bytes + bytes / 4, 1ull << 16);
+    hip_check(hipHostMalloc(&b.ptr, b.bytes, hipHostMallocDefault), "hipHostMalloc");
+    return b;
+  }
+  void release(Block b) {
+    if (!b.ptr) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    free_.push_back(b);
+    if (free_.size() > kKeep) {  // drop the smallest
+      size_t k = 0;
+      for (size_t i = 1; i < free_.size(); ++i)
+        if (free_[i].bytes < free_[k].bytes) k = i;
+      (void)hipHostFree(free_[k].ptr);
+      free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(k));
+    }
+  }
+
+ private:
+  static constexpr size_t kKeep = 8;
+  std::vector<Block> free_;
+  std::mutex mu_;
+};
+
+// Process-wide pool (never destroyed: blocks may outlive static teardown).
+PinnedPool& pinned_pool();
 
 // The runtime of the calling thread's current device (created on first use).
 Runtime& runtime();
