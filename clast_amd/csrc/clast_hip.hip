@@ -207,14 +207,12 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
   ix.direct = use_direct(B, nent);
   const uint64_t nwords = std::max<uint64_t>(1, (1ull << B) / 64);
   const uint64_t nblk = (nwords + 7) / 8;
+  ix.bitmap.alloc(nwords);
+  k_fill<unsigned long long><<<grid_for(nwords, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, 0ull);
   if (ix.direct) {
-    ix.bitmap.alloc(1);
+    // absent prefixes are never read (the bitmap answers first): no fill
     ix.rank.alloc(1);
     ix.cells.alloc(1ull << B);
-    k_fill<uint4><<<grid_for(1ull << B, kBlock), kBlock, 0, st>>>(ix.cells.ptr, 1ull << B, make_uint4(0, 0, 0, 0));
-  } else {
-    ix.bitmap.alloc(nwords);
-    k_fill<unsigned long long><<<grid_for(nwords, kBlock), kBlock, 0, st>>>(ix.bitmap.ptr, nwords, 0ull);
   }
   uint32_t ncell = 0;
   if (nent > 0) {
@@ -226,7 +224,7 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
     rt.read_scalars(scal, &ncell, 1);
     uint32_t* cell_off = ws.get<uint32_t>("ix.coff", static_cast<uint64_t>(ncell) + 1);
     k_cell_build<<<grid_for(nent, kBlock), kBlock, 0, st>>>(ek, nent, ix.shift, head, cell, cell_off,
-                                                             ix.direct ? nullptr : ix.bitmap.ptr);
+                                                             ix.bitmap.ptr);
     if (!ix.direct) ix.cells.alloc(ncell);
     k_cell_records<<<grid_for(ncell, kBlock), kBlock, 0, st>>>(cell_off, ncell, ix.epos.ptr, ek, ix.shift,
                                                                 ix.direct ? 1 : 0, ix.cells.ptr);
@@ -557,17 +555,26 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
     const bool packed = gbits + dbits + qbits <= 63;
     const SegPack sp{d_off, qbits, dbits};
-    const SegOut so{ss, sgs, stt, sq, hcnt, ovf_list, tot32 + 1, ovf_flag};
-    auto segments = [&](uint32_t nwaves, const uint32_t* list, const uint32_t* nlist) {
-      const uint32_t grid = std::min<uint32_t>(grid_for(nwaves, kSegWaves), 1u << 20);
-      if (packed) k_seed_segments<true><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist);
-      else k_seed_segments<false><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist);
+    uint32_t* big_list = ws.get<uint32_t>("s.big", q.nseq);
+    const SegOut so{ss, sgs, stt, sq, hcnt, big_list, tot32 + 6, ovf_list, tot32 + 1, ovf_flag};
+    // nwaves: waves to launch (at most one per listed strand)
+    auto segments = [&](bool small, uint32_t nwaves, const uint32_t* list, const uint32_t* nlist, int mode) {
+      const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(nwaves, kSegWaves), 1u << 20));
+      if (small) {
+        if (packed) k_seed_segments<true, kSegSmall><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist, mode);
+        else k_seed_segments<false, kSegSmall><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist, mode);
+      } else {
+        if (packed) k_seed_segments<true, kSegBig><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist, mode);
+        else k_seed_segments<false, kSegBig><<<grid, kBlock, 0, st>>>(a, cellp, scnt, soff, sp, so, list, nlist, mode);
+      }
       rt.check_launch("k_seed_segments");
     };
     mark(EV_WRITE0);
     if (!keep_stages) {
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
-      segments(q.nseq, nullptr, nullptr);
+      segments(true, q.nseq, nullptr, nullptr, kSegFirst);
+      // the big list's length stays on the device: a bounded grid strides over it
+      segments(false, std::min<uint32_t>(q.nseq, 4096u * kSegWaves), big_list, tot32 + 6, kSegBigPass);
       rt.read_scalars(tot32 + 1, &novf, 1);
     } else {
       // every strand on the global path: list them all
@@ -603,7 +610,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(fmark, q.nseq, 0u);
       k_fix_list<<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, static_cast<uint32_t>(nunits), ovf_flag,
                                                               fmark, flist, tot32 + 5);
-      segments(static_cast<uint32_t>(nunits), flist, tot32 + 5);
+      segments(false, static_cast<uint32_t>(nunits), flist, tot32 + 5, kSegRerun);
     }
     mark(EV_WRITE1);
     if (novf > 0) {

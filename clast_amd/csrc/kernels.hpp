@@ -217,14 +217,14 @@ static __global__ __launch_bounds__(kBlock) void k_cell_build(const uint64_t* __
   if (head[e]) {
     const uint64_t h = keys[e] >> shift;
     cell_off[cell[e]] = e;
-    if (bitmap) atomicOr(&bitmap[h >> 6], 1ull << (h & 63));
+    atomicOr(&bitmap[h >> 6], 1ull << (h & 63));
   }
   if (e + 1 == n) cell_off[cell[e] + head[e]] = n;
 }
 
 // Record of the r-th present prefix, stored at r (bitmap form) or at the
-// prefix value itself (direct form: a table over every prefix, zero count
-// where absent, so a lookup is one record read).
+// prefix value itself (direct form: a table over every prefix, so a present
+// k-mer costs one record read and no rank directory).
 static __global__ __launch_bounds__(kBlock) void k_cell_records(const uint32_t* __restrict__ cell_off, uint32_t ncell,
                                                                 const int2* __restrict__ epos,
                                                                 const uint64_t* __restrict__ keys, int shift,
@@ -255,7 +255,7 @@ struct Lookup {
   const int2* epos;
   int shift;          // 2 lMer - B
   int key64;
-  int direct;         // cells is indexed by the prefix itself (no bitmap)
+  int direct;         // cells is indexed by the prefix itself (no rank directory)
 };
 
 // Entries of a valid query k-mer (krnlBinarySearch.cu:3-33 + the cell
@@ -265,15 +265,16 @@ __device__ __forceinline__ uint32_t lookup(const Lookup& L, uint64_t key, uint32
   const uint64_t h = key >> L.shift;
   lo = 0;
   first = make_int2(0, 0);
+  // presence first: the bitmap stays in the Infinity Cache, so an absent
+  // k-mer never reaches HBM
+  const uint64_t w = h >> 6;
+  const unsigned long long word = L.bitmap[w];
+  const int bit = static_cast<int>(h & 63);
+  if (!((word >> bit) & 1ull)) return 0;
   uint4 c;
   if (L.direct) {
     c = L.cells[h];
-    if (c.y == 0) return 0;
   } else {
-    const uint64_t w = h >> 6;
-    const unsigned long long word = L.bitmap[w];
-    const int bit = static_cast<int>(h & 63);
-    if (!((word >> bit) & 1ull)) return 0;
     uint32_t r = L.rank[w >> 3];
     for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
     r += static_cast<uint32_t>(__popcll(word & ((1ull << bit) - 1ull)));
@@ -416,7 +417,12 @@ __device__ __forceinline__ uint32_t wave_sum_incl(uint32_t v) {
   return v;
 }
 
-constexpr int kSegCap = 1024;          // seeds of one strand sorted in LDS
+// Seeds of one strand sorted in LDS by one wavefront: the pass over every
+// strand reserves room for kSegSmall (occupancy is bounded by LDS, and most
+// strands hold far fewer); strands over it run in a pass with room for
+// kSegBig; strands over that take the global path.
+constexpr int kSegSmall = 256;
+constexpr int kSegBig = 2048;
 constexpr int kSegWaves = kBlock / 64;
 
 struct SegOut {
@@ -425,7 +431,9 @@ struct SegOut {
   int32_t* ht;            // hits than seeds)
   int32_t* hq;
   uint32_t* hcnt;         // hits per strand
-  uint32_t* ovf;          // strands with more than kSegCap seeds
+  uint32_t* big;          // strands over kSegSmall seeds (first pass)
+  uint32_t* nbig;
+  uint32_t* ovf;          // strands over kSegBig seeds (global path)
   uint32_t* novf;
   uint8_t* ovf_flag;      // per strand
 };
@@ -434,16 +442,18 @@ struct SegOut {
 // + d_off, qIdx) when the three fit in 63 bits; otherwise (target sequence,
 // diagonal + d_off) and qIdx apart.  Either sorts as (seq, diagonal, qIdx);
 // seeds the corner filter removes are ~0 and sort last.
-template <bool PACKED>
+template <bool PACKED, int CAP>
 struct SegLds;
-template <>
-struct SegLds<true> {
-  uint64_t k[kSegWaves][kSegCap];
+template <int CAP>
+struct SegLds<true, CAP> {
+  static constexpr bool kPacked = true;
+  uint64_t k[kSegWaves][CAP];
 };
-template <>
-struct SegLds<false> {
-  uint64_t k[kSegWaves][kSegCap];
-  int32_t q[kSegWaves][kSegCap];
+template <int CAP>
+struct SegLds<false, CAP> {
+  static constexpr bool kPacked = false;
+  uint64_t k[kSegWaves][CAP];
+  int32_t q[kSegWaves][CAP];
 };
 
 struct SegPack {
@@ -451,10 +461,9 @@ struct SegPack {
   int qbits, dbits;  // packed layout only
 };
 
-template <bool PACKED>
-__device__ __forceinline__ void seg_put(SegLds<PACKED>& L, int w, uint32_t at, const SegPack& pk, int gs, int d,
-                                        int q) {
-  if constexpr (PACKED) {
+template <class Lds>
+__device__ __forceinline__ void seg_put(Lds& L, int w, uint32_t at, const SegPack& pk, int gs, int d, int q) {
+  if constexpr (Lds::kPacked) {
     L.k[w][at] = (static_cast<uint64_t>(gs) << (pk.dbits + pk.qbits)) |
                  (static_cast<uint64_t>(d + pk.d_off) << pk.qbits) | static_cast<uint64_t>(q);
   } else {
@@ -463,17 +472,16 @@ __device__ __forceinline__ void seg_put(SegLds<PACKED>& L, int w, uint32_t at, c
   }
 }
 
-template <bool PACKED>
-__device__ __forceinline__ void seg_none(SegLds<PACKED>& L, int w, uint32_t at) {
+template <class Lds>
+__device__ __forceinline__ void seg_none(Lds& L, int w, uint32_t at) {
   L.k[w][at] = ~0ull;
-  if constexpr (!PACKED) L.q[w][at] = INT_MAX;
+  if constexpr (!Lds::kPacked) L.q[w][at] = INT_MAX;
 }
 
-template <bool PACKED>
-__device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t i, const SegPack& pk, int& gs, int& d,
-                                        int& q) {
+template <class Lds>
+__device__ __forceinline__ void seg_get(const Lds& L, int w, uint32_t i, const SegPack& pk, int& gs, int& d, int& q) {
   const uint64_t k = L.k[w][i];
-  if constexpr (PACKED) {
+  if constexpr (Lds::kPacked) {
     gs = static_cast<int>(k >> (pk.dbits + pk.qbits));
     d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
     q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
@@ -484,6 +492,8 @@ __device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t
   }
 }
 
+enum SegMode { kSegFirst = 0, kSegBigPass = 1, kSegRerun = 2 };
+
 // One wavefront per query strand: its seeds (positions in qIdx order, cells
 // in index order; the corner filter applied) go to LDS, are bitonic-sorted
 // by (target sequence, diagonal, qIdx) — the …sortSeeds.cuh:14-55 order
@@ -491,35 +501,39 @@ __device__ __forceinline__ void seg_get(const SegLds<PACKED>& L, int w, uint32_t
 // sorted list with wave scans: K(i) = near(i) or i is its unit's element 0;
 // D(i) = K(i) and not near(i, previous K of the group).  Surviving seeds are
 // written as hits at the start of the strand's own seed range (their order
-// is fixed later by complete sort keys).  Strands with more than kSegCap
-// seeds are listed for the global path.
+// is fixed later by complete sort keys).
 //
 // Element 0 of a unit is the first seed of the unit's first strand with a
-// seed in that target window.  The pass over every strand (list == null)
-// records each strand as a candidate (unit_first) and takes no seed as
-// element 0; the strands that end up first in some unit are then run again
-// (list, *nlist of them) with unit_first final.
-template <bool PACKED>
+// seed in that target window.  The first pass (every strand; strands over
+// CAP go to the big list) and the big pass (the big list; strands over CAP
+// go to the global path) record each strand as a candidate (unit_first) and
+// take no seed as element 0; the strands that end up first in some unit are
+// then run again (kSegRerun) with unit_first final.
+template <bool PACKED, int CAP>
 static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint4* __restrict__ cellp,
                                                                  const uint32_t* __restrict__ scnt,
                                                                  const uint32_t* __restrict__ soff, SegPack pk,
                                                                  SegOut o, const uint32_t* __restrict__ list,
-                                                                 const uint32_t* __restrict__ nlist) {
-  __shared__ SegLds<PACKED> L;
+                                                                 const uint32_t* __restrict__ nlist, int mode) {
+  __shared__ SegLds<PACKED, CAP> L;
   const int w = wave_id(), lane = lane_id();
-  const bool rerun = list != nullptr;
-  const uint32_t count = rerun ? *nlist : a.nseq;
+  const bool rerun = mode == kSegRerun;
+  const uint32_t count = list ? *nlist : a.nseq;
   for (uint32_t k = blockIdx.x * kSegWaves + w; k < count; k += gridDim.x * kSegWaves) {
-    const uint32_t s = rerun ? list[k] : k;
+    const uint32_t s = list ? list[k] : k;
     const uint32_t n = scnt[s];
     if (n == 0) {
       if (lane == 0) o.hcnt[s] = 0;
       continue;
     }
-    if (n > static_cast<uint32_t>(kSegCap)) {
+    if (n > static_cast<uint32_t>(CAP)) {
       if (lane == 0) {
-        o.ovf[atomicAdd(o.novf, 1u)] = s;
-        o.ovf_flag[s] = 1;
+        if (mode == kSegFirst) {
+          o.big[atomicAdd(o.nbig, 1u)] = s;
+        } else {
+          o.ovf[atomicAdd(o.novf, 1u)] = s;
+          o.ovf_flag[s] = 1;
+        }
       }
       continue;
     }
@@ -540,17 +554,17 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
         const int d = e.y - qIdx;
         if (corner_ok(a, d, qLen, a.tlen[e.x])) {
-          seg_put<PACKED>(L, w, at, pk, e.x, d, qIdx);
+          seg_put(L, w, at, pk, e.x, d, qIdx);
           ++mine;
         } else {
-          seg_none<PACKED>(L, w, at);
+          seg_none(L, w, at);
         }
       }
     }
     const uint32_t m = __shfl(wave_sum_incl(mine), 63, 64);  // seeds past the corner filter
     uint32_t np = 1;
     while (np < n) np <<= 1;
-    for (uint32_t i = n + lane; i < np; i += 64) seg_none<PACKED>(L, w, i);
+    for (uint32_t i = n + lane; i < np; i += 64) seg_none(L, w, i);
     wave_lds_sync();
     // bitonic sort ascending
     for (uint32_t kk = 2; kk <= np; kk <<= 1) {
@@ -589,13 +603,13 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       bool ghead = false, nr = false, kk = false;
       int gs = 0, d = 0, t = 0, q = 0;
       if (valid) {
-        seg_get<PACKED>(L, w, i, pk, gs, d, q);
+        seg_get(L, w, i, pk, gs, d, q);
         t = d + q;
         bool twhead = true;
         ghead = true;
         if (i > 0) {
           int pgs, pd, pq;
-          seg_get<PACKED>(L, w, i - 1, pk, pgs, pd, pq);
+          seg_get(L, w, i - 1, pk, pgs, pd, pq);
           if (pgs == gs) {
             ghead = false;
             twhead = false;
@@ -623,7 +637,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         dd = true;
         if (prevk >= 0 && prevk >= gstart) {
           int kgs, kd, kq;
-          seg_get<PACKED>(L, w, static_cast<uint32_t>(prevk), pk, kgs, kd, kq);
+          seg_get(L, w, static_cast<uint32_t>(prevk), pk, kgs, kd, kq);
           dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
         }
       }
@@ -659,7 +673,7 @@ static __global__ __launch_bounds__(kBlock) void k_fix_list(const uint32_t* __re
 }
 
 // ---------------------------------------------------------------- global path
-// (strands with more than kSegCap seeds, and the stage-retaining mode)
+// (strands with more than kSegBig seeds, and the stage-retaining mode)
 //
 // Seed key (strand, target sequence, diagonal + d_off); a seed the corner
 // filter removes keeps its slot with every bit below the strand set, which

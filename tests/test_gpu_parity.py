@@ -179,24 +179,30 @@ def test_result_sort():
 
 
 def test_long_reads_mix_fast_and_global_paths():
-    # 6-10 kbp reads: their true strands carry > 1024 seeds and take the
-    # global path; short reads in the same query set take the per-strand path
+    # at stride 2: 6-10 kbp reads carry > 2048 seeds on their true strand and
+    # take the global path, 0.8-1.5 kbp reads (> 256 seeds) the big LDS pass,
+    # 150 bp reads the first LDS pass, all in one query set
     t, _ = workload(n_t=3, t_len=60_000, n_r=1)
     long_r = gen.make_reads(gen.ReadSpec(21, 12, len_mode=gen.LEN_LOGUNIFORM, len_a=6000, len_b=10_000,
                                          sub_rate=0.01, indel_rate=0.002), t)
+    mid_r = gen.make_reads(gen.ReadSpec(23, 20, len_mode=gen.LEN_LOGUNIFORM, len_a=800, len_b=1500,
+                                        sub_rate=0.02, indel_rate=0.005), t)
     short_r = gen.make_reads(gen.ReadSpec(22, 80, sub_rate=0.03, indel_rate=0.01), t)
-    bases = np.concatenate([long_r.bases, short_r.bases])
-    off = np.concatenate([long_r.offsets, short_r.offsets[1:] + long_r.offsets[-1]]).astype(np.int64)
-    r = gen.Reads(bases, off)
+    parts = [long_r, mid_r, short_r]
+    bases = np.concatenate([x.bases for x in parts])
+    off = [np.zeros(1, np.int64)]
+    for x in parts:
+        off.append(x.offsets[1:] + off[-1][-1])
+    r = gen.Reads(bases, np.concatenate(off).astype(np.int64))
     for local in (0, 1):
         p = native.Params(total_db_size=db_size(t), local=local, stride=2)
         tl, ql = lists(t, r, p.lmer)
         ix = native.Index(p, tl, [(0, 3)])
-        q = native.Query(p, ql, [(0, 46), (46, 92)])
+        q = native.Query(p, ql, [(0, 56), (56, 112)])
         print(f"long reads local={local}", flush=True)
         got = native.search_batch(p, ix, q)
         assert got.times()[7] > 3000  # seeds
-        o = oracle.search(p, tl, [(0, 3)], ql, [(0, 46), (46, 92)])
+        o = oracle.search(p, tl, [(0, 3)], ql, [(0, 56), (56, 112)])
         assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
 
 
