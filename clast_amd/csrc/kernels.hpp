@@ -247,6 +247,8 @@ static __global__ __launch_bounds__(kBlock) void k_rank_count(const unsigned lon
   cnt[b] = c;
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 struct Lookup {
   const unsigned long long* bitmap;
   const uint32_t* rank;
@@ -273,7 +275,9 @@ __device__ __forceinline__ uint32_t lookup(const Lookup& L, uint64_t key, uint32
   if (!((word >> bit) & 1ull)) return 0;
   uint4 c;
   if (L.direct) {
-    c = L.cells[h];
+    // streamed past the caches: the bitmap is what should stay resident
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(&L.cells[h]));
+    c = make_uint4(v.x, v.y, v.z, v.w);
   } else {
     uint32_t r = L.rank[w >> 3];
     for (uint64_t j = w & ~7ull; j < w; ++j) r += static_cast<uint32_t>(__popcll(L.bitmap[j]));
@@ -353,7 +357,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint4*
                                                               uint32_t* __restrict__ scnt) {
   const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = p < a.npos;
-  const uint32_t s = valid ? a.pos_strand[p] : 0xFFFFFFFFu;
+  const uint32_t s = valid ? __builtin_nontemporal_load(&a.pos_strand[p]) : 0xFFFFFFFFu;
   uint32_t n = 0;
   if (valid) {
     const int qIdx = static_cast<int>(p - a.pos_off[s]);
@@ -368,7 +372,12 @@ static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint4*
     uint32_t lo = 0;
     int2 first = make_int2(0, 0);
     if (ok) n = lookup(a.lk, key, lo, first);
-    cellp[p] = make_uint4(lo, n, static_cast<uint32_t>(first.x), static_cast<uint32_t>(first.y));
+    u32x4 v;
+    v.x = lo;
+    v.y = n;
+    v.z = static_cast<uint32_t>(first.x);
+    v.w = static_cast<uint32_t>(first.y);
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(&cellp[p]));  // read again only by the next kernel
   }
   // strand totals: positions are strand-ordered, so each strand's lanes of a
   // wavefront are contiguous; segmented inclusive scan, the segment's last
@@ -494,6 +503,27 @@ __device__ __forceinline__ void seg_get(const Lds& L, int w, uint32_t i, const S
 
 enum SegMode { kSegFirst = 0, kSegBigPass = 1, kSegRerun = 2 };
 
+__device__ __forceinline__ void seg_decode(uint64_t k, const SegPack& pk, int& gs, int& d, int& q) {
+  gs = static_cast<int>(k >> (pk.dbits + pk.qbits));
+  d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
+  q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
+}
+
+// Bitonic sort of one u64 per lane across the wavefront (ascending by lane).
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t k) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int kk = 2; kk <= 64; kk <<= 1) {
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      const uint64_t o = __shfl_xor(k, j, 64);
+      const bool keep_min = ((lane & kk) == 0) == ((lane & j) == 0);
+      k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+    }
+  }
+  return k;
+}
+
 // One wavefront per query strand: its seeds (positions in qIdx order, cells
 // in index order; the corner filter applied) go to LDS, are bitonic-sorted
 // by (target sequence, diagonal, qIdx) — the …sortSeeds.cuh:14-55 order
@@ -562,6 +592,66 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       }
     }
     const uint32_t m = __shfl(wave_sum_incl(mine), 63, 64);  // seeds past the corner filter
+    const int qw = a.qwin[s];
+    if constexpr (PACKED) {
+      if (n <= 64) {
+        // one seed per lane: sort and filter in registers
+        wave_lds_sync();
+        const uint64_t key = wave_sort64(lane < static_cast<int>(n) ? L.k[w][lane] : ~0ull);
+        const bool valid = lane < static_cast<int>(m);
+        int gs, d, q;
+        seg_decode(key, pk, gs, d, q);
+        const int t = d + q;
+        int pgs, pd, pq;
+        seg_decode(__shfl_up(key, 1, 64), pk, pgs, pd, pq);
+        bool ghead = false, kk = false;
+        if (valid) {
+          bool twhead = true, nr = false;
+          ghead = true;
+          if (lane > 0) {
+            if (pgs == gs) {
+              ghead = false;
+              twhead = false;
+              nr = near_pair(t, d, pd + pq, pd, a.width, a.gap);
+            } else {
+              twhead = a.twin[pgs] != a.twin[gs];
+            }
+          }
+          const int tw = a.twin[gs];
+          bool elem0 = false;
+          if (twhead) {
+            if (rerun) elem0 = a.unit_first[tw * a.nqwin + qw] == s;
+            else note_unit_first(a, tw, qw, s);
+          }
+          kk = nr || elem0;
+        }
+        const int kin = wave_max_scan_incl<int>(kk ? lane : -1);
+        const int gin = wave_max_scan_incl<int>(ghead ? lane : -1);
+        int prevk = __shfl_up(kin, 1, 64);
+        if (lane == 0) prevk = -1;
+        const uint64_t kp = __shfl(key, prevk < 0 ? 0 : prevk, 64);
+        bool dd = false;
+        if (valid && kk) {
+          dd = true;
+          if (prevk >= 0 && prevk >= gin) {
+            int kgs, kd, kq;
+            seg_decode(kp, pk, kgs, kd, kq);
+            dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
+          }
+        }
+        const unsigned long long mk = __ballot(dd);
+        if (dd) {
+          const uint32_t at = soff[s] + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+          o.hs[at] = static_cast<int32_t>(s);
+          o.hgs[at] = gs;
+          o.ht[at] = t;
+          o.hq[at] = q;
+        }
+        if (lane == 0) o.hcnt[s] = static_cast<uint32_t>(__popcll(mk));
+        wave_lds_sync();
+        continue;
+      }
+    }
     uint32_t np = 1;
     while (np < n) np <<= 1;
     for (uint32_t i = n + lane; i < np; i += 64) seg_none(L, w, i);
@@ -594,7 +684,6 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       }
     }
     // filters over the sorted seeds, 64 at a time
-    const int qw = a.qwin[s];
     int carry_k = -1, carry_g = -1;
     uint32_t nh = 0;
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
