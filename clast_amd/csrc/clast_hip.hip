@@ -116,6 +116,7 @@ struct clast_index_s {
   DevArray<uint64_t> win_slot;
   DevArray<uint64_t> seq_slot;
   DevArray<int32_t> seq_len, seq_win, seq_ref;
+  DevArray<int2> seq_info;  // (length, window) per target sequence
   uint32_t ncell = 0;
   DevArray<uint32_t> ekey32;
   DevArray<uint64_t> ekey64;
@@ -302,6 +303,11 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
   ix.seq_slot.upload(seq_slot.data(), seq_slot.size(), st);
   ix.seq_len.upload(ix.h_seq_len.data(), ix.h_seq_len.size(), st);
   ix.seq_win.upload(ix.h_seq_win.data(), ix.h_seq_win.size(), st);
+  {
+    std::vector<int2> info(ix.h_seq_len.size());
+    for (size_t i = 0; i < info.size(); ++i) info[i] = make_int2(ix.h_seq_len[i], ix.h_seq_win[i]);
+    ix.seq_info.upload(info.data(), info.size(), st);
+  }
   ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
   DevArray<uint32_t> d_win_stride, d_win_seq;
   d_win_stride.upload(win_stride.data(), win_stride.size(), st);
@@ -504,8 +510,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
                 ix.key64 ? static_cast<const void*>(ix.ekey64.ptr) : static_cast<const void*>(ix.ekey32.ptr),
                 ix.epos.ptr, ix.shift, ix.key64 ? 1 : 0, ix.direct ? 1 : 0};
   a.epos = ix.epos.ptr;
-  a.tlen = ix.seq_len.ptr;
-  a.twin = ix.seq_win.ptr;
+  a.tinfo = ix.seq_info.ptr;
   a.nqwin = q.nwin;
   a.lmer = p.lmer;
   a.gap = p.gap;
@@ -553,8 +558,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t novf = 0;
     k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
     const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
-    const bool packed = gbits + dbits + qbits <= 63;
-    const SegPack sp{d_off, qbits, dbits};
+    const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
+    const bool packed = wbits + gbits + dbits + qbits <= 63;
+    const SegPack sp{d_off, qbits, dbits, gbits};
     uint32_t* big_list = ws.get<uint32_t>("s.big", q.nseq);
     const SegOut so{ss, sgs, stt, sq, hcnt, big_list, tot32 + 6, ovf_list, tot32 + 1, ovf_flag};
     // nwaves: waves to launch (at most one per listed strand)

@@ -316,8 +316,7 @@ struct SeedArgs {
   uint32_t nseq, npos;
   Lookup lk;
   const int2* epos;           // (target sequence, intra-sequence index) per entry
-  const int32_t* tlen;        // per target sequence
-  const int32_t* twin;        // target window of each target sequence
+  const int2* tinfo;          // per target sequence: (length, window)
   int nqwin;
   int lmer, gap, width, local;
   uint32_t* unit_first;       // per (target window, query window): first strand with a seed
@@ -447,10 +446,11 @@ struct SegOut {
   uint8_t* ovf_flag;      // per strand
 };
 
-// LDS image of one strand's seeds.  Packed: one u64 (target sequence, diagonal
-// + d_off, qIdx) when the three fit in 63 bits; otherwise (target sequence,
-// diagonal + d_off) and qIdx apart.  Either sorts as (seq, diagonal, qIdx);
-// seeds the corner filter removes are ~0 and sort last.
+// LDS image of one strand's seeds.  Packed: one u64 (target window, target
+// sequence, diagonal + d_off, qIdx) when the four fit in 63 bits — the
+// window is monotone in the sequence index, so it does not change the order;
+// otherwise (target sequence, diagonal + d_off) and qIdx apart.  Either sorts
+// as (seq, diagonal, qIdx); ~0 pads a list to a power of two and sorts last.
 template <bool PACKED, int CAP>
 struct SegLds;
 template <int CAP>
@@ -467,17 +467,23 @@ struct SegLds<false, CAP> {
 
 struct SegPack {
   int d_off;
-  int qbits, dbits;  // packed layout only
+  int qbits, dbits, gbits;  // packed layout field widths
+};
+
+// One seed: target window and sequence, diagonal, qIdx.
+struct SeedV {
+  int tw, gs, d, q;
 };
 
 template <class Lds>
-__device__ __forceinline__ void seg_put(Lds& L, int w, uint32_t at, const SegPack& pk, int gs, int d, int q) {
+__device__ __forceinline__ void seg_put(Lds& L, int w, uint32_t at, const SegPack& pk, const SeedV& v) {
   if constexpr (Lds::kPacked) {
-    L.k[w][at] = (static_cast<uint64_t>(gs) << (pk.dbits + pk.qbits)) |
-                 (static_cast<uint64_t>(d + pk.d_off) << pk.qbits) | static_cast<uint64_t>(q);
+    L.k[w][at] = (static_cast<uint64_t>(v.tw) << (pk.gbits + pk.dbits + pk.qbits)) |
+                 (static_cast<uint64_t>(v.gs) << (pk.dbits + pk.qbits)) |
+                 (static_cast<uint64_t>(v.d + pk.d_off) << pk.qbits) | static_cast<uint64_t>(v.q);
   } else {
-    L.k[w][at] = (static_cast<uint64_t>(gs) << 32) | static_cast<uint32_t>(d + pk.d_off);
-    L.q[w][at] = q;
+    L.k[w][at] = (static_cast<uint64_t>(v.gs) << 32) | static_cast<uint32_t>(v.d + pk.d_off);
+    L.q[w][at] = v.q;
   }
 }
 
@@ -487,33 +493,40 @@ __device__ __forceinline__ void seg_none(Lds& L, int w, uint32_t at) {
   if constexpr (!Lds::kPacked) L.q[w][at] = INT_MAX;
 }
 
+__device__ __forceinline__ SeedV seg_decode(uint64_t k, const SegPack& pk) {
+  SeedV v;
+  v.tw = static_cast<int>(k >> (pk.gbits + pk.dbits + pk.qbits));
+  v.gs = static_cast<int>((k >> (pk.dbits + pk.qbits)) & ((1ull << pk.gbits) - 1ull));
+  v.d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
+  v.q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
+  return v;
+}
+
+// Seed i of a sorted LDS list (the unpacked form reads the window).
 template <class Lds>
-__device__ __forceinline__ void seg_get(const Lds& L, int w, uint32_t i, const SegPack& pk, int& gs, int& d, int& q) {
+__device__ __forceinline__ SeedV seg_get(const Lds& L, int w, uint32_t i, const SegPack& pk, const int2* tinfo) {
   const uint64_t k = L.k[w][i];
   if constexpr (Lds::kPacked) {
-    gs = static_cast<int>(k >> (pk.dbits + pk.qbits));
-    d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
-    q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
+    return seg_decode(k, pk);
   } else {
-    gs = static_cast<int>(k >> 32);
-    d = static_cast<int>(static_cast<uint32_t>(k)) - pk.d_off;
-    q = L.q[w][i];
+    SeedV v;
+    v.gs = static_cast<int>(k >> 32);
+    v.d = static_cast<int>(static_cast<uint32_t>(k)) - pk.d_off;
+    v.q = L.q[w][i];
+    v.tw = tinfo[v.gs].y;
+    return v;
   }
 }
 
 enum SegMode { kSegFirst = 0, kSegBigPass = 1, kSegRerun = 2 };
 
-__device__ __forceinline__ void seg_decode(uint64_t k, const SegPack& pk, int& gs, int& d, int& q) {
-  gs = static_cast<int>(k >> (pk.dbits + pk.qbits));
-  d = static_cast<int>((k >> pk.qbits) & ((1ull << pk.dbits) - 1ull)) - pk.d_off;
-  q = static_cast<int>(k & ((1ull << pk.qbits) - 1ull));
-}
-
-// Bitonic sort of one u64 per lane across the wavefront (ascending by lane).
-__device__ __forceinline__ uint64_t wave_sort64(uint64_t k) {
+// Bitonic sort of one u64 per lane, lanes [0, N) ascending.  Lanes from N
+// on must hold ~0 (with N < 64 they never exchange with the lower lanes).
+template <int N>
+__device__ __forceinline__ uint64_t wave_sort(uint64_t k) {
   const int lane = lane_id();
 #pragma unroll
-  for (int kk = 2; kk <= 64; kk <<= 1) {
+  for (int kk = 2; kk <= N; kk <<= 1) {
 #pragma unroll
     for (int j = kk >> 1; j > 0; j >>= 1) {
       const uint64_t o = __shfl_xor(k, j, 64);
@@ -524,21 +537,42 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t k) {
   return k;
 }
 
+// Isolate-filter flags of one sorted seed v with predecessor pv (has_pred:
+// v is not the list's first): group head (new target sequence), window head
+// (new target window) and near(v) (…deleteIsolateSeeds.cuh:28-37).
+struct SeedMarks {
+  bool ghead, twhead, nr;
+};
+__device__ __forceinline__ SeedMarks seed_marks(const SeedArgs& a, const SeedV& v, const SeedV& pv, bool has_pred) {
+  SeedMarks m{true, true, false};
+  if (has_pred) {
+    if (pv.gs == v.gs) {
+      m.ghead = false;
+      m.twhead = false;
+      m.nr = near_pair(v.d + v.q, v.d, pv.d + pv.q, pv.d, a.width, a.gap);
+    } else {
+      m.twhead = pv.tw != v.tw;
+    }
+  }
+  return m;
+}
+
 // One wavefront per query strand: its seeds (positions in qIdx order, cells
-// in index order; the corner filter applied) go to LDS, are bitonic-sorted
-// by (target sequence, diagonal, qIdx) — the …sortSeeds.cuh:14-55 order
-// restricted to one qID — and the isolate and duplicate filters run on the
-// sorted list with wave scans: K(i) = near(i) or i is its unit's element 0;
-// D(i) = K(i) and not near(i, previous K of the group).  Surviving seeds are
-// written as hits at the start of the strand's own seed range (their order
-// is fixed later by complete sort keys).
+// in index order; the corner filter applied) go to LDS, are sorted by
+// (target sequence, diagonal, qIdx) — the …sortSeeds.cuh:14-55 order
+// restricted to one qID; in registers for up to 64 seeds, bitonic in LDS
+// above — and the isolate and duplicate filters run on the sorted list with
+// wave scans: K(i) = near(i) or i is its unit's element 0; D(i) = K(i) and
+// not near(i, previous K of the group).  Surviving seeds are written as hits
+// at the start of the strand's own seed range (their order is fixed later by
+// complete sort keys).
 //
 // Element 0 of a unit is the first seed of the unit's first strand with a
 // seed in that target window.  The first pass (every strand; strands over
 // CAP go to the big list) and the big pass (the big list; strands over CAP
-// go to the global path) record each strand as a candidate (unit_first) and
-// take no seed as element 0; the strands that end up first in some unit are
-// then run again (kSegRerun) with unit_first final.
+// go to the global path) record each strand as a candidate (unit_first; the
+// value is read first so the atomic is rare) and take no seed as element 0; the strands that end up first in
+// some unit are then run again (kSegRerun) with unit_first final.
 template <bool PACKED, int CAP>
 static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint4* __restrict__ cellp,
                                                                  const uint32_t* __restrict__ scnt,
@@ -546,11 +580,16 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
                                                                  SegOut o, const uint32_t* __restrict__ list,
                                                                  const uint32_t* __restrict__ nlist, int mode) {
   __shared__ SegLds<PACKED, CAP> L;
-  const int w = wave_id(), lane = lane_id();
+  // wave-uniform strand index: the per-strand header comes through scalar loads
+  const int w = __builtin_amdgcn_readfirstlane(wave_id()), lane = lane_id();
   const bool rerun = mode == kSegRerun;
   const uint32_t count = list ? *nlist : a.nseq;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t* __restrict__ pos_off = a.pos_off;
+  const int32_t* __restrict__ qlen = a.qlen;
+  const int32_t* __restrict__ qwin = a.qwin;
   for (uint32_t k = blockIdx.x * kSegWaves + w; k < count; k += gridDim.x * kSegWaves) {
-    const uint32_t s = list ? list[k] : k;
+    const uint32_t s = __builtin_amdgcn_readfirstlane(list ? list[k] : k);
     const uint32_t n = scnt[s];
     if (n == 0) {
       if (lane == 0) o.hcnt[s] = 0;
@@ -567,94 +606,86 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       }
       continue;
     }
-    const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
-    const int qLen = a.qlen[s];
-    // generate the strand's seeds into LDS: slots from a wave scan of the
-    // positions' entry counts
-    uint32_t base = 0, mine = 0;
+    const uint32_t p0 = pos_off[s], p1 = pos_off[s + 1];
+    const int qLen = qlen[s];
+    const int qw = qwin[s];
+    const uint32_t hbase = soff[s];
+    // generate the strand's seeds into LDS in any order (they are sorted
+    // next): entry j of every lane's cell per round, appended by ballot;
+    // seeds the corner filter removes are not written
+    uint32_t m = 0;
     for (uint32_t c0 = p0; c0 < p1; c0 += 64) {
       const uint32_t p = c0 + lane;
       uint4 cp = make_uint4(0, 0, 0, 0);
       if (p < p1) cp = cellp[p];
-      const uint32_t inc = wave_sum_incl(cp.y);
-      uint32_t at = base + inc - cp.y;
-      base += __shfl(inc, 63, 64);
       const int qIdx = static_cast<int>(p - p0);
-      for (uint32_t j = 0; j < cp.y; ++j, ++at) {
-        const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
-        const int d = e.y - qIdx;
-        if (corner_ok(a, d, qLen, a.tlen[e.x])) {
-          seg_put(L, w, at, pk, e.x, d, qIdx);
-          ++mine;
-        } else {
-          seg_none(L, w, at);
+      for (uint32_t j = 0; __any(j < cp.y); ++j) {
+        bool ok = false;
+        SeedV v{0, 0, 0, qIdx};
+        if (j < cp.y) {
+          const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
+          const int2 ti = a.tinfo[e.x];
+          v.gs = e.x;
+          v.tw = ti.y;
+          v.d = e.y - qIdx;
+          ok = corner_ok(a, v.d, qLen, ti.x);
         }
+        const unsigned long long bm = __ballot(ok);
+        if (ok) seg_put(L, w, m + static_cast<uint32_t>(__popcll(bm & below)), pk, v);
+        m += static_cast<uint32_t>(__popcll(bm));
       }
     }
-    const uint32_t m = __shfl(wave_sum_incl(mine), 63, 64);  // seeds past the corner filter
-    const int qw = a.qwin[s];
+    if (m == 0) {
+      if (lane == 0) o.hcnt[s] = 0;
+      continue;
+    }
     if constexpr (PACKED) {
-      if (n <= 64) {
+      if (m <= 64) {
         // one seed per lane: sort and filter in registers
         wave_lds_sync();
-        const uint64_t key = wave_sort64(lane < static_cast<int>(n) ? L.k[w][lane] : ~0ull);
+        const uint64_t k0 = lane < static_cast<int>(m) ? L.k[w][lane] : ~0ull;
+        const uint64_t key = m <= 8 ? wave_sort<8>(k0) : m <= 16 ? wave_sort<16>(k0)
+                             : m <= 32 ? wave_sort<32>(k0) : wave_sort<64>(k0);
         const bool valid = lane < static_cast<int>(m);
-        int gs, d, q;
-        seg_decode(key, pk, gs, d, q);
-        const int t = d + q;
-        int pgs, pd, pq;
-        seg_decode(__shfl_up(key, 1, 64), pk, pgs, pd, pq);
-        bool ghead = false, kk = false;
+        const SeedV v = seg_decode(key, pk);
+        const SeedV pv = seg_decode(__shfl_up(key, 1, 64), pk);
+        SeedMarks mk{false, false, false};
+        bool kk = false;
         if (valid) {
-          bool twhead = true, nr = false;
-          ghead = true;
-          if (lane > 0) {
-            if (pgs == gs) {
-              ghead = false;
-              twhead = false;
-              nr = near_pair(t, d, pd + pq, pd, a.width, a.gap);
-            } else {
-              twhead = a.twin[pgs] != a.twin[gs];
-            }
-          }
-          const int tw = a.twin[gs];
+          mk = seed_marks(a, v, pv, lane > 0);
           bool elem0 = false;
-          if (twhead) {
-            if (rerun) elem0 = a.unit_first[tw * a.nqwin + qw] == s;
-            else note_unit_first(a, tw, qw, s);
+          if (mk.twhead) {
+            uint32_t* uf = &a.unit_first[v.tw * a.nqwin + qw];
+            if (rerun) elem0 = *uf == s;
+            else if (s < *uf) atomicMin(uf, s);
           }
-          kk = nr || elem0;
+          kk = mk.nr || elem0;
         }
         const int kin = wave_max_scan_incl<int>(kk ? lane : -1);
-        const int gin = wave_max_scan_incl<int>(ghead ? lane : -1);
+        const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? lane : -1);
         int prevk = __shfl_up(kin, 1, 64);
         if (lane == 0) prevk = -1;
-        const uint64_t kp = __shfl(key, prevk < 0 ? 0 : prevk, 64);
+        const SeedV kv = seg_decode(__shfl(key, prevk < 0 ? 0 : prevk, 64), pk);
         bool dd = false;
         if (valid && kk) {
-          dd = true;
-          if (prevk >= 0 && prevk >= gin) {
-            int kgs, kd, kq;
-            seg_decode(kp, pk, kgs, kd, kq);
-            dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
-          }
+          dd = !(prevk >= 0 && prevk >= gin && near_pair(v.d + v.q, v.d, kv.d + kv.q, kv.d, a.width, a.gap));
         }
-        const unsigned long long mk = __ballot(dd);
+        const unsigned long long hm = __ballot(dd);
         if (dd) {
-          const uint32_t at = soff[s] + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+          const uint32_t at = hbase + static_cast<uint32_t>(__popcll(hm & below));
           o.hs[at] = static_cast<int32_t>(s);
-          o.hgs[at] = gs;
-          o.ht[at] = t;
-          o.hq[at] = q;
+          o.hgs[at] = v.gs;
+          o.ht[at] = v.d + v.q;
+          o.hq[at] = v.q;
         }
-        if (lane == 0) o.hcnt[s] = static_cast<uint32_t>(__popcll(mk));
+        if (lane == 0) o.hcnt[s] = static_cast<uint32_t>(__popcll(hm));
         wave_lds_sync();
         continue;
       }
     }
     uint32_t np = 1;
-    while (np < n) np <<= 1;
-    for (uint32_t i = n + lane; i < np; i += 64) seg_none(L, w, i);
+    while (np < m) np <<= 1;
+    for (uint32_t i = m + lane; i < np; i += 64) seg_none(L, w, i);
     wave_lds_sync();
     // bitonic sort ascending
     for (uint32_t kk = 2; kk <= np; kk <<= 1) {
@@ -689,34 +720,23 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
       const uint32_t i = c0 + lane;
       const bool valid = i < m;
-      bool ghead = false, nr = false, kk = false;
-      int gs = 0, d = 0, t = 0, q = 0;
+      SeedMarks mk{false, false, false};
+      SeedV v{0, 0, 0, 0};
+      bool kk = false;
       if (valid) {
-        seg_get(L, w, i, pk, gs, d, q);
-        t = d + q;
-        bool twhead = true;
-        ghead = true;
-        if (i > 0) {
-          int pgs, pd, pq;
-          seg_get(L, w, i - 1, pk, pgs, pd, pq);
-          if (pgs == gs) {
-            ghead = false;
-            twhead = false;
-            nr = near_pair(t, d, pd + pq, pd, a.width, a.gap);
-          } else {
-            twhead = a.twin[pgs] != a.twin[gs];
-          }
-        }
-        const int tw = a.twin[gs];
+        v = seg_get(L, w, i, pk, a.tinfo);
+        const SeedV pv = i > 0 ? seg_get(L, w, i - 1, pk, a.tinfo) : v;
+        mk = seed_marks(a, v, pv, i > 0);
         bool elem0 = false;
-        if (twhead) {
-          if (rerun) elem0 = a.unit_first[tw * a.nqwin + qw] == s;
-          else note_unit_first(a, tw, qw, s);
+        if (mk.twhead) {
+          uint32_t* uf = &a.unit_first[v.tw * a.nqwin + qw];
+          if (rerun) elem0 = *uf == s;
+          else if (s < *uf) atomicMin(uf, s);
         }
-        kk = nr || elem0;
+        kk = mk.nr || elem0;
       }
       const int kin = wave_max_scan_incl<int>(kk ? static_cast<int>(i) : -1);
-      const int gin = wave_max_scan_incl<int>(ghead ? static_cast<int>(i) : -1);
+      const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? static_cast<int>(i) : -1);
       int kex = __shfl_up(kin, 1, 64);
       if (lane == 0) kex = -1;
       const int prevk = kex > carry_k ? kex : carry_k;
@@ -725,20 +745,19 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       if (valid && kk) {
         dd = true;
         if (prevk >= 0 && prevk >= gstart) {
-          int kgs, kd, kq;
-          seg_get(L, w, static_cast<uint32_t>(prevk), pk, kgs, kd, kq);
-          dd = !near_pair(t, d, kd + kq, kd, a.width, a.gap);
+          const SeedV kv = seg_get(L, w, static_cast<uint32_t>(prevk), pk, a.tinfo);
+          dd = !near_pair(v.d + v.q, v.d, kv.d + kv.q, kv.d, a.width, a.gap);
         }
       }
-      const unsigned long long mk = __ballot(dd);
+      const unsigned long long hm = __ballot(dd);
       if (dd) {
-        const uint32_t at = soff[s] + nh + static_cast<uint32_t>(__popcll(mk & ((1ull << lane) - 1ull)));
+        const uint32_t at = hbase + nh + static_cast<uint32_t>(__popcll(hm & below));
         o.hs[at] = static_cast<int32_t>(s);
-        o.hgs[at] = gs;
-        o.ht[at] = t;
-        o.hq[at] = q;
+        o.hgs[at] = v.gs;
+        o.ht[at] = v.d + v.q;
+        o.hq[at] = v.q;
       }
-      nh += static_cast<uint32_t>(__popcll(mk));
+      nh += static_cast<uint32_t>(__popcll(hm));
       const int lk = __shfl(kin, 63, 64), lg = __shfl(gin, 63, 64);
       carry_k = lk > carry_k ? lk : carry_k;
       carry_g = lg > carry_g ? lg : carry_g;
@@ -820,9 +839,10 @@ static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPa
   for (uint32_t j = 0; j < cp.y; ++j, ++o) {
     const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
     const int d = e.y - qIdx;
-    if (corner_ok(a, d, qLen, a.tlen[e.x])) {
+    const int2 ti = a.tinfo[e.x];
+    if (corner_ok(a, d, qLen, ti.x)) {
       skey[o] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
-      note_unit_first(a, a.twin[e.x], qw, s);
+      note_unit_first(a, ti.y, qw, s);
     } else {
       skey[o] = sk | ((1ull << (pk.gs_bits + pk.d_bits)) - 1ull);
     }
