@@ -148,11 +148,25 @@ struct clast_hits_s {
   // qHL match score twin qwin), then n E-values
   clast::PinnedPool::Block rows;
   int64_t n = 0;
+  hipEvent_t ready = nullptr;  // the rows' copy into `rows` has finished
+  void wait() {
+    if (ready) {
+      clast::hip_check(hipEventSynchronize(ready), "hipEventSynchronize");
+      (void)hipEventDestroy(ready);
+      ready = nullptr;
+    }
+  }
   const int32_t* col(int c) const { return static_cast<const int32_t*>(rows.ptr) + static_cast<int64_t>(c) * n; }
   const double* ev() const {
     return reinterpret_cast<const double*>(static_cast<const int32_t*>(rows.ptr) + 10 * n);
   }
-  ~clast_hits_s() { clast::pinned_pool().release(rows); }
+  ~clast_hits_s() {
+    if (ready) {
+      (void)hipEventSynchronize(ready);
+      (void)hipEventDestroy(ready);
+    }
+    clast::pinned_pool().release(rows);
+  }
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -838,23 +852,38 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       rt.check_launch("k_pair_place");
     }
   }
-  // gather + D2H
+  // gather, then the copy to pinned host memory on the copy stream (the
+  // caller's reads wait for it; the next search's kernels do not)
+  out.wait();
   clast::pinned_pool().release(out.rows);
   out.rows = {};
   out.n = 0;
   if (nfinal) {
-    int32_t* o32 = ws.get<int32_t>("o.i32", static_cast<uint64_t>(nfinal) * 10);
-    double* oev = ws.get<double>("o.ev", nfinal);
+    const int slot = rt.next_out_slot();
+    const std::string tag = slot ? "o1" : "o0";
+    // staging buffer `slot` is free once the copy that last read it is done
+    if (rt.out_used(slot)) hip_check(hipStreamWaitEvent(st, rt.out_done(slot), 0), "hipStreamWaitEvent");
+    uint8_t* ob = ws.get<uint8_t>(tag, static_cast<uint64_t>(nfinal) * 48);
+    int32_t* o32 = reinterpret_cast<int32_t*>(ob);
+    double* oev = reinterpret_cast<double*>(ob + static_cast<uint64_t>(nfinal) * 40);
     OutArgs oa{nfinal, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
                ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
     k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
     rt.check_launch("k_gather_out");
     out.rows = clast::pinned_pool().acquire(static_cast<uint64_t>(nfinal) * 48);
     out.n = nfinal;
-    hip_check(hipMemcpyAsync(out.rows.ptr, o32, sizeof(int32_t) * 10 * static_cast<uint64_t>(nfinal),
-                             hipMemcpyDeviceToHost, st), "D2H rows");
-    hip_check(hipMemcpyAsync(const_cast<double*>(out.ev()), oev, sizeof(double) * nfinal, hipMemcpyDeviceToHost, st),
-              "D2H E-values");
+    hipEvent_t gathered;
+    hip_check(hipEventCreateWithFlags(&gathered, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(gathered, st), "hipEventRecord");
+    hipStream_t cs = rt.copy_stream();
+    hip_check(hipStreamWaitEvent(cs, gathered, 0), "hipStreamWaitEvent");
+    (void)hipEventDestroy(gathered);  // released once the wait is satisfied
+    hip_check(hipMemcpyAsync(out.rows.ptr, ob, static_cast<uint64_t>(nfinal) * 48, hipMemcpyDeviceToHost, cs),
+              "D2H rows");
+    hip_check(hipEventRecord(rt.out_done(slot), cs), "hipEventRecord");
+    rt.set_out_used(slot);
+    hip_check(hipEventCreateWithFlags(&out.ready, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(out.ready, cs), "hipEventRecord");
   }
   mark(EV_RES1);
   rt.sync();
@@ -990,12 +1019,20 @@ int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, in
 
 int64_t clast_hits_count(clast_hits h) { return h ? h->n : -1; }
 
+int clast_hits_wait(clast_hits h) {
+  return guard([&] {
+    if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
+    h->wait();
+  });
+}
+
 int clast_hits_copy(clast_hits h, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
                     int32_t* qHL, int32_t* match, int32_t* score, double* evalue) {
   return guard([&] {
     if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
     int32_t* dst[8] = {tID, tIdx, qID, qIdx, tHL, qHL, match, score};
     if (!h->n) return;
+    h->wait();
     for (int c = 0; c < 8; ++c)
       if (dst[c]) std::memcpy(dst[c], h->col(c), static_cast<size_t>(h->n) * sizeof(int32_t));
     if (evalue) std::memcpy(evalue, h->ev(), static_cast<size_t>(h->n) * sizeof(double));
@@ -1006,6 +1043,7 @@ int clast_hits_copy_windows(clast_hits h, int32_t* twin, int32_t* qwin) {
   return guard([&] {
     if (!h) throw ClastError(CLAST_ERR_ARG, "null hits");
     if (!h->n) return;
+    h->wait();
     if (twin) std::memcpy(twin, h->col(8), static_cast<size_t>(h->n) * sizeof(int32_t));
     if (qwin) std::memcpy(qwin, h->col(9), static_cast<size_t>(h->n) * sizeof(int32_t));
   });

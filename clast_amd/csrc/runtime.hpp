@@ -1,6 +1,7 @@
-// Per-device runtime of libclast_hip: one HIP stream (default priority), a
-// growable named workspace for stage scratch, pinned staging for scalar
-// read-backs, and the error type that the C ABI turns into status codes.
+// Per-device runtime of libclast_hip: a compute stream and a copy stream
+// (both default priority), a growable named workspace for stage scratch,
+// pinned staging for scalar read-backs, and the error type that the C ABI
+// turns into status codes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -56,7 +57,7 @@ struct DevArray {
 // queued kernel still uses the old buffer.
 class Workspace {
  public:
-  explicit Workspace(hipStream_t s) : stream_(s) {}
+  Workspace(hipStream_t s, hipStream_t copy) : stream_(s), copy_(copy) {}
   ~Workspace() {
     for (auto& kv : slots_) (void)hipFree(kv.second.ptr);
   }
@@ -67,6 +68,7 @@ class Workspace {
     if (s.bytes < bytes) {
       if (s.ptr) {
         hip_check(hipStreamSynchronize(stream_), "workspace sync");
+        hip_check(hipStreamSynchronize(copy_), "workspace sync");
         hip_check(hipFree(s.ptr), "hipFree");
         s.ptr = nullptr;
         s.bytes = 0;
@@ -88,7 +90,7 @@ class Workspace {
     void* ptr = nullptr;
     uint64_t bytes = 0;
   };
-  hipStream_t stream_;
+  hipStream_t stream_, copy_;
   std::map<std::string, Slot> slots_;
 };
 
@@ -97,15 +99,28 @@ class Runtime {
   explicit Runtime(int device) : device_(device) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : out_done_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipHostMalloc(&pinned_, kPinned, hipHostMallocDefault), "hipHostMalloc");
-    ws_ = new Workspace(stream_);
+    ws_ = new Workspace(stream_, copy_stream_);
   }
   ~Runtime() {
     delete ws_;
     (void)hipHostFree(pinned_);
+    for (auto& e : out_done_) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(copy_stream_);
     (void)hipStreamDestroy(stream_);
   }
   hipStream_t stream() const { return stream_; }
+  // Device-to-host copies of result rows run here, so one search's copy
+  // overlaps the next search's kernels.
+  hipStream_t copy_stream() const { return copy_stream_; }
+  // Result rows are staged in one of two device buffers, alternating; the
+  // event marks the end of the last copy out of buffer k.
+  int next_out_slot() { return out_slot_ ^= 1; }
+  hipEvent_t out_done(int k) const { return out_done_[k]; }
+  bool out_used(int k) const { return out_used_[k]; }
+  void set_out_used(int k) { out_used_[k] = true; }
   int device() const { return device_; }
   Workspace& ws() { return *ws_; }
   std::mutex& mutex() { return mu_; }
@@ -125,6 +140,10 @@ class Runtime {
   static constexpr size_t kPinned = 4096;
   int device_;
   hipStream_t stream_ = nullptr;
+  hipStream_t copy_stream_ = nullptr;
+  hipEvent_t out_done_[2] = {nullptr, nullptr};
+  bool out_used_[2] = {false, false};
+  int out_slot_ = 0;
   void* pinned_ = nullptr;
   Workspace* ws_ = nullptr;
   std::mutex mu_;

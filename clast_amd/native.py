@@ -89,6 +89,7 @@ def hip() -> C.CDLL:
             "clast_search": (C.c_int, [pp, vp, vp, i32, i32, C.POINTER(vp)]),
             "clast_search_batch": (C.c_int, [pp, vp, vp, i32, C.POINTER(vp)]),
             "clast_hits_count": (i64, [vp]),
+            "clast_hits_wait": (C.c_int, [vp]),
             "clast_hits_copy": (C.c_int, [vp] + [vp] * 9),
             "clast_hits_copy_windows": (C.c_int, [vp, vp, vp]),
             "clast_hits_stage_count": (i64, [vp, i32]),
@@ -320,6 +321,11 @@ class Hits:
 
     def __len__(self) -> int:
         return int(hip().clast_hits_count(self.h))
+
+    def wait(self) -> None:
+        """Blocks until the rows are in host memory (they arrive after the
+        search returns, overlapping whatever the caller launches next)."""
+        _check(hip().clast_hits_wait(self.h), "clast_hits_wait")
 
     def arrays(self) -> dict[str, np.ndarray]:
         n = len(self)

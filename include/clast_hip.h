@@ -132,8 +132,14 @@ int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, in
                        clast_hits* out);
 
 /* CDeviceHitList::getResult (CDeviceHitList.cu:425-451): the 9 result
- * arrays (tID and qID are list IDs).  Null pointers are skipped. */
+ * arrays (tID and qID are list IDs).  Null pointers are skipped.  The rows
+ * reach host memory asynchronously after a search returns (so the copy
+ * overlaps the next search); clast_hits_copy and clast_hits_copy_windows
+ * wait for them, clast_hits_count does not need to. */
 int64_t clast_hits_count(clast_hits h);
+/* No counterpart in the reference: blocks until the rows of h are in host
+ * memory (what clast_hits_copy does before copying). */
+int clast_hits_wait(clast_hits h);
 int clast_hits_copy(clast_hits h, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
                     int32_t* qHL, int32_t* match, int32_t* score, double* evalue);
 /* Target / query window index of each result row (batched search). */
@@ -144,10 +150,13 @@ int64_t clast_hits_stage_count(clast_hits h, int32_t stage);
 int clast_hits_stage_copy(clast_hits h, int32_t stage, int32_t* tID, int32_t* tIdx, int32_t* qID,
                           int32_t* qIdx, int32_t* tHL, int32_t* qHL, int32_t* match, int32_t* score);
 /* Device time of the search's stages in milliseconds, from hipEvents on the
- * library stream: out[0] seed lookup/count kernel, [1] seed write kernel,
- * [2] seed sort, [3] isolate/duplicate stage, [4] DP kernel, [5] E-value +
- * filters + result order + copy-out, [6] whole search; out[7] is the number
- * of seeds (after the boundary filters) as a float. */
+ * library's compute stream: out[0] seed lookup kernel, [1] seed generation,
+ * per-strand sort and isolate/duplicate filters (LDS passes, global-path
+ * seed write, first-strand re-run), [2] global-path seed sort, [3]
+ * global-path filters, [4] DP kernel, [5] E-value + filters + result order
+ * + gather, [6] whole search (the copy to host memory runs on a second
+ * stream and is not included); out[7] is the number of raw seeds (index
+ * entries matched, before the corner filter) as a float. */
 int clast_hits_stage_times(clast_hits h, float* out8);
 int clast_hits_destroy(clast_hits h);
 
