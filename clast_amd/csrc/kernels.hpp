@@ -22,33 +22,29 @@ namespace clast {
 
 // ---------------------------------------------------------------- bases
 
-// Base classes.  Every base byte is stored on the device as a class code:
-// A/C/G/T one-hot in bits 0-3, any other letter 0x10 (never matches, N vs N
-// included), E F I J L O P Q Z 0x20 (scores -1), X 0x40 (bad area) — the
-// 26x26 POINT_MATRIX / MATCH_COUNTER of krnlMatrix.cuh:11-67 as a rule.
-constexpr uint32_t kClassX = 0x40u;
-constexpr uint32_t kClassStar = 0x20u;
-constexpr uint32_t kClassOther = 0x10u;
+// Base classes.  Every base byte is stored on the device as a class index:
+// A 0, C 1, G 2, T 3 (the 2-bit k-mer codes of SRead2bit.cuh:12-18), any
+// other letter 4 (never matches, N vs N included), E F I J L O P Q Z 5
+// (scores -1), X 6 (bad area) — the 26x26 POINT_MATRIX / MATCH_COUNTER of
+// krnlMatrix.cuh:11-67 as a rule.
+constexpr uint32_t kClsOther = 4u, kClsStar = 5u, kClsX = 6u;
 
 __device__ __forceinline__ uint32_t dp_class(uint32_t c) {
   const uint32_t idx = c - 'A';
   constexpr uint32_t star = (1u << 4) | (1u << 5) | (1u << 8) | (1u << 9) | (1u << 11) | (1u << 14) | (1u << 15) |
                             (1u << 16) | (1u << 25);  // E F I J L O P Q Z
-  uint32_t v = (idx < 26u && ((star >> idx) & 1u)) ? kClassStar : kClassOther;
-  v = c == 'X' ? kClassX : v;
-  v = c == 'A' ? 1u : v;
-  v = c == 'C' ? 2u : v;
-  v = c == 'G' ? 4u : v;
-  v = c == 'T' ? 8u : v;
+  uint32_t v = (idx < 26u && ((star >> idx) & 1u)) ? kClsStar : kClsOther;
+  v = c == 'X' ? kClsX : v;
+  v = c == 'A' ? 0u : v;
+  v = c == 'C' ? 1u : v;
+  v = c == 'G' ? 2u : v;
+  v = c == 'T' ? 3u : v;
   return v;
 }
 
-// 2-bit code of a class (A0 C1 G2 T3, SRead2bit.cuh:12-18); 4 for anything
-// that is not A/C/G/T (SRepelHasOddBase.cuh:6-14 makes the k-mer -1).
-__device__ __forceinline__ uint32_t base_code(uint32_t cls) {
-  const uint32_t b = cls & 0xFu;
-  return b ? static_cast<uint32_t>(__builtin_ctz(b)) : 4u;
-}
+// 2-bit code of a class (A0 C1 G2 T3, SRead2bit.cuh:12-18); 4 or more for
+// anything that is not A/C/G/T (SRepelHasOddBase.cuh:6-14 makes the k-mer -1).
+__device__ __forceinline__ uint32_t base_code(uint32_t cls) { return cls; }
 
 // Bytes -> class codes, in place (once per upload).
 static __global__ __launch_bounds__(kBlock) void k_classify(uint8_t* __restrict__ p, uint64_t n) {
@@ -1016,40 +1012,72 @@ struct DPArgs {
 
 constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
 
-// Sequential reader of one DP lane's bytes: indexes g, g + step, g + 2 step,
-// ... of a device byte array padded to whole 16-byte blocks, read a block
-// at a time.  Entering a block requests the one after it, so a load is in
-// flight 16 rows ahead, and one load serves 16 rows: the 64 scattered
-// target and query lines of a wavefront are fetched once each instead of
-// once per row (per-row byte loads thrash L2 once enough wavefronts are
-// resident).  Blocks outside the array are clamped; their bytes are never
-// used.
-struct ByteStream {
+// Sequential reader of one DP lane's bytes — indexes g, g + step, g + 2 step,
+// ... of a device byte array padded to whole 16-byte blocks — in groups of
+// 16 on a schedule shared by the whole wavefront (every lane is at the same
+// stream position j, the row counter): entering a group builds its 16 bytes
+// from two aligned blocks loaded one group earlier and requests the next
+// group's blocks, so loads land directly in their registers and are not
+// waited for until 16 rows later.  One pair of loads serves 16 rows: the
+// scattered target and query lines of a wavefront are fetched once each.
+// Blocks outside the array are clamped; their bytes are never used.
+struct ByteWin {
   const uint4* blk;
-  int64_t g;
-  int32_t cb, bmax, step;
-  uint4 cur, nxt;
+  int32_t bi, bmax, step;  // next group's first block index, last block, +1 / -1
+  uint32_t d, bs;          // the window's dword / byte offset inside its blocks (fixed per lane)
+  uint4 pa, pb;            // blocks of the next group
+  uint32_t w0, w1, w2, w3; // current group, stream order, low byte first
 
+  __device__ __forceinline__ void request() {
+    pa = blk[min(max(bi, 0), bmax)];
+    pb = blk[min(max(bi + 1, 0), bmax)];
+    bi += step;
+  }
   __device__ __forceinline__ void init(const uint8_t* base, int32_t last_block, int64_t g0, int st) {
     blk = reinterpret_cast<const uint4*>(base);
     bmax = last_block;
     step = st;
-    g = g0;
-    cb = static_cast<int32_t>(g0 >> 4);
-    cur = blk[min(max(cb, 0), bmax)];
-    nxt = blk[min(max(cb + step, 0), bmax)];
+    // first group's memory range: [g0, g0 + 15] forward, [g0 - 15, g0] backward
+    const int64_t lo = st > 0 ? g0 : g0 - 15;
+    bi = static_cast<int32_t>(lo >> 4);
+    const uint32_t off = static_cast<uint32_t>(lo & 15);
+    d = off >> 2;
+    bs = off & 3u;
+    request();
   }
-  __device__ __forceinline__ uint32_t next() {
-    const int32_t b = static_cast<int32_t>(g >> 4);
-    if (b != cb) {
-      cb = b;
-      cur = nxt;
-      nxt = blk[min(max(b + step, 0), bmax)];
+  __device__ __forceinline__ void refill() {
+    const uint32_t x0 = pa.x, x1 = pa.y, x2 = pa.z, x3 = pa.w, x4 = pb.x, x5 = pb.y, x6 = pb.z, x7 = pb.w;
+    // y[j] = x[j + d]: shift by one dword on bit 0 of d, then by two on bit 1
+    const bool s1 = (d & 1u) != 0u, s2 = (d & 2u) != 0u;
+    const uint32_t a0 = s1 ? x1 : x0, a1 = s1 ? x2 : x1, a2 = s1 ? x3 : x2, a3 = s1 ? x4 : x3;
+    const uint32_t a4 = s1 ? x5 : x4, a5 = s1 ? x6 : x5, a6 = s1 ? x7 : x6;
+    const uint32_t y0 = s2 ? a2 : a0, y1 = s2 ? a3 : a1, y2 = s2 ? a4 : a2, y3 = s2 ? a5 : a3, y4 = s2 ? a6 : a4;
+    // bytes off .. off+15 of the two blocks
+    const uint32_t m0 = __builtin_amdgcn_alignbyte(y1, y0, bs);
+    const uint32_t m1 = __builtin_amdgcn_alignbyte(y2, y1, bs);
+    const uint32_t m2 = __builtin_amdgcn_alignbyte(y3, y2, bs);
+    const uint32_t m3 = __builtin_amdgcn_alignbyte(y4, y3, bs);
+    if (step > 0) {
+      w0 = m0; w1 = m1; w2 = m2; w3 = m3;
+    } else {  // backward: stream order is memory order reversed
+      w0 = __builtin_amdgcn_perm(0u, m3, 0x00010203u);
+      w1 = __builtin_amdgcn_perm(0u, m2, 0x00010203u);
+      w2 = __builtin_amdgcn_perm(0u, m1, 0x00010203u);
+      w3 = __builtin_amdgcn_perm(0u, m0, 0x00010203u);
     }
-    const uint32_t sel = static_cast<uint32_t>(g >> 2) & 3u;
-    const uint32_t word = sel == 0 ? cur.x : sel == 1 ? cur.y : sel == 2 ? cur.z : cur.w;
-    const uint32_t v = (word >> ((static_cast<uint32_t>(g) & 3u) * 8u)) & 0xFFu;
-    g += step;
+    request();
+  }
+  // Byte at stream position j; j must be the same in every lane and advance
+  // by one per call.
+  __device__ __forceinline__ uint32_t next(int j) {
+    if ((j & 15) == 0) refill();
+    const uint32_t v = w0 & 0xFFu;
+    w0 >>= 8;
+    if ((j & 3) == 3) {
+      w0 = w1;
+      w1 = w2;
+      w2 = w3;
+    }
     return v;
   }
 };
@@ -1059,7 +1087,20 @@ struct ByteStream {
 // one-sided guard — backward ti < tLen, forward ti >= 0 (krnlAlignment.cu:
 // 121-127,160-165 / 234-240,271-276).  Anything outside is 'X'.
 __device__ __forceinline__ uint32_t t_mask(uint32_t v, int ti, int lo, int hi) {
-  return ti >= lo && ti < hi ? v : kClassX;
+  return 4u * (ti >= lo && ti < hi ? v : kClsX);
+}
+
+// Substitution score (POINT_MATRIX) and match-count increment
+// (MATCH_COUNTER) of query class qc against each target class, one signed
+// 4-bit field per target class (field 4·tc); -8 stands for the X value
+// -1000.  Fields for target A C G T other: -3 / 0, the match +1 / +1;
+// E F I J L O P Q Z: -1 / -1; X: -1000 / -1000.
+__device__ __forceinline__ void row_words(uint32_t qc, uint32_t& wpm, uint32_t& wmc) {
+  constexpr uint32_t kPm = 0x8FDDDDDu, kMc = 0x8F00000u;  // query class with no match in the table
+  constexpr uint32_t kStar = 0x8FFFFFFu, kAllX = 0x8888888u;
+  const uint32_t sh = 4u * (qc & 3u);
+  wpm = qc < 4u ? kPm ^ (0xCu << sh) : qc == kClsOther ? kPm : qc == kClsStar ? kStar : kAllX;
+  wmc = qc < 4u ? kMc | (0x1u << sh) : qc == kClsOther ? kMc : qc == kClsStar ? kStar : kAllX;
 }
 
 // One query row of the band (calcutlateCell, krnlAlignment.cu:42-77, for
@@ -1069,45 +1110,43 @@ __device__ __forceinline__ uint32_t t_mask(uint32_t v, int ti, int lo, int hi) {
 // array.
 template <int W, bool LOCAL>
 __device__ __forceinline__ void dp_row(int (&S)[W + 2], int (&V)[W + 2], int (&M)[W + 2], const uint32_t (&T)[W],
-                                       uint32_t qc, int r, int& bestS, int& bestM, int& bestR, int& bestB) {
+                                       uint32_t wpm, uint32_t wmc, int r, int& bestS, int& bestM, int& bestR,
+                                       int& bestB) {
   int hp = -30000;  // band 0's horizontal value (initTempNodeArray margin)
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
     const int v = max(S[b + 1] + (kGapOpen + kGapExt), V[b + 1] + kGapExt);
     const int h = max(S[b - 1] + (kGapOpen + kGapExt), hp + kGapExt);
-    const uint32_t tc = T[b - 1];
-    const uint32_t u = tc | qc;
-    const bool m = (tc & qc & 0xFu) != 0u;
-    const int pm = (u & kClassX) ? kBad : ((u & kClassStar) ? -1 : (m ? kMatch : kMismatch));
-    const int mc = (u & kClassX) ? kBad : ((u & kClassStar) ? -1 : (m ? 1 : 0));
+    // T holds 4 x the target class: the field of this target in the row words
+    const int pr = __builtin_amdgcn_sbfe(static_cast<int>(wpm), T[b - 1], 4);
+    const int mr = __builtin_amdgcn_sbfe(static_cast<int>(wmc), T[b - 1], 4);
+    const bool bad = pr == -8;
+    const int pm = bad ? kBad : pr;
+    const int mc = bad ? kBad : mr;
     const int pt = S[b] + pm;
-    int ns, nm;
-    if (pt > max(v, h)) {
-      ns = pt;
-      nm = M[b] + mc;
-    } else if (v > h) {
-      ns = v;
-      nm = M[b + 1];
-    } else {
-      ns = h;
-      nm = M[b - 1];
-    }
+    // the diagonal wins only if strictly above both gaps, then v over h
+    // unless h is at least as high; the score is the maximum either way.
+    // The match count is selected with masks so that no lane branches.
+    const int vh = max(v, h);
+    const int nm_gap = v > h ? M[b + 1] : M[b - 1];
+    const int nm_diag = M[b] + mc;
+    const int dmask = -static_cast<int>(pt > vh);
+    const int nm = nm_gap ^ ((nm_gap ^ nm_diag) & dmask);
+    const int ns = max(pt, vh);
     S[b] = ns;
     V[b] = v;
     M[b] = nm;
     hp = h;
     if (LOCAL) {
-      if (bestS < ns) {
-        bestS = ns;
-        bestM = nm;
-        bestR = r;
-        bestB = b - 1;
-      }
+      const bool better = bestS < ns;
+      bestS = better ? ns : bestS;
+      bestM = better ? nm : bestM;
+      bestR = better ? r : bestR;
+      bestB = better ? b - 1 : bestB;
     }
   }
 }
 
-// initTempNodeArray (krnlAlignment.cu:6-38) for one hit's band.
 template <int G>
 __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3], int (&M)[2 * G + 3]) {
   constexpr int half = G + 1;
@@ -1130,9 +1169,9 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
 // tIdx/qIdx, which the backward pass does not change), so one thread runs
 // both and sums the results.  Hits arrive ordered by (read length, seed
 // qIdx), so the lanes of a wavefront run near-equal row counts in both
-// loops.  Query and target bytes arrive through 16-byte block streams.
+// loops.  Query and target bytes arrive through 16-byte windows (ByteWin).
 template <int G, bool LOCAL>
-static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= a.n) return;
@@ -1161,19 +1200,21 @@ static __global__ __launch_bounds__(kBlock) void k_dp(DPArgs a) {
     const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
     const int lo = bwd ? wlo : max(wlo, 0);
     const int hi = bwd ? min(whi, tLen) : whi;
-    ByteStream tst, qst;
+    ByteWin tst, qst;
     tst.init(a.tbase, a.t_bmax, static_cast<int64_t>(tslot) + tb, ts);
     qst.init(a.qbase, a.q_bmax, qg + (bwd ? qIdx + L : qIdx - 1), ts);
 #pragma unroll
-    for (int k = 0; k < W; ++k) T[k] = t_mask(tst.next(), tb + ts * k, lo, hi);
+    for (int k = 0; k < W; ++k) T[k] = t_mask(tst.next(k), tb + ts * k, lo, hi);
     int bestS = 0, bestM = 0, bestR = -1, bestB = G;
     int tn_i = tb + ts * W;
 #pragma unroll 1
     for (int r = 0; r < rows; ++r) {
-      const uint32_t qc = qst.next();
-      const uint32_t tn = t_mask(tst.next(), tn_i, lo, hi);
+      const int ru = __builtin_amdgcn_readfirstlane(r);  // every active lane is at row r
+      uint32_t wpm, wmc;
+      row_words(qst.next(ru), wpm, wmc);
+      const uint32_t tn = t_mask(tst.next(ru + W), tn_i, lo, hi);
       tn_i += ts;
-      dp_row<W, LOCAL>(S, V, M, T, qc, r, bestS, bestM, bestR, bestB);
+      dp_row<W, LOCAL>(S, V, M, T, wpm, wmc, r, bestS, bestM, bestR, bestB);
 #pragma unroll
       for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
       T[W - 1] = tn;
