@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_INT32_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, one int32 op per lane per clock
 
 
 def log(msg: str) -> None:
@@ -195,7 +196,7 @@ def main() -> int:
             for ix in indexes:
                 h = native.search_batch(p, ix, q)
                 rows += len(h)
-                record.append(h.times())
+                record.append((h.times(), h.counts()))
                 if prev is not None:
                     prev.close()
                 prev = h
@@ -226,7 +227,8 @@ def main() -> int:
             lat.append((time.perf_counter() - ts) * 1e3)
             h.close()
     lat = np.array(lat)
-    times = np.array(rec)  # per search: count, segments, sort, filt, dp, res, total, raw seeds
+    times = np.array([x[0] for x in rec])  # per search: lookup, segments, sort, filt, dp, res, total, raw seeds
+    counts = np.array([x[1] for x in rec], dtype=np.float64)  # raw seeds, DP hits, DP cells, rows
     # k_seed_count algorithmic bytes per launch: 16 B per query k-mer position
     # (8 B index key lookup + 8 B cell (offset, count)); positions per launch
     # are taken from the query chunks
@@ -241,10 +243,26 @@ def main() -> int:
     bytes_count = 16.0 * pos_per_launch
     achieved = float(bytes_count.sum() / (count_ms.sum() / 1e3) / 1e9)
     traffic = measured_traffic("k_seed_count") if args.workload == "large" and args.scale == 1.0 else None
+    # per-stage work against MI355X peaks (algorithmic bytes / ops, SURVEY-style
+    # conventions): lookup 16 B per query k-mer; segments 24 B per raw seed
+    # (expansion) + 32 B (sort) + 16 B (isolate/duplicate pass) + 16 B per hit
+    # out; DP 14 int32 ops per cell; results 40 B per hit in + 48 B per row out
+    sec = lambda col: float(times[:, col].sum()) / 1e3  # noqa: E731
+    seeds_raw, dp_hits, cells, out_rows = (float(counts[:, c].sum()) for c in range(4))
+    seg_bytes = 72.0 * seeds_raw + 16.0 * dp_hits
+    res_bytes = 40.0 * dp_hits + 48.0 * out_rows
+    stages = {
+        "seed_lookup": {"GB/s": round(bytes_count.sum() / sec(0) / 1e9, 1), "peak": HBM_PEAK_GBS},
+        "seed_segments": {"GB/s": round(seg_bytes / sec(1) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "raw_seeds_per_search": int(seeds_raw / len(counts))},
+        "dp": {"GCUPS": round(cells / sec(4) / 1e9, 1), "Tops/s": round(14.0 * cells / sec(4) / 1e12, 2),
+               "peak_Tops/s": VALU_INT32_PEAK_TOPS, "hits_per_search": int(dp_hits / len(counts)),
+               "cells_per_search": int(cells / len(counts))},
+        "results": {"GB/s": round(res_bytes / sec(5) / 1e9, 1), "peak": HBM_PEAK_GBS},
+    }
     stage_names = ["seed_lookup", "seed_segments", "global_seed_sort", "global_seed_filters", "dp", "results",
                    "search_total"]
     stage_ms = {n: round(float(times[:, i].sum() / args.steps), 3) for i, n in enumerate(stage_names)}
-    dp_ms = float(times[:, 4].sum())
     log(f"stage ms per step: {stage_ms}")
 
     out = {
@@ -271,6 +289,7 @@ def main() -> int:
                        "p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "stage_ms_per_step": stage_ms,
+        "stages": stages,
         "index_build_s": round(index_s, 3),
         "roofline": {"kernel": "k_seed_count", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -170,6 +170,7 @@ struct clast_hits_s {
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t counts[4] = {0, 0, 0, 0};  // raw seeds, DP hits, DP cells, rows
 };
 
 namespace {
@@ -506,7 +507,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 2);
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 2);
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 4);
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 4, 0ull);
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
@@ -774,7 +776,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     const uint32_t g = grid_for(nhit, kBlock);
     k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, hgs, ht, hq, osc, qhl, q.qlen.ptr, ix.seq_win.ptr, q.smin.ptr,
                                        use_cut ? 1 : 0, p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev,
-                                       kflag, rk);
+                                       kflag, rk, p.lmer, 2 * p.gap + 1,
+                                       reinterpret_cast<unsigned long long*>(tot64 + 2));
     rt.check_launch("k_hit_filter");
     PairArgs pa{nhit, q.nseq, hbase, hs, hgs, ot, oq, kflag, rk};
     k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 4, 1, 0u);
@@ -896,6 +899,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
+  uint64_t cells = 0;
+  rt.read_scalars(tot64 + 2, &cells, 1);
+  out.counts[0] = nseed;
+  out.counts[1] = nhit;
+  out.counts[2] = static_cast<int64_t>(cells);
+  out.counts[3] = nfinal;
 }
 }  // namespace
 
@@ -1069,6 +1078,13 @@ int clast_hits_stage_times(clast_hits h, float* out8) {
   return guard([&] {
     if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
     std::memcpy(out8, h->times, sizeof h->times);
+  });
+}
+
+int clast_hits_counts(clast_hits h, int64_t* out4) {
+  return guard([&] {
+    if (!h || !out4) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out4, h->counts, sizeof h->counts);
   });
 }
 

@@ -1300,7 +1300,8 @@ __device__ __forceinline__ bool row_less(const RowKey& x, const RowKey& y) {
 // calculateEvalue (krnlCalculateEvalue.cu:18-21) in FP64, the E-value cut
 // as an integer score threshold per strand (score >= smin[s] <=> E <= cutOff,
 // threshold computed on the host with the same formula), the too-short cut
-// (…deleteBadHits.cuh:41-48), and the hit's order key.
+// (…deleteBadHits.cuh:41-48), the hit's order key, and the number of DP
+// cells the hits took.
 static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const int32_t* __restrict__ hs,
                                                               const int32_t* __restrict__ hgs,
                                                               const int32_t* __restrict__ ht,
@@ -1312,8 +1313,14 @@ static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const 
                                                               const int32_t* __restrict__ smin, int use_cut,
                                                               int min_qhl, double kd, double lambda,
                                                               double* __restrict__ ev, uint32_t* __restrict__ flag,
-                                                              RowKey* __restrict__ rk) {
+                                                              RowKey* __restrict__ rk, int lmer, int band,
+                                                              unsigned long long* __restrict__ cells) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  // DP cells of the hits (both directions: band x (qLen - lMer) rows), per wavefront
+  unsigned long long c = i < n ? static_cast<unsigned long long>(band) * (qlen[hs[i]] - lmer) : 0ull;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (lane_id() == 0 && c) atomicAdd(cells, c);
   if (i >= n) return;
   const int s = hs[i];
   const int sc = score[i];

@@ -90,6 +90,7 @@ def hip() -> C.CDLL:
             "clast_search_batch": (C.c_int, [pp, vp, vp, i32, C.POINTER(vp)]),
             "clast_hits_count": (i64, [vp]),
             "clast_hits_wait": (C.c_int, [vp]),
+            "clast_hits_counts": (C.c_int, [vp, vp]),
             "clast_hits_copy": (C.c_int, [vp] + [vp] * 9),
             "clast_hits_copy_windows": (C.c_int, [vp, vp, vp]),
             "clast_hits_stage_count": (i64, [vp, i32]),
@@ -350,6 +351,12 @@ class Hits:
             for k in HIT_FIELDS[4:]:
                 del out[k]
         return out
+
+    def counts(self) -> np.ndarray:
+        """raw seeds, hits entering the DP, DP cells, result rows."""
+        c = np.zeros(4, np.int64)
+        _check(hip().clast_hits_counts(self.h, _ptr(c)), "clast_hits_counts")
+        return c
 
     def times(self) -> np.ndarray:
         """seed count, seed write, seed sort, filters, DP, results, total (ms), seeds."""

@@ -158,6 +158,11 @@ int clast_hits_stage_copy(clast_hits h, int32_t stage, int32_t* tID, int32_t* tI
  * stream and is not included); out[7] is the number of raw seeds (index
  * entries matched, before the corner filter) as a float. */
 int clast_hits_stage_times(clast_hits h, float* out8);
+/* No counterpart in the reference (work accounting for rooflines): out[0]
+ * raw seeds (index entries matched, before the corner filter), [1] hits
+ * entering the DP, [2] DP cells (band width x rows, both directions), [3]
+ * result rows. */
+int clast_hits_counts(clast_hits h, int64_t* out4);
 int clast_hits_destroy(clast_hits h);
 
 /* resultSorting (utilResultSorting.cu:6-47): stable sort of n result rows by
