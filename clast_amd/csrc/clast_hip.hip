@@ -725,24 +725,25 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
     thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
     om = ws.get<int32_t>("d.m", nhit); osc = ws.get<int32_t>("d.s", nhit);
-    // buckets by (read length, seed qIdx): converged lanes in both DP loops
-    uint32_t* dporder = nullptr;
-    if (nhit > 1) {
-      uint64_t* dk = ws.get<uint64_t>("d.key", nhit);
-      dporder = ws.get<uint32_t>("d.order", nhit);
-      const int qb = bits_for(static_cast<uint64_t>(q.max_qlen) + 1);
-      k_dp_rows_key<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, q.max_qlen, qb, dk, dporder);
-      radix_sort_pairs(rt, dk, dporder, nhit,
-                       qb + bits_for(static_cast<uint64_t>(q.max_qlen - q.min_qlen) + 1));
-    }
-    DPArgs da{nhit, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, q.bmax, ix.bmax, p.lmer, ot, oq, thl, qhl,
-              om, osc};
+    // one task per (hit, direction), longest first
+    const uint32_t ntask = 2 * nhit;
+    uint64_t* dk = ws.get<uint64_t>("d.key", ntask);
+    uint32_t* dporder = ws.get<uint32_t>("d.order", ntask);
+    int4* dres = ws.get<int4>("d.res", ntask);
+    const int maxrows = std::max(q.max_qlen - p.lmer, 0);
+    k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, maxrows, dk,
+                                                               dporder);
+    radix_sort_pairs(rt, dk, dporder, ntask, bits_for(static_cast<uint64_t>(maxrows) + 1));
+    DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
+              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, dres, q.bmax, ix.bmax, p.lmer, ot, oq, thl,
+              qhl, om, osc};
     mark(EV_DP0);
     if (p.local) launch_dp_local(p.gap, da, st);
     else launch_dp_global(p.gap, da, st);
     mark(EV_DP1);
     rt.check_launch("k_dp");
+    k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);
+    rt.check_launch("k_dp_combine");
   }
   if (keep_stages) {
     std::vector<int32_t> c[6], hsv, hgv;

@@ -999,7 +999,8 @@ struct DPArgs {
   const int32_t* tlen;
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
-  const uint32_t* order;   // optional: hit processed by thread i (length buckets)
+  const uint32_t* order;   // task of thread i: (hit << 1) | direction, longest first
+  int4* dres;              // per task 2 * hit + direction: (move + end, move, match, score)
   int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
   int lmer;
   int32_t* o_tidx;
@@ -1163,22 +1164,27 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
   }
 }
 
-// One thread per hit, the band held in VGPRs: the right extension
-// ({global,local}AlignBackward) then the left one ({global,local}AlignForward).
-// The two directions are independent (the forward pass reads the seed's
-// tIdx/qIdx, which the backward pass does not change), so one thread runs
-// both and sums the results.  Hits arrive ordered by (read length, seed
-// qIdx), so the lanes of a wavefront run near-equal row counts in both
-// loops.  Query and target bytes arrive through 16-byte windows (ByteWin).
+// One thread per (hit, direction), the band held in VGPRs: the right
+// extension ({global,local}AlignBackward, direction 0) or the left one
+// ({global,local}AlignForward, direction 1).  The two are independent (the
+// forward pass reads the seed's tIdx/qIdx, which the backward pass does not
+// change), so they run as separate tasks, longest first: a long read's two
+// extensions proceed side by side and every wavefront's lanes run near-equal
+// row counts.  Query and target bytes arrive through 16-byte windows
+// (ByteWin); k_dp_combine adds the two directions up.
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= a.n) return;
-  const uint32_t i = a.order ? a.order[t] : t;
+  const uint32_t task = a.order[t];
+  const uint32_t i = task >> 1;
+  const bool bwd = (task & 1u) == 0u;
   const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
   const int L = a.lmer;
   const int qLen = a.qlen[s];
+  const int rows = bwd ? qLen - (qIdx + L) : qIdx;
+  if (rows == 0) return;
   const int tLen = a.tlen[gs];
   const uint64_t tslot = a.tslot[gs];
   const int w = a.twin[gs];
@@ -1186,73 +1192,71 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - static_cast<int64_t>(tslot));
   int S[W + 2], V[W + 2], M[W + 2];
   uint32_t T[W];
-  int oT = tIdx, oQ = qIdx, tHL = L, qHL = L, mm = L, sc = L * kMatch;
   const int64_t qg = static_cast<int64_t>(a.qslot[s]);
-
+  dp_init<G>(S, V, M);
+  // target index tb + ts*j feeds ring slot j (j < W) and then row j - W
+  const int ts = bwd ? 1 : -1;
+  const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+  const int lo = bwd ? wlo : max(wlo, 0);
+  const int hi = bwd ? min(whi, tLen) : whi;
+  ByteWin tst, qst;
+  tst.init(a.tbase, a.t_bmax, static_cast<int64_t>(tslot) + tb, ts);
+  qst.init(a.qbase, a.q_bmax, qg + (bwd ? qIdx + L : qIdx - 1), ts);
+#pragma unroll
+  for (int k = 0; k < W; ++k) T[k] = t_mask(tst.next(k), tb + ts * k, lo, hi);
+  int bestS = 0, bestM = 0, bestR = -1, bestB = G;
+  int tn_i = tb + ts * W;
 #pragma unroll 1
-  for (int dir = 0; dir < 2; ++dir) {
-    const bool bwd = dir == 0;
-    const int rows = bwd ? qLen - (qIdx + L) : qIdx;
-    if (rows == 0) continue;
-    dp_init<G>(S, V, M);
-    // target index tb + ts*j feeds ring slot j (j < W) and then row j - W
-    const int ts = bwd ? 1 : -1;
-    const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
-    const int lo = bwd ? wlo : max(wlo, 0);
-    const int hi = bwd ? min(whi, tLen) : whi;
-    ByteWin tst, qst;
-    tst.init(a.tbase, a.t_bmax, static_cast<int64_t>(tslot) + tb, ts);
-    qst.init(a.qbase, a.q_bmax, qg + (bwd ? qIdx + L : qIdx - 1), ts);
+  for (int r = 0; r < rows; ++r) {
+    const int ru = __builtin_amdgcn_readfirstlane(r);  // every active lane is at row r
+    uint32_t wpm, wmc;
+    row_words(qst.next(ru), wpm, wmc);
+    const uint32_t tn = t_mask(tst.next(ru + W), tn_i, lo, hi);
+    tn_i += ts;
+    dp_row<W, LOCAL>(S, V, M, T, wpm, wmc, r, bestS, bestM, bestR, bestB);
 #pragma unroll
-    for (int k = 0; k < W; ++k) T[k] = t_mask(tst.next(k), tb + ts * k, lo, hi);
-    int bestS = 0, bestM = 0, bestR = -1, bestB = G;
-    int tn_i = tb + ts * W;
-#pragma unroll 1
-    for (int r = 0; r < rows; ++r) {
-      const int ru = __builtin_amdgcn_readfirstlane(r);  // every active lane is at row r
-      uint32_t wpm, wmc;
-      row_words(qst.next(ru), wpm, wmc);
-      const uint32_t tn = t_mask(tst.next(ru + W), tn_i, lo, hi);
-      tn_i += ts;
-      dp_row<W, LOCAL>(S, V, M, T, wpm, wmc, r, bestS, bestM, bestR, bestB);
-#pragma unroll
-      for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
-      T[W - 1] = tn;
-    }
-    int move, endp;
-    if (LOCAL) {
-      move = bestR + 1;
-      endp = bestB - G;
-    } else {
-      // select end node: strict '<' from the constant band 0 (:170-186)
-      bestS = S[0];
-      bestM = M[0];
-      int e = -1;
-#pragma unroll
-      for (int b = 1; b <= W; ++b)
-        if (bestS < S[b]) {
-          bestS = S[b];
-          bestM = M[b];
-          e = b - 1;
-        }
-      move = rows;
-      endp = e - G;
-    }
-    if (!bwd) {
-      oT -= move + endp;
-      oQ -= move;
-    }
-    tHL += move + endp;
-    qHL += move;
-    mm += bestM;
-    sc += bestS;
+    for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
+    T[W - 1] = tn;
   }
-  a.o_tidx[i] = oT;
-  a.o_qidx[i] = oQ;
-  a.o_thl[i] = tHL;
-  a.o_qhl[i] = qHL;
-  a.o_match[i] = mm;
-  a.o_score[i] = sc;
+  int move, endp;
+  if (LOCAL) {
+    move = bestR + 1;
+    endp = bestB - G;
+  } else {
+    // select end node: strict '<' from the constant band 0 (:170-186)
+    bestS = S[0];
+    bestM = M[0];
+    int e = -1;
+#pragma unroll
+    for (int b = 1; b <= W; ++b)
+      if (bestS < S[b]) {
+        bestS = S[b];
+        bestM = M[b];
+        e = b - 1;
+      }
+    move = rows;
+    endp = e - G;
+  }
+  a.dres[task] = make_int4(move + endp, move, bestM, bestS);
+}
+
+// Both directions of each hit (…alignmentHits.cu:122-348): the backward
+// extension lengthens tHL/qHL; the forward one also moves tIdx/qIdx back.
+static __global__ __launch_bounds__(kBlock) void k_dp_combine(DPArgs a, uint32_t nhit) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nhit) return;
+  const int s = a.hs[i], tIdx = a.ht[i], qIdx = a.hq[i];
+  const int L = a.lmer;
+  const int qLen = a.qlen[s];
+  const int4 z = make_int4(0, 0, 0, 0);
+  const int4 b = qLen - (qIdx + L) > 0 ? a.dres[2 * i] : z;
+  const int4 f = qIdx > 0 ? a.dres[2 * i + 1] : z;
+  a.o_tidx[i] = tIdx - f.x;
+  a.o_qidx[i] = qIdx - f.y;
+  a.o_thl[i] = L + b.x + f.x;
+  a.o_qhl[i] = L + b.y + f.y;
+  a.o_match[i] = L + b.z + f.z;
+  a.o_score[i] = L * kMatch + b.w + f.w;
 }
 
 // Launchers of k_dp for a runtime gap (2..16); each lives in its own
@@ -1260,18 +1264,20 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
 
-// DP bucket key of a hit: (maxq - read length, seed qIdx).  Both directions
-// together always run qLen - lMer rows; sorting by length (longest first)
-// and then by qIdx gives the lanes of a wavefront near-equal backward and
-// forward row counts.
-static __global__ __launch_bounds__(kBlock) void k_dp_rows_key(uint32_t n, const int32_t* __restrict__ hs,
-                                                               const int32_t* __restrict__ hq,
-                                                               const int32_t* __restrict__ qlen, int maxq, int qbits,
-                                                               uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  key[i] = (static_cast<uint64_t>(maxq - qlen[hs[i]]) << qbits) | static_cast<uint64_t>(hq[i]);
-  idx[i] = i;
+// DP task order: rows descending (key = maxrows - rows), so the longest
+// extensions start first and a wavefront's lanes run near-equal row counts.
+static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(uint32_t nhit, const int32_t* __restrict__ hs,
+                                                                const int32_t* __restrict__ hq,
+                                                                const int32_t* __restrict__ qlen, int lmer,
+                                                                int maxrows, uint64_t* __restrict__ key,
+                                                                uint32_t* __restrict__ task) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= 2 * nhit) return;
+  const uint32_t i = t >> 1;
+  const int q = hq[i];
+  const int rows = (t & 1u) ? q : qlen[hs[i]] - (q + lmer);
+  key[t] = static_cast<uint64_t>(maxrows - rows);
+  task[t] = t;
 }
 
 // ---------------------------------------------------------------- results
