@@ -507,8 +507,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 2);
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 4);
-  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 4, 0ull);
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 12);  // [2, 10): DP cell counters
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 12, 0ull);
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
@@ -900,8 +900,10 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
+  uint64_t cell8[8];
+  rt.read_scalars(tot64 + 2, cell8, 8);
   uint64_t cells = 0;
-  rt.read_scalars(tot64 + 2, &cells, 1);
+  for (uint64_t c : cell8) cells += c;
   out.counts[0] = nseed;
   out.counts[1] = nhit;
   out.counts[2] = static_cast<int64_t>(cells);

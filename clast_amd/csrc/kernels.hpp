@@ -1307,7 +1307,7 @@ __device__ __forceinline__ bool row_less(const RowKey& x, const RowKey& y) {
 // as an integer score threshold per strand (score >= smin[s] <=> E <= cutOff,
 // threshold computed on the host with the same formula), the too-short cut
 // (…deleteBadHits.cuh:41-48), the hit's order key, and the number of DP
-// cells the hits took.
+// cells the hits took (summed over cells[0..8)).
 static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const int32_t* __restrict__ hs,
                                                               const int32_t* __restrict__ hgs,
                                                               const int32_t* __restrict__ ht,
@@ -1321,12 +1321,20 @@ static __global__ __launch_bounds__(kBlock) void k_hit_filter(uint32_t n, const 
                                                               double* __restrict__ ev, uint32_t* __restrict__ flag,
                                                               RowKey* __restrict__ rk, int lmer, int band,
                                                               unsigned long long* __restrict__ cells) {
+  __shared__ unsigned long long wsum[kBlock / 64];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  // DP cells of the hits (both directions: band x (qLen - lMer) rows), per wavefront
+  // DP cells of the hits (both directions: band x (qLen - lMer) rows): one
+  // atomic per workgroup, spread over 8 counters
   unsigned long long c = i < n ? static_cast<unsigned long long>(band) * (qlen[hs[i]] - lmer) : 0ull;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (lane_id() == 0 && c) atomicAdd(cells, c);
+  if (lane_id() == 0) wsum[wave_id()] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kBlock / 64; ++k) t += wsum[k];
+    if (t) atomicAdd(&cells[blockIdx.x & 7u], t);
+  }
   if (i >= n) return;
   const int s = hs[i];
   const int sc = score[i];
