@@ -280,7 +280,8 @@ def main() -> int:
             h.close()
     lat = np.array(lat)
     times = np.array([x[0] for x in rec])  # per search: lookup, segments, sort, filt, dp, res, total, raw seeds
-    counts = np.array([x[1] for x in rec], dtype=np.float64)  # raw seeds, DP hits, DP cells, rows
+    # raw seeds, DP hits, DP cells, rows, 16-bit DP tasks, their cells
+    counts = np.array([x[1] for x in rec], dtype=np.float64)
     # k_seed_count algorithmic bytes per launch: 16 B per query k-mer position
     # (8 B index key lookup + 8 B cell (offset, count)); positions per launch
     # are taken from the query chunks
@@ -295,8 +296,8 @@ def main() -> int:
     bytes_count = 16.0 * pos_per_launch
     achieved = float(bytes_count.sum() / (count_ms.sum() / 1e3) / 1e9)
     traffic = measured_traffic("k_seed_count") if args.workload == "large" and args.scale == 1.0 else None
-    # per-stage work against MI355X peaks (algorithmic bytes / ops, SURVEY-style
-    # conventions): lookup 16 B per query k-mer; segments 24 B per raw seed
+    # per-stage work against MI355X peaks (algorithmic bytes / ops): lookup
+    # 16 B per query k-mer; segments 24 B per raw seed
     # (expansion) + 32 B (sort) + 16 B (isolate/duplicate pass) + 16 B per hit
     # out; DP 14 int32 ops per cell; results 40 B per hit in + 48 B per row out
     sec = lambda col: float(times[:, col].sum()) / 1e3  # noqa: E731
@@ -309,7 +310,8 @@ def main() -> int:
                           "raw_seeds_per_search": int(seeds_raw / len(counts))},
         "dp": {"GCUPS": round(cells / sec(4) / 1e9, 1), "Tops/s": round(14.0 * cells / sec(4) / 1e12, 2),
                "peak_Tops/s": VALU_INT32_PEAK_TOPS, "hits_per_search": int(dp_hits / len(counts)),
-               "cells_per_search": int(cells / len(counts))},
+               "cells_per_search": int(cells / len(counts)),
+               "cells_16bit_frac": round(float(counts[:, 5].sum()) / max(cells, 1.0), 4)},
         "results": {"GB/s": round(res_bytes / sec(5) / 1e9, 1), "peak": HBM_PEAK_GBS},
     }
     stage_names = ["seed_lookup", "seed_segments", "global_seed_sort", "global_seed_filters", "dp", "results",

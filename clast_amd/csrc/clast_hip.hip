@@ -117,6 +117,8 @@ struct clast_index_s {
   DevArray<uint64_t> seq_slot;
   DevArray<int32_t> seq_len, seq_win, seq_ref;
   DevArray<int2> seq_info;  // (length, window) per target sequence
+  DevArray<uint8_t> seq_clean;  // no star / X class in the sequence (16-bit DP)
+  DevArray<uint32_t> tnib;      // 4-bit copy of tbase (16-bit DP)
   uint32_t ncell = 0;
   DevArray<uint32_t> ekey32;
   DevArray<uint64_t> ekey64;
@@ -138,6 +140,8 @@ struct clast_query_s {
   DevArray<uint8_t> qbase;
   DevArray<uint32_t> qslot, pos_off, pos_strand;
   DevArray<int32_t> qlen, qref, qwin, smin;
+  DevArray<uint8_t> qclean;  // no star / X class in the strand (16-bit DP)
+  DevArray<uint32_t> qnib;   // 4-bit copy of qbase, other -> 7 (16-bit DP)
   std::vector<int32_t> h_qlen, h_qref, h_qwin, h_id_begin;
   bool smin_valid = false;
   double smin_key[4] = {0, 0, 0, 0};
@@ -170,7 +174,7 @@ struct clast_hits_s {
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int64_t counts[4] = {0, 0, 0, 0};  // raw seeds, DP hits, DP cells, rows
+  int64_t counts[6] = {0, 0, 0, 0, 0, 0};  // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells
 };
 
 namespace {
@@ -209,6 +213,14 @@ static bool use_direct(int B, uint32_t nent) {
     if (std::strcmp(f, "auto")) throw ClastError(CLAST_ERR_ARG, "CLAST_INDEX_FORM must be direct, bitmap or auto");
   }
   return prefixes * sizeof(uint4) <= (16ull << 30) && static_cast<uint64_t>(nent) * 16 >= prefixes;
+}
+
+// CLAST_DP16=0 sends every DP task to the int32 kernel (k_dp); by default
+// the tasks that fit run in packed 16-bit lanes (k_dp16).  Both give the
+// same results; the switch exists to compare them.
+static bool dp16_disabled() {
+  const char* f = std::getenv("CLAST_DP16");
+  return f && !std::strcmp(f, "0");
 }
 
 // Cell lookup over the top B key bits of the sorted entries (keys in ek,
@@ -324,6 +336,16 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
     ix.seq_info.upload(info.data(), info.size(), st);
   }
   ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
+  {
+    const uint64_t ndw = 2 * (static_cast<uint64_t>(ix.bmax) + 1);
+    ix.tnib.alloc(ndw);
+    k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(ix.tbase.ptr, ndw, 0, ix.tnib.ptr);
+  }
+  ix.seq_clean.alloc(std::max(ix.nseq, 1));
+  if (ix.nseq)
+    k_clean_flags<uint64_t><<<grid_for(ix.nseq, kBlock / 64), kBlock, 0, st>>>(ix.tbase.ptr, ix.seq_slot.ptr,
+                                                                               ix.seq_len.ptr, ix.nseq,
+                                                                               ix.seq_clean.ptr);
   DevArray<uint32_t> d_win_stride, d_win_seq;
   d_win_stride.upload(win_stride.data(), win_stride.size(), st);
   d_win_seq.upload(win_seq.data(), win_seq.size(), st);
@@ -422,6 +444,15 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
   q.qlen.upload(q.h_qlen.data(), q.h_qlen.size(), st);
   q.qref.upload(q.h_qref.data(), q.h_qref.size(), st);
   q.qwin.upload(q.h_qwin.data(), q.h_qwin.size(), st);
+  {
+    const uint64_t ndw = 2 * (static_cast<uint64_t>(q.bmax) + 1);
+    q.qnib.alloc(ndw);
+    k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(q.qbase.ptr, ndw, 1, q.qnib.ptr);
+  }
+  q.qclean.alloc(std::max<uint64_t>(q.nseq, 1));
+  if (q.nseq)
+    k_clean_flags<uint32_t><<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.qbase.ptr, q.qslot.ptr, q.qlen.ptr,
+                                                                              q.nseq, q.qclean.ptr);
   q.smin.alloc(std::max<uint64_t>(q.nseq, 1));
   rt.sync();
 }
@@ -507,8 +538,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 2);
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 12);  // [2, 10): DP cell counters
-  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 12, 0ull);
+  // [2, 10): DP cell counters, [10, 18): rows of 16-bit DP tasks, [18]: their count
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 20);
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 20, 0ull);
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
@@ -731,15 +763,29 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* dporder = ws.get<uint32_t>("d.order", ntask);
     int4* dres = ws.get<int4>("d.res", ntask);
     const int maxrows = std::max(q.max_qlen - p.lmer, 0);
-    k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, maxrows, dk,
-                                                               dporder);
-    radix_sort_pairs(rt, dk, dporder, ntask, bits_for(static_cast<uint64_t>(maxrows) + 1));
+    const int rowbits = bits_for(static_cast<uint64_t>(maxrows) + 1);
+    // class bit on top of the rows, within the radix passes the rows need
+    const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
+    const int rowshift = std::max(0, rowbits - (sortbits - 1));
+    uint32_t* n16 = ws.get<uint32_t>("d.n16", 1);
+    const bool packed = !p.local && !dp16_disabled();
+    const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
+                         ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
+                         maxrows, rowshift, sortbits - 1, dk, dporder,
+                         reinterpret_cast<unsigned long long*>(tot64 + 10)};
+    k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
+    radix_sort_pairs(rt, dk, dporder, ntask, sortbits);
+    k_count_below<<<1, 64, 0, st>>>(dk, ntask, sortbits - 1, n16, reinterpret_cast<unsigned long long*>(tot64 + 18));
     DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, dres, q.bmax, ix.bmax, p.lmer, ot, oq, thl,
-              qhl, om, osc};
+              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, n16, q.qnib.ptr, ix.tnib.ptr, dres, q.bmax,
+              ix.bmax, p.lmer, ot, oq, thl, qhl, om, osc};
     mark(EV_DP0);
-    if (p.local) launch_dp_local(p.gap, da, st);
-    else launch_dp_global(p.gap, da, st);
+    if (p.local) {
+      launch_dp_local(p.gap, da, st);
+    } else {
+      if (packed) launch_dp16(p.gap, da, st);
+      launch_dp_global(p.gap, da, st);
+    }
     mark(EV_DP1);
     rt.check_launch("k_dp");
     k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);
@@ -900,14 +946,19 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
-  uint64_t cell8[8];
-  rt.read_scalars(tot64 + 2, cell8, 8);
-  uint64_t cells = 0;
-  for (uint64_t c : cell8) cells += c;
+  uint64_t cnt[17];
+  rt.read_scalars(tot64 + 2, cnt, 17);
+  uint64_t cells = 0, rows16 = 0;
+  for (int k = 0; k < 8; ++k) {
+    cells += cnt[k];
+    rows16 += cnt[8 + k];
+  }
   out.counts[0] = nseed;
   out.counts[1] = nhit;
   out.counts[2] = static_cast<int64_t>(cells);
   out.counts[3] = nfinal;
+  out.counts[4] = static_cast<int64_t>(cnt[16]);
+  out.counts[5] = static_cast<int64_t>(rows16) * (2 * p.gap + 1);
 }
 }  // namespace
 
@@ -1084,10 +1135,10 @@ int clast_hits_stage_times(clast_hits h, float* out8) {
   });
 }
 
-int clast_hits_counts(clast_hits h, int64_t* out4) {
+int clast_hits_counts(clast_hits h, int64_t* out6) {
   return guard([&] {
-    if (!h || !out4) throw ClastError(CLAST_ERR_ARG, "null argument");
-    std::memcpy(out4, h->counts, sizeof h->counts);
+    if (!h || !out6) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out6, h->counts, sizeof h->counts);
   });
 }
 

@@ -52,6 +52,46 @@ static __global__ __launch_bounds__(kBlock) void k_classify(uint8_t* __restrict_
   if (i < n) p[i] = static_cast<uint8_t>(dp_class(p[i]));
 }
 
+// clean[k] = 1 when the bytes [slot[k], slot[k] + len[k]) of a class array
+// hold no star or X class (scores and match counts of such a sequence's
+// cells stay within 16 bits; see k_dp16).  One wavefront per sequence, four
+// aligned 16-byte blocks per lane and step; bytes outside the range are
+// masked off.
+template <class SlotT>
+static __global__ __launch_bounds__(kBlock) void k_clean_flags(const uint8_t* __restrict__ base,
+                                                               const SlotT* __restrict__ slot,
+                                                               const int32_t* __restrict__ len, uint32_t n,
+                                                               uint8_t* __restrict__ clean) {
+  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (k >= n) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t a = static_cast<uint64_t>(slot[k]);
+  const uint64_t e = a + static_cast<uint64_t>(max(len[k], 0));
+  const uint4* blk = reinterpret_cast<const uint4*>(base);
+  uint32_t bad = 0;
+  for (uint64_t b0 = (a >> 4) + lane; b0 * 16 < e; b0 += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t b = b0 + 64u * u;
+      if (b * 16 >= e) break;
+      const uint4 v = blk[b];
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        // bytes of classes 5 and 6 (the only ones above 4) get bit 3 set by +3
+        uint32_t hit = (w[d] + 0x03030303u) & 0x08080808u;
+        const uint64_t p = b * 16 + 4u * d;  // first byte of the dword
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          if (p + y < a || p + y >= e) hit &= ~(0xFFu << (8 * y));
+        bad |= hit;
+      }
+    }
+  }
+  const bool any = __builtin_amdgcn_ballot_w64(bad != 0u) != 0ull;
+  if (lane == 0) clean[k] = any ? 0 : 1;
+}
+
 template <class T>
 __device__ __forceinline__ uint32_t upper_bound_u(const T* a, uint32_t lo, uint32_t hi, T v) {
   // first index in [lo, hi) with a[i] > v
@@ -1000,6 +1040,9 @@ struct DPArgs {
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
   const uint32_t* order;   // task of thread i: (hit << 1) | direction, longest first
+  const uint32_t* n16;     // order[0, *n16) are the 16-bit tasks (k_dp16), the rest go to k_dp
+  const uint32_t* qnib;    // 4-bit copies of qbase / tbase (k_pack_nibbles), read by k_dp16
+  const uint32_t* tnib;
   int4* dres;              // per task 2 * hit + direction: (move + end, move, match, score)
   int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
   int lmer;
@@ -1176,8 +1219,9 @@ template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= a.n) return;
-  const uint32_t task = a.order[t];
+  const uint32_t first = *a.n16;
+  if (t >= a.n - first) return;
+  const uint32_t task = a.order[first + t];
   const uint32_t i = task >> 1;
   const bool bwd = (task & 1u) == 0u;
   const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
@@ -1259,25 +1303,394 @@ static __global__ __launch_bounds__(kBlock) void k_dp_combine(DPArgs a, uint32_t
   a.o_score[i] = L * kMatch + b.w + f.w;
 }
 
+// ------------------------------------------- banded DP in packed 16-bit lanes
+//
+// Global mode, two tasks per thread, one in each 16-bit half of every band
+// register.  A task qualifies (k_dp_task_keys) when its query is at most
+// kPacked16MaxLen bases, neither its query strand nor its target sequence
+// holds a star or X class, and every target index its band touches lies in
+// the target sequence and in the window.  Then every substitution is +1 / -3
+// with match increment 1 / 0, and the int32 recurrence of k_dp stays inside
+// 16 bits: a score is at most the row count and at least band 1's
+// horizontal margin -30002 less 2 per band (band b's horizontal term is
+// >= -30002 - 2 (b - 1)), so scores lie in [-30075, 30001]; a match count
+// starts at 0 or the margin -30000 and grows by at most one per row.  The
+// results equal k_dp's bit for bit.
+
+constexpr int kPacked16MaxLen = 30000;
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_bits(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s16x2 pk_val(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 pk_splat(int v) { return s16x2{static_cast<short>(v), static_cast<short>(v)}; }
+// all-ones in the halves where a < b (saturating difference, so no wrap)
+__device__ __forceinline__ uint32_t pk_less(s16x2 a, s16x2 b) {
+  return pk_bits(__builtin_elementwise_sub_sat(a, b) >> 15);
+}
+// a in the halves selected by mask m, b elsewhere
+__device__ __forceinline__ uint32_t pk_pick(uint32_t m, uint32_t a, uint32_t b) { return b ^ ((a ^ b) & m); }
+
+// 4-bit class arrays read by k_dp16: nibble k of dword d is slot 8 d + k
+// of the byte class array (padded to whole 16-byte blocks, so whole 8-byte
+// units here).  The query copy maps class 4 (other) to 7, which matches no
+// target class (classes 0..4 in the bands k_dp16 runs).
+static __global__ __launch_bounds__(kBlock) void k_pack_nibbles(const uint8_t* __restrict__ bytes, uint64_t ndw,
+                                                                int query, uint32_t* __restrict__ nib) {
+  const uint64_t d = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (d >= ndw) return;
+  const uint2 v = reinterpret_cast<const uint2*>(bytes)[d];
+  uint32_t lo = v.x, hi = v.y;
+  if (query) {  // byte table: 0 1 2 3 4 5 6 7 -> 0 1 2 3 7 5 6 7
+    lo = __builtin_amdgcn_perm(0x07060507u, 0x03020100u, lo);
+    hi = __builtin_amdgcn_perm(0x07060507u, 0x03020100u, hi);
+  }
+  lo &= 0x0F0F0F0Fu;
+  hi &= 0x0F0F0F0Fu;
+  // byte k's nibble to bits 4k: pairs of bytes first, then the pairs
+  lo |= lo >> 4;
+  hi |= hi >> 4;
+  nib[d] = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+}
+
+// Nibbles of a dword in reverse order.
+__device__ __forceinline__ uint32_t nibble_reverse(uint32_t x) {
+  x = __builtin_amdgcn_perm(0u, x, 0x00010203u);
+  return ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
+}
+
+// Sequential reader of one half's slots g, g + step, g + 2 step, ... of a
+// 4-bit class array in groups of 16, on a schedule shared by the wavefront
+// (every lane is at the same group): a group's window spans two aligned
+// 16-slot units held in r0 / r1; after group k the one of them that no later
+// group reads (r[k & 1]) is reloaded with the unit after the other, so each
+// load is requested 16 rows before its use and lands in its own register.
+// Units outside [0, umax] are clamped; their slots are never used.
+struct NibWin {
+  int32_t u;       // unit requested next
+  int32_t step;    // +1 / -1 (memory direction)
+  uint32_t s4;     // bit offset of the window inside its first unit
+  uint2 r0, r1;
+
+  __device__ __forceinline__ static uint2 load(const uint2* base, int32_t i, int32_t umax) {
+    return base[min(max(i, 0), umax)];
+  }
+  __device__ __forceinline__ void init(const uint2* base, int32_t umax, int64_t g0, int st) {
+    step = st;
+    const int64_t lo = st > 0 ? g0 : g0 - 15;  // first group's lowest slot
+    const int32_t u0 = static_cast<int32_t>(lo >> 4);
+    s4 = static_cast<uint32_t>(lo & 15) * 4u;
+    // group 0: forward reloads its low unit (r0) next, backward its high one
+    r0 = load(base, st > 0 ? u0 : u0 + 1, umax);
+    r1 = load(base, st > 0 ? u0 + 1 : u0, umax);
+    u = st > 0 ? u0 + 2 : u0 - 1;
+  }
+  // Group k, k & 1 == ODD: its 16 classes in stream order, rows 0-7 in w0
+  // and 8-15 in w1 (row j in bits 4 (j & 7)).
+  template <int ODD>
+  __device__ __forceinline__ void group(const uint2* base, int32_t umax, uint32_t& w0, uint32_t& w1) {
+    const uint2 a = ODD ? r1 : r0;  // reloaded below
+    const uint2 b = ODD ? r0 : r1;
+    const bool fwd = step > 0;
+    const uint2 lo = fwd ? a : b, hi = fwd ? b : a;
+    // bits [s4, s4 + 64) of hi:lo
+    const bool big = s4 >= 32u;
+    const uint32_t x0 = big ? lo.y : lo.x, x1 = big ? hi.x : lo.y, x2 = big ? hi.y : hi.x;
+    const uint32_t sh = s4 & 31u;
+    const uint32_t m0 = __builtin_amdgcn_alignbit(x1, x0, sh), m1 = __builtin_amdgcn_alignbit(x2, x1, sh);
+    w0 = fwd ? m0 : nibble_reverse(m1);
+    w1 = fwd ? m1 : nibble_reverse(m0);
+    const uint2 v = load(base, u, umax);
+    u += step;
+    if (ODD) r1 = v;
+    else r0 = v;
+  }
+};
+
+// Row codes of the two halves for 16 rows: word k holds rows 4k .. 4k + 3,
+// the low half's classes in bits 0-15 and the high half's in bits 16-31
+// (row 4k + i in bits 4 i of each half).
+struct PairRows {
+  uint32_t p0, p1, p2, p3;
+  __device__ __forceinline__ void build(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
+    p0 = __builtin_amdgcn_perm(b0, a0, 0x05040100u);
+    p1 = __builtin_amdgcn_perm(b0, a0, 0x07060302u);
+    p2 = __builtin_amdgcn_perm(b1, a1, 0x05040100u);
+    p3 = __builtin_amdgcn_perm(b1, a1, 0x07060302u);
+  }
+  // codes of the next row (both halves); j is the row inside the group,
+  // the same in every lane
+  __device__ __forceinline__ uint32_t next(int j) {
+    const uint32_t v = p0 & 0x000F000Fu;
+    p0 >>= 4;
+    if ((j & 3) == 3) {
+      p0 = p1;
+      p1 = p2;
+      p2 = p3;
+    }
+    return v;
+  }
+};
+
+// One query row (k_dp's dp_row, global mode) for two tasks.  Z holds S - 7:
+// both gap terms then read it unchanged, and the diagonal term is
+// S + (4 mc - 3) = Z + 4 + 4 mc.  T[b-1] and Q hold the target and query
+// codes of both halves; mc = 1 - min(t ^ q, 1).
+template <int W>
+__device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
+                                         const uint32_t (&T)[W], uint32_t Q) {
+  s16x2 hp = pk_splat(-30000);
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const s16x2 v = __builtin_elementwise_max(Z[b + 1], V[b + 1] + pk_splat(kGapExt));
+    const s16x2 h = __builtin_elementwise_max(Z[b - 1], hp + pk_splat(kGapExt));
+    const u16x2 x = __builtin_bit_cast(u16x2, T[b - 1] ^ Q);
+    const s16x2 mc = __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(u16x2{1, 1}, x));
+    const s16x2 pt = Z[b] + (mc << 2) + pk_splat(kMatch - kMismatch);
+    const s16x2 vh = __builtin_elementwise_max(v, h);
+    const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
+    const uint32_t nm = pk_pick(pk_less(vh, pt), pk_bits(M[b] + mc), ngap);
+    Z[b] = __builtin_elementwise_max(pt, vh) + pk_splat(kGapOpen + kGapExt);
+    V[b] = v;
+    M[b] = pk_val(nm);
+    hp = h;
+  }
+}
+
+// End node of the last row (k_dp's global selection: strict '<' from the
+// constant band 0) of both halves; half h goes to dres[task] when store.
+template <int G>
+__device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x2 (&M)[2 * G + 3], int4* dres,
+                                         uint32_t task, int half, int rows) {
+  constexpr int W = 2 * G + 1;
+  s16x2 bz = Z[0];
+  uint32_t bm = pk_bits(M[0]), be = pk_bits(pk_splat(-1));
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const uint32_t m = pk_less(bz, Z[b]);
+    bz = __builtin_elementwise_max(bz, Z[b]);
+    bm = pk_pick(m, pk_bits(M[b]), bm);
+    be = pk_pick(m, pk_bits(pk_splat(b - 1)), be);
+  }
+  const int sc = (half ? bz.y : bz.x) - (kGapOpen + kGapExt);
+  const int mt = half ? pk_val(bm).y : pk_val(bm).x;
+  const int e = (half ? pk_val(be).y : pk_val(be).x) - G;
+  dres[task] = make_int4(rows + e, rows, mt, sc);
+}
+
+// Tasks order[2t] and order[2t + 1] (neighbours in the longest-first order,
+// so near-equal rows) in the low and high halves; a lone last task is run
+// in both.  Each half's end node is taken at its own last row.  Target and
+// query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
+template <int G>
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
+  constexpr int W = 2 * G + 1;
+  // the target streams start kLead positions early, so that both streams
+  // enter groups on rows r = 0 (mod 16) with the same group parity
+  constexpr int kLead = (32 - (W & 31)) & 31;
+  constexpr int kInitGroups = (W + kLead) / 16;  // even
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = *a.n16;
+  if (2 * t >= n) return;
+  const uint32_t task0 = a.order[2 * t];
+  const uint32_t task1 = 2 * t + 1 < n ? a.order[2 * t + 1] : task0;
+  const int L = a.lmer;
+  const uint2* tn2 = reinterpret_cast<const uint2*>(a.tnib);
+  const uint2* qn2 = reinterpret_cast<const uint2*>(a.qnib);
+  const int32_t tumax = a.t_bmax, qumax = a.q_bmax;  // one 16-slot unit per 16-byte block
+  NibWin tw[2], qw[2];
+  int rows[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t task = h ? task1 : task0;
+    const uint32_t i = task >> 1;
+    const bool bwd = (task & 1u) == 0u;
+    const int s = a.hs[i], tIdx = a.ht[i], qIdx = a.hq[i];
+    rows[h] = bwd ? a.qlen[s] - (qIdx + L) : qIdx;
+    const int ts = bwd ? 1 : -1;
+    const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+    tw[h].init(tn2, tumax, static_cast<int64_t>(a.tslot[a.hgs[i]]) + tb - ts * kLead, ts);
+    qw[h].init(qn2, qumax, static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), ts);
+  }
+  s16x2 Z[W + 2], V[W + 2], M[W + 2];
+  constexpr int half = G + 1;
+#pragma unroll
+  for (int b = 0; b < W + 2; ++b) {
+    int sc, vv, mm;
+    if (b < half) {
+      sc = vv = mm = -30000;
+    } else if (b == half) {
+      sc = 0; vv = kGapOpen; mm = 0;
+    } else {
+      sc = (b - half) * kGapExt + kGapOpen; vv = sc + kGapOpen; mm = 0;
+    }
+    Z[b] = pk_splat(sc + kGapOpen + kGapExt);
+    V[b] = pk_splat(vv);
+    M[b] = pk_splat(mm);
+  }
+  // target ring: stream positions kLead .. kLead + W - 1
+  PairRows tp, qp;
+  uint32_t T[W];
+#pragma unroll
+  for (int g = 0; g < kInitGroups; ++g) {
+    uint32_t a0, a1, b0, b1;
+    if (g & 1) {
+      tw[0].group<1>(tn2, tumax, a0, a1);
+      tw[1].group<1>(tn2, tumax, b0, b1);
+    } else {
+      tw[0].group<0>(tn2, tumax, a0, a1);
+      tw[1].group<0>(tn2, tumax, b0, b1);
+    }
+    tp.build(a0, a1, b0, b1);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t v = tp.next(j);
+      const int k = 16 * g + j - kLead;
+      if (k >= 0 && k < W) T[k] = v;
+    }
+  }
+  const int rmax = max(rows[0], rows[1]);
+  const int rsplit = rows[0] != rows[1] ? min(rows[0], rows[1]) : -1;
+  // 32 rows per iteration: even group, 16 rows, odd group, 16 rows (each
+  // stream register then has one definition in the loop, so the loads land
+  // in place); every active lane is at the same row
+  auto rows16 = [&](int r0) {
+#pragma unroll 1
+    for (int j = 0; j < 16; ++j) {
+      const int ju = __builtin_amdgcn_readfirstlane(j);
+      const int r = r0 + ju;
+      if (r >= rmax) break;
+      const uint32_t Q = qp.next(ju);
+      const uint32_t tn = tp.next(ju);  // target position r + W + kLead = r (mod 16)
+      dp_row16<W>(Z, V, M, T, Q);
+#pragma unroll
+      for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
+      T[W - 1] = tn;
+      if (r + 1 == rsplit) {
+        const int h = rows[0] == rsplit ? 0 : 1;
+        dp16_end<G>(Z, M, a.dres, h ? task1 : task0, h, rsplit);
+      }
+    }
+  };
+#pragma unroll 1
+  for (int rg = 0; rg < rmax; rg += 32) {
+    {
+      uint32_t a0, a1, b0, b1, c0, c1, d0, d1;
+      qw[0].group<0>(qn2, qumax, a0, a1);
+      qw[1].group<0>(qn2, qumax, b0, b1);
+      tw[0].group<0>(tn2, tumax, c0, c1);
+      tw[1].group<0>(tn2, tumax, d0, d1);
+      qp.build(a0, a1, b0, b1);
+      tp.build(c0, c1, d0, d1);
+    }
+    rows16(rg);
+    if (rg + 16 >= rmax) break;
+    {
+      uint32_t a0, a1, b0, b1, c0, c1, d0, d1;
+      qw[0].group<1>(qn2, qumax, a0, a1);
+      qw[1].group<1>(qn2, qumax, b0, b1);
+      tw[0].group<1>(tn2, tumax, c0, c1);
+      tw[1].group<1>(tn2, tumax, d0, d1);
+      qp.build(a0, a1, b0, b1);
+      tp.build(c0, c1, d0, d1);
+    }
+    rows16(rg + 16);
+  }
+  // the longer half (both when equal; a lone task's copy writes the same)
+  if (rows[0] >= rows[1] && rows[0] > 0) dp16_end<G>(Z, M, a.dres, task0, 0, rows[0]);
+  if (rows[1] >= rows[0] && rows[1] > 0) dp16_end<G>(Z, M, a.dres, task1, 1, rows[1]);
+}
+
 // Launchers of k_dp for a runtime gap (2..16); each lives in its own
 // translation unit (dp_global.hip, dp_local.hip).
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
+// k_dp16 over order[0, *n16) (dp16.hip); a.n is the task count.
+void launch_dp16(int gap, const DPArgs& a, hipStream_t st);
 
-// DP task order: rows descending (key = maxrows - rows), so the longest
-// extensions start first and a wavefront's lanes run near-equal row counts.
-static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(uint32_t nhit, const int32_t* __restrict__ hs,
-                                                                const int32_t* __restrict__ hq,
-                                                                const int32_t* __restrict__ qlen, int lmer,
-                                                                int maxrows, uint64_t* __restrict__ key,
-                                                                uint32_t* __restrict__ task) {
+// DP task order: the tasks k_dp16 can take first (class bit clear), then
+// the rest; each part by rows descending (key low bits = (maxrows - rows) >>
+// rowshift, the shift keeping the class bit inside the same radix passes),
+// so the longest extensions start first and a wavefront's lanes run
+// near-equal row counts.  The order only affects speed: every task writes
+// its own result.
+struct TaskKeyArgs {
+  uint32_t nhit;
+  const int32_t *hs, *hgs, *ht, *hq, *qlen;
+  const uint8_t* qclean;   // per query strand: no star / X class
+  const int32_t* tlen;
+  const int32_t* twin;
+  const uint64_t* tslot;
+  const uint64_t* wslot;
+  const uint8_t* tclean;   // per target sequence: no star / X class
+  int lmer, gap, packed;   // packed = 0: every task goes to k_dp (local mode)
+  int maxrows, rowshift, classbit;
+  uint64_t* key;
+  uint32_t* task;
+  unsigned long long* rows16;  // 8 counters (by workgroup): rows of the 16-bit tasks
+};
+
+static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
+  __shared__ uint32_t wsum[kBlock / 64];
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= 2 * nhit) return;
-  const uint32_t i = t >> 1;
-  const int q = hq[i];
-  const int rows = (t & 1u) ? q : qlen[hs[i]] - (q + lmer);
-  key[t] = static_cast<uint64_t>(maxrows - rows);
-  task[t] = t;
+  bool p16 = false;
+  int rows = 0;
+  if (t < 2 * a.nhit) {
+    const uint32_t i = t >> 1;
+    const bool bwd = (t & 1u) == 0u;
+    const int s = a.hs[i], q = a.hq[i];
+    const int qLen = a.qlen[s];
+    rows = bwd ? qLen - (q + a.lmer) : q;
+    if (a.packed && rows > 0 && qLen <= kPacked16MaxLen && a.qclean[s]) {
+      const int gs = a.hgs[i], tIdx = a.ht[i];
+      if (a.tclean[gs]) {
+        // the band covers target indices tb + ts * [0, rows + W - 2] (k_dp's ring)
+        const int64_t span = rows + 2 * a.gap - 1;
+        const int64_t first = bwd ? tIdx + a.lmer - a.gap : tIdx + a.gap - 1 - span;
+        const int64_t last = first + span;
+        const int64_t ts = static_cast<int64_t>(a.tslot[gs]);
+        const int w = a.twin[gs];
+        const int64_t lo = max(static_cast<int64_t>(a.wslot[w]) - ts, int64_t{0});
+        const int64_t hi = min(static_cast<int64_t>(a.wslot[w + 1]) - ts, static_cast<int64_t>(a.tlen[gs]));
+        p16 = first >= lo && last < hi;
+      }
+    }
+    a.key[t] = (static_cast<uint64_t>(!p16) << a.classbit) |
+               static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
+    a.task[t] = t;
+  }
+  const uint32_t r16 = wave_sum_incl(p16 ? static_cast<uint32_t>(rows) : 0u);
+  if (lane_id() == 63) wsum[wave_id()] = r16;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += wsum[w];
+    if (tot) atomicAdd(&a.rows16[blockIdx.x & 7u], static_cast<unsigned long long>(tot));
+  }
+}
+
+// Number of keys of a sorted array below 2^bit (the tasks with the class bit
+// clear): one wavefront narrows [lo, hi) 64-fold per step.
+static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t n, int bit, uint32_t* __restrict__ out,
+                                     unsigned long long* __restrict__ out64) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t lo = 0, hi = n;  // answer in [lo, hi]
+  while (lo < hi) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t p = lo + lane * step;
+    const bool above = p >= hi || (keys[p] >> bit) != 0;
+    const uint64_t b = __builtin_amdgcn_ballot_w64(above);
+    const uint32_t f = b ? static_cast<uint32_t>(__builtin_ctzll(b)) : 64u;  // first lane at or past the answer
+    const uint32_t nlo = f == 0 ? lo : lo + (f - 1) * step + 1;
+    const uint32_t nhi = f == 64 ? hi : min(hi, lo + f * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  if (lane == 0) {
+    *out = lo;
+    *out64 = lo;
+  }
 }
 
 // ---------------------------------------------------------------- results

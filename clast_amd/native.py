@@ -353,8 +353,9 @@ class Hits:
         return out
 
     def counts(self) -> np.ndarray:
-        """raw seeds, hits entering the DP, DP cells, result rows."""
-        c = np.zeros(4, np.int64)
+        """raw seeds, hits entering the DP, DP cells, result rows, DP tasks
+        run in packed 16-bit lanes, their cells."""
+        c = np.zeros(6, np.int64)
         _check(hip().clast_hits_counts(self.h, _ptr(c)), "clast_hits_counts")
         return c
 

@@ -206,6 +206,56 @@ def test_long_reads_mix_fast_and_global_paths():
         assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
 
 
+@pytest.mark.parametrize("gap", [2, 8, 16])
+def test_dp_16bit_lanes(gap, monkeypatch):
+    # The DP runs a task in packed 16-bit lanes when its query has at most
+    # 30000 bases, neither its query strand nor its target sequence holds a
+    # star (E F I J L O P Q Z) or X letter, and its band stays inside the
+    # target sequence and window; every other task takes the int32 kernel.
+    # Both kinds in one set: a target sequence and some reads with such
+    # letters, reads from sequence ends, read lengths 60-1500 (two tasks of
+    # unequal rows share a lane).  Rows equal the oracle's with the 16-bit
+    # path on and off.
+    t, r = workload(n_t=5, t_len=20_000, n_r=240, len_mode=gen.LEN_LOGUNIFORM, read_len=60, len_b=1500,
+                    indel=0.02, seed=11)
+    tb = t.bases.copy()
+    o1 = int(t.offsets[1])
+    tb[o1 + 3000:o1 + 20_000:997] = ord("X")
+    tb[o1 + 3500:o1 + 20_000:1499] = ord("E")
+    t = gen.Target(tb, t.offsets)
+    seqs = [r.seq(i) for i in range(len(r))]
+    for i in range(0, len(seqs), 7):  # star letters in every 7th read
+        b = bytearray(seqs[i])
+        b[len(b) // 2] = ord("Q")
+        seqs[i] = bytes(b)
+    for k in (0, 2, 3, 4):  # reads from both ends of clean sequences
+        seqs += [t.seq(k)[:180], t.seq(k)[-180:], t.seq(k)[5:90], t.seq(k)[-95:-4]]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t), gap=gap)
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 45_000, query=False)
+    qwins = native.windows(ql.gateway, 40_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    print(f"dp16 gap={gap}: {len(twins)}x{len(qwins)} windows", flush=True)
+    h = native.search_batch(p, ix, q)
+    got, cnt = h.arrays(), h.counts()
+    monkeypatch.setenv("CLAST_DP16", "0")
+    h0 = native.search_batch(p, ix, q)
+    got0, cnt0 = h0.arrays(), h0.counts()
+    ix.close()
+    q.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    ntask = 2 * int(cnt[1])
+    assert 0 < cnt[4] < ntask and 0 < cnt[5] < cnt[2], f"16-bit tasks {cnt[4]} of {ntask}"
+    assert cnt0[4] == 0 and cnt0[2] == cnt[2]
+    assert_rows_equal(got, o["final"], what=f"dp16 gap {gap}")
+    assert_rows_equal(got0, o["final"], what=f"int32 dp gap {gap}")
+
+
 @pytest.mark.parametrize("form", ["direct", "bitmap"])
 @pytest.mark.parametrize("lmer", [12, 15, 17])
 def test_index_forms(form, lmer, monkeypatch):
