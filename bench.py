@@ -280,7 +280,8 @@ def main() -> int:
             h.close()
     lat = np.array(lat)
     times = np.array([x[0] for x in rec])  # per search: lookup, segments, sort, filt, dp, res, total, raw seeds
-    # raw seeds, DP hits, DP cells, rows, 16-bit DP tasks, their cells
+    # raw seeds, DP hits, DP cells, rows, 16-bit DP tasks, their cells, long
+    # DP tasks (one lane per band), their cells
     counts = np.array([x[1] for x in rec], dtype=np.float64)
     # k_seed_count algorithmic bytes per launch: 16 B per query k-mer position
     # (8 B index key lookup + 8 B cell (offset, count)); positions per launch
@@ -311,7 +312,8 @@ def main() -> int:
         "dp": {"GCUPS": round(cells / sec(4) / 1e9, 1), "Tops/s": round(14.0 * cells / sec(4) / 1e12, 2),
                "peak_Tops/s": VALU_INT32_PEAK_TOPS, "hits_per_search": int(dp_hits / len(counts)),
                "cells_per_search": int(cells / len(counts)),
-               "cells_16bit_frac": round(float(counts[:, 5].sum()) / max(cells, 1.0), 4)},
+               "cells_16bit_frac": round(float(counts[:, 5].sum()) / max(cells, 1.0), 4),
+               "cells_band_frac": round(float(counts[:, 7].sum()) / max(cells, 1.0), 4)},
         "results": {"GB/s": round(res_bytes / sec(5) / 1e9, 1), "peak": HBM_PEAK_GBS},
     }
     stage_names = ["seed_lookup", "seed_segments", "global_seed_sort", "global_seed_filters", "dp", "results",

@@ -174,7 +174,8 @@ struct clast_hits_s {
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int64_t counts[6] = {0, 0, 0, 0, 0, 0};  // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells
+  // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells, long tasks, their cells
+  int64_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -220,6 +221,12 @@ static bool use_direct(int B, uint32_t nent) {
 // same results; the switch exists to compare them.
 static bool dp16_disabled() {
   const char* f = std::getenv("CLAST_DP16");
+  return f && !std::strcmp(f, "0");
+}
+// CLAST_DP_BAND=0 keeps long extensions (at least kBandRows rows) off the
+// lane-per-band kernel (k_dp_band); same results, for comparison.
+static bool band_disabled() {
+  const char* f = std::getenv("CLAST_DP_BAND");
   return f && !std::strcmp(f, "0");
 }
 
@@ -538,9 +545,10 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const int dbits = bits_for(static_cast<uint64_t>(ix.max_tlen) + q.max_qlen + 2);
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
-  // [2, 10): DP cell counters, [10, 18): rows of 16-bit DP tasks, [18]: their count
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 20);
-  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 20, 0ull);
+  // [2, 10): DP cell counters, [10, 18): rows of 16-bit DP tasks, [18]: their
+  // count, [19]: long task count, [20, 28): their rows, [28, 36): all DP rows
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 36);
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 36, 0ull);
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
@@ -764,28 +772,45 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     int4* dres = ws.get<int4>("d.res", ntask);
     const int maxrows = std::max(q.max_qlen - p.lmer, 0);
     const int rowbits = bits_for(static_cast<uint64_t>(maxrows) + 1);
-    // class bit on top of the rows, within the radix passes the rows need
-    const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
-    const int rowshift = std::max(0, rowbits - (sortbits - 1));
-    uint32_t* n16 = ws.get<uint32_t>("d.n16", 1);
+    // class bits on top of the rows (two with long tasks possible, else one:
+    // classes 1 and 2 only), within the radix passes the rows need
     const bool packed = !p.local && !dp16_disabled();
+    const bool band = maxrows >= kBandRows && !band_disabled();
+    const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
+    const int classbit = sortbits - (band ? 2 : 1);
+    const int rowshift = std::max(0, rowbits - classbit);
+    const uint64_t cbase = band ? 0 : 1;  // lowest class in the keys
+    uint32_t* split = ws.get<uint32_t>("d.split", 2);
+    int* band_rows = ws.get<int>("d.bandrows", 1);
+    if (band) {
+      auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
+      k_dp_rows_sum<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, rsum);
+      k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * rt.compute_units() * 128ull, band_rows);
+    }
     const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
                          ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
-                         maxrows, rowshift, sortbits - 1, dk, dporder,
-                         reinterpret_cast<unsigned long long*>(tot64 + 10)};
+                         band ? 1 : 0, band_rows, maxrows, rowshift, classbit, dk, dporder,
+                         reinterpret_cast<unsigned long long*>(tot64 + 10),
+                         reinterpret_cast<unsigned long long*>(tot64 + 20)};
     k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
     radix_sort_pairs(rt, dk, dporder, ntask, sortbits);
-    k_count_below<<<1, 64, 0, st>>>(dk, ntask, sortbits - 1, n16, reinterpret_cast<unsigned long long*>(tot64 + 18));
+    k_count_below<<<1, 64, 0, st>>>(dk, ntask, (1 - cbase) << classbit, split, nullptr);
+    k_count_below<<<1, 64, 0, st>>>(dk, ntask, (2 - cbase) << classbit, split + 1, nullptr);
+    k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
     DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, n16, q.qnib.ptr, ix.tnib.ptr, dres, q.bmax,
-              ix.bmax, p.lmer, ot, oq, thl, qhl, om, osc};
+              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres, q.bmax,
+              ix.bmax, p.lmer, static_cast<uint32_t>(4 * std::max(rt.compute_units(), 1)), ot, oq, thl, qhl, om,
+              osc};
     mark(EV_DP0);
-    if (p.local) {
-      launch_dp_local(p.gap, da, st);
-    } else {
-      if (packed) launch_dp16(p.gap, da, st);
-      launch_dp_global(p.gap, da, st);
+    // the long tasks run beside the others (side stream, raised wave priority)
+    if (band) {
+      rt.fork(0);
+      launch_dp_band(p.gap, p.local != 0, da, rt.side_stream());
     }
+    if (packed) launch_dp16(p.gap, da, st);
+    if (p.local) launch_dp_local(p.gap, da, st);
+    else launch_dp_global(p.gap, da, st);
+    if (band) rt.join(0);
     mark(EV_DP1);
     rt.check_launch("k_dp");
     k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);
@@ -946,12 +971,13 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
-  uint64_t cnt[17];
-  rt.read_scalars(tot64 + 2, cnt, 17);
-  uint64_t cells = 0, rows16 = 0;
+  uint64_t cnt[26];
+  rt.read_scalars(tot64 + 2, cnt, 26);
+  uint64_t cells = 0, rows16 = 0, rowsband = 0;
   for (int k = 0; k < 8; ++k) {
     cells += cnt[k];
     rows16 += cnt[8 + k];
+    rowsband += cnt[18 + k];
   }
   out.counts[0] = nseed;
   out.counts[1] = nhit;
@@ -959,6 +985,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   out.counts[3] = nfinal;
   out.counts[4] = static_cast<int64_t>(cnt[16]);
   out.counts[5] = static_cast<int64_t>(rows16) * (2 * p.gap + 1);
+  out.counts[6] = static_cast<int64_t>(cnt[17]);
+  out.counts[7] = static_cast<int64_t>(rowsband) * (2 * p.gap + 1);
 }
 }  // namespace
 
@@ -1135,10 +1163,10 @@ int clast_hits_stage_times(clast_hits h, float* out8) {
   });
 }
 
-int clast_hits_counts(clast_hits h, int64_t* out6) {
+int clast_hits_counts(clast_hits h, int64_t* out8) {
   return guard([&] {
-    if (!h || !out6) throw ClastError(CLAST_ERR_ARG, "null argument");
-    std::memcpy(out6, h->counts, sizeof h->counts);
+    if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out8, h->counts, sizeof h->counts);
   });
 }
 

@@ -1040,12 +1040,13 @@ struct DPArgs {
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
   const uint32_t* order;   // task of thread i: (hit << 1) | direction, longest first
-  const uint32_t* n16;     // order[0, *n16) are the 16-bit tasks (k_dp16), the rest go to k_dp
+  const uint32_t* split;   // order[0, split[0]) go to k_dp_band, [split[0], split[1]) to k_dp16, the rest to k_dp
   const uint32_t* qnib;    // 4-bit copies of qbase / tbase (k_pack_nibbles), read by k_dp16
   const uint32_t* tnib;
   int4* dres;              // per task 2 * hit + direction: (move + end, move, match, score)
   int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
   int lmer;
+  uint32_t round;          // wavefronts per dispatch round (one per SIMD): k_dp / k_dp16 snake order
   int32_t* o_tidx;
   int32_t* o_qidx;
   int32_t* o_thl;
@@ -1215,12 +1216,25 @@ __device__ __forceinline__ void dp_init(int (&S)[2 * G + 3], int (&V)[2 * G + 3]
 // extensions proceed side by side and every wavefront's lanes run near-equal
 // row counts.  Query and target bytes arrive through 16-byte windows
 // (ByteWin); k_dp_combine adds the two directions up.
+// Wavefront w's chunk of a longest-first task order: wavefronts are placed
+// one per SIMD per round, so within every other round the order runs
+// backwards (a snake) and each SIMD gets long and short chunks alike.
+__device__ __forceinline__ uint32_t snake_chunk(uint32_t w, uint32_t nchunk, uint32_t round) {
+  const uint32_t r = w / round, pos = w - r * round;
+  const uint32_t cnt = min(round, nchunk - r * round);
+  return r * round + ((r & 1u) ? cnt - 1u - pos : pos);
+}
+
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t first = *a.n16;
-  if (t >= a.n - first) return;
+  const uint32_t first = a.split[1];
+  const uint32_t n = a.n - first;
+  const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t nchunk = (n + 63) / 64;
+  if (wv >= nchunk) return;
+  const uint32_t t = snake_chunk(wv, nchunk, a.round) * 64 + (threadIdx.x & 63u);
+  if (t >= n) return;
   const uint32_t task = a.order[first + t];
   const uint32_t i = task >> 1;
   const bool bwd = (task & 1u) == 0u;
@@ -1490,11 +1504,16 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
   constexpr int kInitGroups = (W + kLead) / 16;  // even
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t n = *a.n16;
+  const uint32_t first = a.split[0];
+  const uint32_t n = a.split[1] - first;
+  const uint32_t npair = (n + 1) / 2;
+  const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t nchunk = (npair + 63) / 64;
+  if (wv >= nchunk) return;
+  const uint32_t t = snake_chunk(wv, nchunk, a.round) * 64 + (threadIdx.x & 63u);
   if (2 * t >= n) return;
-  const uint32_t task0 = a.order[2 * t];
-  const uint32_t task1 = 2 * t + 1 < n ? a.order[2 * t + 1] : task0;
+  const uint32_t task0 = a.order[first + 2 * t];
+  const uint32_t task1 = 2 * t + 1 < n ? a.order[first + 2 * t + 1] : task0;
   const int L = a.lmer;
   const uint2* tn2 = reinterpret_cast<const uint2*>(a.tnib);
   const uint2* qn2 = reinterpret_cast<const uint2*>(a.qnib);
@@ -1602,19 +1621,261 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   if (rows[1] >= rows[0] && rows[1] > 0) dp16_end<G>(Z, M, a.dres, task1, 1, rows[1]);
 }
 
+// ------------------------------------------- banded DP, one lane per band
+//
+// Long extensions (at least kBandRows rows) have a latency of rows x W cells
+// in k_dp / k_dp16, one lane walking the whole band row by row.  Here a task
+// pair occupies W consecutive lanes, lane b holding band b (k_dp's band
+// b + 1): cell (r, b) of task A is computed at step 2r + b and of task B at
+// step 2r + b + 1.  Cell (r, b) reads cell (r, b-1) (horizontal term, new
+// score and match count) and cell (r-1, b+1) (old score, vertical term,
+// match count), both computed one step earlier by the neighbouring lanes
+// (DPP lane shifts), and its own cell (r-1, b) from two steps earlier.  The
+// query class of row r moves up the lanes from lane 0, which reads it; the
+// target class of position r + b moves down from lane W-1.  Each lane
+// alternates A and B, so every lane computes a cell every step, and a
+// task's latency is two steps per row.  Arithmetic, tie rules and margins
+// are k_dp's (int32, any letters, both modes).
+
+constexpr int kBandRows = 1024;
+
+// value of lane - 1 / lane + 1 (DPP wave_shr:1 / wave_shl:1); the wavefront's
+// end lanes get 0, which no group reads (its end lanes use constants)
+__device__ __forceinline__ int from_below(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int from_above(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true); }
+// a where m is all ones, b where it is zero
+__device__ __forceinline__ int pick(int m, int a, int b) { return b ^ ((a ^ b) & m); }
+
+// One task's per-lane view in k_dp_band.
+struct BandTask {
+  int rows;
+  int r;          // this lane's row at its next step of this task
+  int ti, ts;     // target index this lane reads next (lane W-1) and its step
+  int lo, hi;     // t_mask range
+  int S, M;       // this lane's cell of the previous row
+  int fS, fM, fR; // local: best cell of this band (score, match, row); global: unused
+  uint32_t t0;    // target class of position b - 1 (this lane's row -1 export)
+  ByteWin st;     // lane 0: query classes; lane W-1: target classes
+};
+
+// row_words without branches: every lane of k_dp_band holds its own row.
+__device__ __forceinline__ void row_words_sel(uint32_t qc, uint32_t& wpm, uint32_t& wmc) {
+  constexpr uint32_t kPm = 0x8FDDDDDu, kMc = 0x8F00000u, kStar = 0x8FFFFFFu, kAllX = 0x8888888u;
+  const uint32_t sh = 4u * (qc & 3u);
+  const uint32_t m4 = 0u - static_cast<uint32_t>(qc < 4u);
+  const uint32_t other_pm = qc == kClsOther ? kPm : qc == kClsStar ? kStar : kAllX;
+  const uint32_t other_mc = qc == kClsOther ? kMc : qc == kClsStar ? kStar : kAllX;
+  wpm = ((kPm ^ (0xCu << sh)) & m4) | (other_pm & ~m4);
+  wmc = ((kMc | (0x1u << sh)) & m4) | (other_mc & ~m4);
+}
+
+// One step of task X in lane b: computes cell (X.r, b) from the exports of
+// the neighbouring lanes' previous step and updates this lane's exports.
+template <int G, bool LOCAL>
+__device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, int Vinit, int Minit, int& eS,
+                                          int& eV, int& eH, int& eM, uint32_t& eT, uint32_t& eQ) {
+  constexpr int W = 2 * G + 1;
+  constexpr int half = G + 1;
+  // band W+1's constant row (k_dp's dp_init)
+  constexpr int kTopS = (W + 1 - half) * kGapExt + kGapOpen;
+  constexpr int kTopV = kTopS + kGapOpen;
+  const int lS = from_below(eS), lH = from_below(eH), lM = from_below(eM);
+  const int rS = from_above(eS), rV = from_above(eV), rM = from_above(eM);
+  const uint32_t lQ = static_cast<uint32_t>(from_below(static_cast<int>(eQ)));
+  const uint32_t rT = static_cast<uint32_t>(from_above(static_cast<int>(eT)));
+  const uint32_t sb = X.st.next(j);
+  const bool bot = b == 0, top = b == W - 1;
+  const uint32_t qc = bot ? sb : lQ;
+  const uint32_t tm = 4u * ((X.ti >= X.lo) & (X.ti < X.hi) ? sb : kClsX);  // t_mask
+  const uint32_t T = top ? tm : rT;
+  X.ti += X.ts;
+  const int v0 = max(rS + (kGapOpen + kGapExt), rV + kGapExt);
+  const int h0 = max(lS + (kGapOpen + kGapExt), lH + kGapExt);
+  const int v = top ? max(kTopS + (kGapOpen + kGapExt), kTopV + kGapExt) : v0;
+  const int h = bot ? max(-30000 + (kGapOpen + kGapExt), -30000 + kGapExt) : h0;
+  const int Mv = top ? 0 : rM;
+  const int Mh = bot ? -30000 : lM;
+  uint32_t wpm, wmc;
+  row_words_sel(qc, wpm, wmc);
+  const int pr = __builtin_amdgcn_sbfe(static_cast<int>(wpm), T, 4);
+  const int mr = __builtin_amdgcn_sbfe(static_cast<int>(wmc), T, 4);
+  const bool bad = pr == -8;
+  const int pm = bad ? kBad : pr;
+  const int mc = bad ? kBad : mr;
+  const int pt = X.S + pm;
+  const int vh = max(v, h);
+  const int nm_gap = v > h ? Mv : Mh;
+  const int nm_diag = X.M + mc;
+  const int dmask = -static_cast<int>(pt > vh);
+  const int nm = nm_gap ^ ((nm_gap ^ nm_diag) & dmask);
+  const int ns = max(pt, vh);
+  const int am = -static_cast<int>(static_cast<uint32_t>(X.r) < static_cast<uint32_t>(X.rows));
+  if (LOCAL) {
+    const int bm = am & -static_cast<int>(X.fS < ns);
+    X.fS = pick(bm, ns, X.fS);
+    X.fM = pick(bm, nm, X.fM);
+    X.fR = pick(bm, X.r, X.fR);
+  }
+  X.S = pick(am, ns, X.S);
+  X.M = pick(am, nm, X.M);
+  // before its first row a lane shows its row -1 (k_dp's initial band)
+  eS = pick(am, ns, Sinit);
+  eV = pick(am, v, Vinit);
+  eM = pick(am, nm, Minit);
+  eT = static_cast<uint32_t>(pick(am, static_cast<int>(T), static_cast<int>(X.t0)));
+  eH = h;
+  eQ = qc;
+  X.r += 1;
+}
+
+template <int G, bool LOCAL>
+static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
+  constexpr int W = 2 * G + 1;
+  constexpr int kPairs = 64 / W;  // task pairs per wavefront
+  constexpr int half = G + 1;
+  __shared__ int red[kBlock / 64][64][6];
+  // these few long tasks bound the DP's end: issue before the other DP
+  // kernels' wavefronts on the same SIMD
+  __builtin_amdgcn_s_setprio(3);
+  const int lane = lane_id();
+  const int grp = lane / W;
+  const int b = lane - grp * W;
+  const uint32_t nband = a.split[0];
+  const uint32_t pair = (blockIdx.x * (kBlock / 64) + wave_id()) * kPairs + static_cast<uint32_t>(grp);
+  // every lane stays to the workgroup barrier; lanes without a pair run
+  // task 0's setup and no steps
+  const bool live = grp < kPairs && 2 * pair < nband;
+  const uint32_t taskA = live ? a.order[2 * pair] : 0u;
+  const uint32_t taskB = live && 2 * pair + 1 < nband ? a.order[2 * pair + 1] : taskA;
+  const int L = a.lmer;
+  const int64_t tbytes = 16 * (static_cast<int64_t>(a.t_bmax) + 1);
+  // X0 is the task this lane works on at even steps: A for even b, B for odd
+  BandTask X0, X1;
+  auto setup = [&](BandTask& x, int k) {
+    const uint32_t task = ((b & 1) ^ k) ? taskB : taskA;
+    const uint32_t i = task >> 1;
+    const bool bwd = (task & 1u) == 0u;
+    const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
+    const int qLen = a.qlen[s];
+    x.rows = bwd ? qLen - (qIdx + L) : qIdx;
+    // task A's cell (r, b) at step 2r + b, B's at 2r + b + 1: the row at
+    // this lane's first step of the task (step k)
+    const int c = (b & 1) ^ k;  // 0: task A's timing, 1: B's
+    x.r = (k - b - c) >> 1;
+    const int tLen = a.tlen[gs];
+    const int64_t tslot = static_cast<int64_t>(a.tslot[gs]);
+    const int w = a.twin[gs];
+    const int wlo = static_cast<int>(static_cast<int64_t>(a.wslot[w]) - tslot);
+    const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - tslot);
+    x.ts = bwd ? 1 : -1;
+    const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+    x.lo = bwd ? wlo : max(wlo, 0);
+    x.hi = bwd ? min(whi, tLen) : whi;
+    // lane W-1 reads position r + W - 1 of row r: its stream starts at
+    // position G, so stream index j = floor(step / 2) in every lane
+    x.ti = tb + x.ts * G;
+    const bool top = b == W - 1;
+    x.st.init(top ? a.tbase : a.qbase, top ? a.t_bmax : a.q_bmax,
+              top ? tslot + x.ti : static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), x.ts);
+    const int tp = tb + x.ts * (b - 1);
+    const int64_t g = min(max(tslot + tp, int64_t{0}), tbytes - 1);
+    x.t0 = t_mask(a.tbase[g], tp, x.lo, x.hi);
+    x.fS = 0; x.fM = 0; x.fR = -1;
+  };
+  setup(X0, 0);
+  setup(X1, 1);
+  // row -1 of band b + 1 (k_dp's dp_init)
+  const int bi = b + 1;
+  const int Sinit = bi < half ? -30000 : bi == half ? 0 : (bi - half) * kGapExt + kGapOpen;
+  const int Vinit = bi < half ? -30000 : bi == half ? kGapOpen : Sinit + kGapOpen;
+  const int Minit = bi < half ? -30000 : 0;
+  X0.S = X1.S = Sinit;
+  X0.M = X1.M = Minit;
+  // exports of step -1 (this lane's X1 task at row -1)
+  int eS = Sinit, eV = Vinit, eH = 0, eM = Minit;
+  uint32_t eT = X1.t0, eQ = 0;
+  // last step: task A ends at lane W-1, step 2 (rowsA - 1) + W - 1; B one later
+  const bool odd = (b & 1) != 0;
+  const int rowsA = odd ? X1.rows : X0.rows, rowsB = odd ? X0.rows : X1.rows;
+  const int tend = live ? max(2 * (rowsA - 1) + W - 1, 2 * (rowsB - 1) + W) + 1 : 0;
+#pragma unroll 1
+  for (int t = 0; t < tend; t += 2) {
+    const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
+    band_step<G, LOCAL>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+    band_step<G, LOCAL>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+  }
+  // end node per task: the group's lanes through LDS, scanned in band order
+  const int wv = wave_id();
+  const int aS = LOCAL ? (odd ? X1.fS : X0.fS) : (odd ? X1.S : X0.S);
+  const int aM = LOCAL ? (odd ? X1.fM : X0.fM) : (odd ? X1.M : X0.M);
+  const int bS = LOCAL ? (odd ? X0.fS : X1.fS) : (odd ? X0.S : X1.S);
+  const int bM = LOCAL ? (odd ? X0.fM : X1.fM) : (odd ? X0.M : X1.M);
+  red[wv][lane][0] = aS;
+  red[wv][lane][1] = aM;
+  red[wv][lane][2] = odd ? X1.fR : X0.fR;
+  red[wv][lane][3] = bS;
+  red[wv][lane][4] = bM;
+  red[wv][lane][5] = odd ? X0.fR : X1.fR;
+  __syncthreads();
+  if (live && b < 2 && !(b == 1 && taskB == taskA)) {
+    const int k = b;  // lane 0 finishes task A, lane 1 task B
+    const int base = grp * W;
+    const int rows = k ? rowsB : rowsA;
+    int move, endp, bestS, bestM;
+    if (LOCAL) {
+      // k_dp keeps the first cell, in row-major order, of the highest score
+      // above 0: highest score, then lowest row, then lowest band, as one key
+      // (0 for a band that never rose above 0)
+      uint64_t best = 0;
+      int bq = -1;
+      for (int q = 0; q < W; ++q) {
+        const int sc = red[wv][base + q][3 * k], rr = red[wv][base + q][3 * k + 2];
+        const uint64_t key = rr < 0 ? 0ull
+                                    : (static_cast<uint64_t>(static_cast<uint32_t>(sc) ^ 0x80000000u) << 32) |
+                                          (static_cast<uint64_t>(0x3FFFFFF - rr) << 6) | static_cast<uint64_t>(63 - q);
+        if (key > best) {
+          best = key;
+          bq = q;
+        }
+      }
+      bestS = bq < 0 ? 0 : red[wv][base + bq][3 * k];
+      bestM = bq < 0 ? 0 : red[wv][base + bq][3 * k + 1];
+      move = bq < 0 ? 0 : red[wv][base + bq][3 * k + 2] + 1;
+      endp = (bq < 0 ? G : bq) - G;
+    } else {
+      bestS = -30000;  // band 0
+      bestM = -30000;
+      int e = -1;
+      for (int q = 0; q < W; ++q) {
+        const int sc = red[wv][base + q][3 * k];
+        if (bestS < sc) {
+          bestS = sc;
+          bestM = red[wv][base + q][3 * k + 1];
+          e = q;
+        }
+      }
+      move = rows;
+      endp = e - G;
+    }
+    a.dres[k ? taskB : taskA] = make_int4(move + endp, move, bestM, bestS);
+  }
+}
+
 // Launchers of k_dp for a runtime gap (2..16); each lives in its own
 // translation unit (dp_global.hip, dp_local.hip).
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
-// k_dp16 over order[0, *n16) (dp16.hip); a.n is the task count.
+// k_dp16 over order[split[0], split[1]) (dp16.hip), k_dp_band over
+// order[0, split[0]) (dp_band.hip); a.n is the task count.
 void launch_dp16(int gap, const DPArgs& a, hipStream_t st);
+void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
-// DP task order: the tasks k_dp16 can take first (class bit clear), then
-// the rest; each part by rows descending (key low bits = (maxrows - rows) >>
-// rowshift, the shift keeping the class bit inside the same radix passes),
-// so the longest extensions start first and a wavefront's lanes run
-// near-equal row counts.  The order only affects speed: every task writes
-// its own result.
+// DP task order by class (key bits from classbit up): 0 the long tasks
+// (k_dp_band), 1 those k_dp16 can take, 2 the rest (k_dp); each class by
+// rows descending (key low bits = (maxrows - rows) >> rowshift, the shift
+// keeping the class bits inside the same radix passes), so the longest
+// extensions start first and a wavefront's lanes run near-equal row counts.
+// The order only affects speed: every task writes its own result.
 struct TaskKeyArgs {
   uint32_t nhit;
   const int32_t *hs, *hgs, *ht, *hq, *qlen;
@@ -1624,15 +1885,18 @@ struct TaskKeyArgs {
   const uint64_t* tslot;
   const uint64_t* wslot;
   const uint8_t* tclean;   // per target sequence: no star / X class
-  int lmer, gap, packed;   // packed = 0: every task goes to k_dp (local mode)
+  int lmer, gap, packed;   // packed = 0: no task goes to k_dp16 (local mode)
+  int band;                // band = 0: no task goes to k_dp_band
+  const int* band_rows;    // tasks with at least this many rows go to k_dp_band
   int maxrows, rowshift, classbit;
   uint64_t* key;
   uint32_t* task;
-  unsigned long long* rows16;  // 8 counters (by workgroup): rows of the 16-bit tasks
+  unsigned long long* rows16;    // 8 counters (by workgroup): rows of the 16-bit tasks
+  unsigned long long* rowsband;  // 8 counters: rows of the long tasks
 };
 
 static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
-  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ uint32_t wsum[2 * (kBlock / 64)];
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   bool p16 = false;
   int rows = 0;
@@ -1642,7 +1906,8 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
     const int s = a.hs[i], q = a.hq[i];
     const int qLen = a.qlen[s];
     rows = bwd ? qLen - (q + a.lmer) : q;
-    if (a.packed && rows > 0 && qLen <= kPacked16MaxLen && a.qclean[s]) {
+    const bool long_task = a.band && rows >= *a.band_rows;
+    if (!long_task && a.packed && rows > 0 && qLen <= kPacked16MaxLen && a.qclean[s]) {
       const int gs = a.hgs[i], tIdx = a.ht[i];
       if (a.tclean[gs]) {
         // the band covers target indices tb + ts * [0, rows + W - 2] (k_dp's ring)
@@ -1656,30 +1921,81 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
         p16 = first >= lo && last < hi;
       }
     }
-    a.key[t] = (static_cast<uint64_t>(!p16) << a.classbit) |
-               static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
+    const uint64_t cls = long_task ? 0u : p16 ? 1u : 2u;
+    a.key[t] = ((cls - (a.band ? 0u : 1u)) << a.classbit) | static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
     a.task[t] = t;
   }
+  const bool lt = a.band && rows >= *a.band_rows;
   const uint32_t r16 = wave_sum_incl(p16 ? static_cast<uint32_t>(rows) : 0u);
-  if (lane_id() == 63) wsum[wave_id()] = r16;
+  const uint32_t rb = wave_sum_incl(lt ? static_cast<uint32_t>(rows) : 0u);
+  if (lane_id() == 63) {
+    wsum[wave_id()] = r16;
+    wsum[kBlock / 64 + wave_id()] = rb;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t tot = 0;
-    for (int w = 0; w < kBlock / 64; ++w) tot += wsum[w];
-    if (tot) atomicAdd(&a.rows16[blockIdx.x & 7u], static_cast<unsigned long long>(tot));
+  if (threadIdx.x < 2) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += wsum[threadIdx.x * (kBlock / 64) + w];
+    if (tot) atomicAdd(&(threadIdx.x ? a.rowsband : a.rows16)[blockIdx.x & 7u], tot);
   }
 }
 
-// Number of keys of a sorted array below 2^bit (the tasks with the class bit
-// clear): one wavefront narrows [lo, hi) 64-fold per step.
-static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t n, int bit, uint32_t* __restrict__ out,
-                                     unsigned long long* __restrict__ out64) {
+// Rows of every DP task, summed into 8 counters (by workgroup).
+static __global__ __launch_bounds__(kBlock) void k_dp_rows_sum(uint32_t nhit, const int32_t* __restrict__ hs,
+                                                               const int32_t* __restrict__ hq,
+                                                               const int32_t* __restrict__ qlen, int lmer,
+                                                               unsigned long long* __restrict__ sums) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  int rows = 0;
+  if (t < 2 * nhit) {
+    const uint32_t i = t >> 1;
+    const int q = hq[i];
+    rows = (t & 1u) ? q : qlen[hs[i]] - (q + lmer);
+  }
+  const uint32_t r = wave_sum_incl(static_cast<uint32_t>(max(rows, 0)));
+  if (lane_id() == 63) wsum[wave_id()] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += wsum[w];
+    if (tot) atomicAdd(&sums[blockIdx.x & 7u], tot);
+  }
+}
+
+// Row threshold of the long tasks: a task over twice one SIMD's average
+// share of all rows (total / slots, slots = SIMDs x 128 tasks per wavefront
+// row in k_dp16) would set the end of the per-lane kernels by itself, so it
+// goes to k_dp_band (whose latency per row is 2 steps, not W cells, at
+// about five times the instructions per cell); never below kBandRows.
+static __global__ void k_band_threshold(const unsigned long long* __restrict__ sums, unsigned long long slots,
+                                        int* __restrict__ band_rows) {
+  if (threadIdx.x != 0) return;
+  unsigned long long tot = 0;
+  for (int k = 0; k < 8; ++k) tot += sums[k];
+  const unsigned long long lim = 2ull * (tot / (slots ? slots : 1ull));
+  *band_rows = static_cast<int>(max(static_cast<unsigned long long>(kBandRows), min(lim, 0x7FFFFFFFull)));
+}
+
+// The 16-bit and long task counts (split[1] - split[0], split[0]) into the
+// work counters.
+static __global__ void k_split_counts(const uint32_t* __restrict__ split, unsigned long long* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = split[1] - split[0];
+    out[1] = split[0];
+  }
+}
+
+// Number of keys of a sorted array below `limit` (the tasks of the classes
+// under a bound): one wavefront narrows [lo, hi) 64-fold per step.
+static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t n, uint64_t limit,
+                                     uint32_t* __restrict__ out, unsigned long long* __restrict__ out64) {
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t lo = 0, hi = n;  // answer in [lo, hi]
   while (lo < hi) {
     const uint32_t step = (hi - lo + 63) / 64;
     const uint32_t p = lo + lane * step;
-    const bool above = p >= hi || (keys[p] >> bit) != 0;
+    const bool above = p >= hi || keys[p] >= limit;
     const uint64_t b = __builtin_amdgcn_ballot_w64(above);
     const uint32_t f = b ? static_cast<uint32_t>(__builtin_ctzll(b)) : 64u;  // first lane at or past the answer
     const uint32_t nlo = f == 0 ? lo : lo + (f - 1) * step + 1;
@@ -1689,7 +2005,7 @@ static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t
   }
   if (lane == 0) {
     *out = lo;
-    *out64 = lo;
+    if (out64) *out64 = lo;
   }
 }
 

@@ -57,7 +57,7 @@ struct DevArray {
 // queued kernel still uses the old buffer.
 class Workspace {
  public:
-  Workspace(hipStream_t s, hipStream_t copy) : stream_(s), copy_(copy) {}
+  Workspace(hipStream_t s, hipStream_t copy, hipStream_t side) : stream_(s), copy_(copy), side_(side) {}
   ~Workspace() {
     for (auto& kv : slots_) (void)hipFree(kv.second.ptr);
   }
@@ -69,6 +69,7 @@ class Workspace {
       if (s.ptr) {
         hip_check(hipStreamSynchronize(stream_), "workspace sync");
         hip_check(hipStreamSynchronize(copy_), "workspace sync");
+        hip_check(hipStreamSynchronize(side_), "workspace sync");
         hip_check(hipFree(s.ptr), "hipFree");
         s.ptr = nullptr;
         s.bytes = 0;
@@ -90,7 +91,7 @@ class Workspace {
     void* ptr = nullptr;
     uint64_t bytes = 0;
   };
-  hipStream_t stream_, copy_;
+  hipStream_t stream_, copy_, side_;
   std::map<std::string, Slot> slots_;
 };
 
@@ -100,14 +101,19 @@ class Runtime {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&side_stream_, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : out_done_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    for (auto& e : side_ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipHostMalloc(&pinned_, kPinned, hipHostMallocDefault), "hipHostMalloc");
-    ws_ = new Workspace(stream_, copy_stream_);
+    ws_ = new Workspace(stream_, copy_stream_, side_stream_);
+    hip_check(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
   }
   ~Runtime() {
     delete ws_;
     (void)hipHostFree(pinned_);
     for (auto& e : out_done_) (void)hipEventDestroy(e);
+    for (auto& e : side_ev_) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(side_stream_);
     (void)hipStreamDestroy(copy_stream_);
     (void)hipStreamDestroy(stream_);
   }
@@ -115,6 +121,19 @@ class Runtime {
   // Device-to-host copies of result rows run here, so one search's copy
   // overlaps the next search's kernels.
   hipStream_t copy_stream() const { return copy_stream_; }
+  // Kernels that overlap the compute stream's work inside one search (the
+  // long DP extensions); fork(k) / join(k) order it against the compute
+  // stream.
+  hipStream_t side_stream() const { return side_stream_; }
+  void fork(int k) {
+    hip_check(hipEventRecord(side_ev_[k], stream_), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(side_stream_, side_ev_[k], 0), "hipStreamWaitEvent");
+  }
+  void join(int k) {
+    hip_check(hipEventRecord(side_ev_[k], side_stream_), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(stream_, side_ev_[k], 0), "hipStreamWaitEvent");
+  }
+  int compute_units() const { return ncu_; }
   // Result rows are staged in one of two device buffers, alternating; the
   // event marks the end of the last copy out of buffer k.
   int next_out_slot() { return out_slot_ ^= 1; }
@@ -126,7 +145,9 @@ class Runtime {
   std::mutex& mutex() { return mu_; }
 
   void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
-  void sync() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  void sync() {
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
 
   // Reads `count` device scalars of type T (count * sizeof(T) <= 4 KiB).
   template <class T>
@@ -141,6 +162,9 @@ class Runtime {
   int device_;
   hipStream_t stream_ = nullptr;
   hipStream_t copy_stream_ = nullptr;
+  hipStream_t side_stream_ = nullptr;
+  hipEvent_t side_ev_[2] = {nullptr, nullptr};
+  int ncu_ = 0;
   hipEvent_t out_done_[2] = {nullptr, nullptr};
   bool out_used_[2] = {false, false};
   int out_slot_ = 0;

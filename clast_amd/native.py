@@ -354,8 +354,9 @@ class Hits:
 
     def counts(self) -> np.ndarray:
         """raw seeds, hits entering the DP, DP cells, result rows, DP tasks
-        run in packed 16-bit lanes, their cells."""
-        c = np.zeros(6, np.int64)
+        run in packed 16-bit lanes, their cells, DP tasks run one lane per
+        band, their cells."""
+        c = np.zeros(8, np.int64)
         _check(hip().clast_hits_counts(self.h, _ptr(c)), "clast_hits_counts")
         return c
 

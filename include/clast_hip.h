@@ -162,8 +162,9 @@ int clast_hits_stage_times(clast_hits h, float* out8);
  * raw seeds (index entries matched, before the corner filter), [1] hits
  * entering the DP, [2] DP cells (band width x rows, both directions), [3]
  * result rows, [4] DP tasks (hit extensions, one per direction) run in packed
- * 16-bit lanes, [5] their cells (included in [2]). */
-int clast_hits_counts(clast_hits h, int64_t* out6);
+ * 16-bit lanes, [5] their cells (included in [2]), [6] DP tasks of at least
+ * 1024 rows run one lane per band, [7] their cells (included in [2]). */
+int clast_hits_counts(clast_hits h, int64_t* out8);
 int clast_hits_destroy(clast_hits h);
 
 /* resultSorting (utilResultSorting.cu:6-47): stable sort of n result rows by

@@ -256,6 +256,48 @@ def test_dp_16bit_lanes(gap, monkeypatch):
     assert_rows_equal(got0, o["final"], what=f"int32 dp gap {gap}")
 
 
+@pytest.mark.parametrize("local,gap", [(0, 2), (0, 8), (0, 16), (1, 8), (1, 16)])
+def test_dp_lane_per_band(local, gap, monkeypatch):
+    # Extensions of at least 1024 rows run one lane per band (two tasks per
+    # lane group, interleaved a step apart); shorter ones take the other DP
+    # kernels.  Reads of 1-6 kbp, some with star or X letters, some from
+    # sequence ends (bands that leave the target), in one set.  Rows equal
+    # the oracle's with the lane-per-band path on and off.
+    t, _ = workload(n_t=3, t_len=50_000, n_r=1, seed=13)
+    long_r = gen.make_reads(gen.ReadSpec(31, 16, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=6000,
+                                         sub_rate=0.02, indel_rate=0.01), t)
+    short_r = gen.make_reads(gen.ReadSpec(32, 40, sub_rate=0.03, indel_rate=0.01), t)
+    seqs = [long_r.seq(i) for i in range(len(long_r))] + [short_r.seq(i) for i in range(len(short_r))]
+    for i in range(0, len(long_r), 5):
+        b = bytearray(seqs[i])
+        b[len(b) // 3] = ord("X")
+        b[len(b) // 2] = ord("J")
+        seqs[i] = bytes(b)
+    seqs += [t.seq(0)[:1500], t.seq(2)[-1400:], t.seq(1)[:1200]]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t), gap=gap, local=local)
+    tl, ql = lists(t, r, p.lmer)
+    twins = [(0, 3)]
+    qwins = native.windows(ql.gateway, 60_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    print(f"band local={local} gap={gap}: {len(qwins)} query windows", flush=True)
+    h = native.search_batch(p, ix, q)
+    got, cnt = h.arrays(), h.counts()
+    monkeypatch.setenv("CLAST_DP_BAND", "0")
+    h0 = native.search_batch(p, ix, q)
+    got0, cnt0 = h0.arrays(), h0.counts()
+    ix.close()
+    q.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert cnt[6] > 0 and cnt[7] > 0 and cnt0[6] == 0 and cnt0[2] == cnt[2], f"band tasks {cnt[6]}"
+    assert_rows_equal(got, o["final"], what=f"band local={local} gap={gap}")
+    assert_rows_equal(got0, o["final"], what=f"no band local={local} gap={gap}")
+
+
 @pytest.mark.parametrize("form", ["direct", "bitmap"])
 @pytest.mark.parametrize("lmer", [12, 15, 17])
 def test_index_forms(form, lmer, monkeypatch):
