@@ -13,8 +13,10 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -23,8 +25,11 @@
 
 namespace clast {
 
+// One runtime per (device, host thread): searches issued from different
+// threads run on their own streams and workspaces and overlap on the GPU.
+// Runtimes live until the process ends.
 static std::mutex g_rt_mu;
-static std::map<int, Runtime*> g_rt;
+static std::map<std::pair<int, std::thread::id>, Runtime*> g_rt;
 
 PinnedPool& pinned_pool() {
   static PinnedPool* pool = new PinnedPool();
@@ -34,11 +39,14 @@ PinnedPool& pinned_pool() {
 Runtime& runtime() {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
+  thread_local std::map<int, Runtime*> mine;  // this thread's, without the lock
+  auto hit = mine.find(dev);
+  if (hit != mine.end()) return *hit->second;
   std::lock_guard<std::mutex> lk(g_rt_mu);
-  auto it = g_rt.find(dev);
-  if (it != g_rt.end()) return *it->second;
-  Runtime* rt = new Runtime(dev);
-  g_rt[dev] = rt;
+  const auto key = std::make_pair(dev, std::this_thread::get_id());
+  auto it = g_rt.find(key);
+  Runtime* rt = it != g_rt.end() ? it->second : (g_rt[key] = new Runtime(dev));
+  mine[dev] = rt;
   return *rt;
 }
 

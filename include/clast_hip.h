@@ -94,7 +94,10 @@ typedef struct clast_query_s* clast_query;
 typedef struct clast_hits_s* clast_hits;
 
 /* cudaSetDevice(setting.getDeviceID()) (main.cu:30). Optional: the first
- * call on a thread without it uses device 0. */
+ * call on a thread without it uses device 0.  Each host thread has its own
+ * streams and scratch on each device (kept until the process ends), so
+ * searches issued from different threads run concurrently; indexes and
+ * query sets may be shared between threads once created. */
 int clast_init(int32_t device);
 const char* clast_last_error(void);
 /* Library version string. */
