@@ -155,7 +155,7 @@ def serve_bench(args) -> int:
     index_s = time.perf_counter() - t1
     n_twin = sum(len(w) for _, w in tchunks)
     log(f"index over {n_twin} target windows built in {index_s:.2f} s")
-    srv = serve.Server(p, ix, q_vram=2e6, max_positions=args.serve_budget)
+    srv = serve.Server(p, ix, q_vram=2e6, max_positions=args.serve_budget, workers=args.serve_workers)
     reqs = serve.poisson_requests(r.bases, r.offsets, rate=args.rate, seed=7, min_reads=1, max_reads=10_000)
     srv.search_alone(serve.Request(-1, reqs[0].bases, reqs[0].offsets))  # warm up
     log(f"{len(reqs)} requests at {args.rate} req/s")
@@ -192,6 +192,7 @@ def main() -> int:
                          "traffic counters) and exit")
     ap.add_argument("--serve", action="store_true", help="run the serving loop instead (Poisson request stream)")
     ap.add_argument("--serve-reads", type=int, default=100_000)
+    ap.add_argument("--serve-workers", type=int, default=2, help="device threads draining the batch queue (--serve)")
     ap.add_argument("--rate", type=float, default=50.0, help="requests per second (--serve)")
     ap.add_argument("--serve-budget", type=int, default=64_000_000,
                     help="query k-mer positions admitted per launch chain (--serve)")

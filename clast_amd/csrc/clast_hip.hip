@@ -810,15 +810,24 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
               ix.bmax, p.lmer, static_cast<uint32_t>(4 * std::max(rt.compute_units(), 1)), ot, oq, thl, qhl, om,
               osc};
     mark(EV_DP0);
-    // the long tasks run beside the others (side stream, raised wave priority)
-    if (band) {
+    // the long tasks (raised wave priority) and the int32 tasks run on the
+    // side stream beside the 16-bit ones, so the three kernels' tails overlap
+    if (packed) {
       rt.fork(0);
-      launch_dp_band(p.gap, p.local != 0, da, rt.side_stream());
+      hipStream_t ss = rt.side_stream();
+      if (band) launch_dp_band(p.gap, false, da, ss);
+      launch_dp_global(p.gap, da, ss);
+      launch_dp16(p.gap, da, st);
+      rt.join(0);
+    } else {
+      if (band) {
+        rt.fork(0);
+        launch_dp_band(p.gap, p.local != 0, da, rt.side_stream());
+      }
+      if (p.local) launch_dp_local(p.gap, da, st);
+      else launch_dp_global(p.gap, da, st);
+      if (band) rt.join(0);
     }
-    if (packed) launch_dp16(p.gap, da, st);
-    if (p.local) launch_dp_local(p.gap, da, st);
-    else launch_dp_global(p.gap, da, st);
-    if (band) rt.join(0);
     mark(EV_DP1);
     rt.check_launch("k_dp");
     k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);

@@ -1345,6 +1345,13 @@ __device__ __forceinline__ uint32_t pk_less(s16x2 a, s16x2 b) {
 }
 // a in the halves selected by mask m, b elsewhere
 __device__ __forceinline__ uint32_t pk_pick(uint32_t m, uint32_t a, uint32_t b) { return b ^ ((a ^ b) & m); }
+// 4 m + z per half in one instruction (the compiler splits a multiply by 4
+// into a shift and an add); halves wrap independently
+__device__ __forceinline__ s16x2 pk_mad4(s16x2 m, s16x2 z) {
+  uint32_t r;
+  asm("v_pk_mad_u16 %0, %1, 4, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(pk_bits(m)), "v"(pk_bits(z)));
+  return pk_val(r);
+}
 
 // 4-bit class arrays read by k_dp16: nibble k of dword d is slot 8 d + k
 // of the byte class array (padded to whole 16-byte blocks, so whole 8-byte
@@ -1461,7 +1468,7 @@ __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s
     const s16x2 h = __builtin_elementwise_max(Z[b - 1], hp + pk_splat(kGapExt));
     const u16x2 x = __builtin_bit_cast(u16x2, T[b - 1] ^ Q);
     const s16x2 mc = __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(u16x2{1, 1}, x));
-    const s16x2 pt = Z[b] + (mc << 2) + pk_splat(kMatch - kMismatch);
+    const s16x2 pt = pk_mad4(mc, Z[b]) + pk_splat(kMatch - kMismatch);
     const s16x2 vh = __builtin_elementwise_max(v, h);
     const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
     const uint32_t nm = pk_pick(pk_less(vh, pt), pk_bits(M[b] + mc), ngap);

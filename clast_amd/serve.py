@@ -12,8 +12,10 @@ queries is per (target window, query window) unit and every request keeps its
 own windows, each request's rows are identical alone and in the mix.
 
 Host preparation of the next batch (list building, windows) overlaps the
-device search of the current one: a preparation thread feeds a queue and the
-device thread drains it.  ctypes releases the GIL during native calls.
+device search of the current one: a preparation thread feeds a queue and
+`workers` device threads drain it, each with its own library streams, so
+that many batches are in flight on the GPU at once.  ctypes releases the GIL
+during native calls.
 """
 from __future__ import annotations
 
@@ -54,11 +56,12 @@ class Server:
     """Continuous-batching search over a resident index."""
 
     def __init__(self, params: native.Params, index: native.Index, q_vram: float = 2e6,
-                 max_positions: int = 64_000_000):
+                 max_positions: int = 64_000_000, workers: int = 2):
         self.p = params
         self.index = index
         self.q_vram = q_vram
         self.max_positions = max_positions
+        self.workers = max(1, workers)
 
     def positions(self, r: Request) -> int:
         ln = np.diff(r.offsets)
@@ -104,7 +107,7 @@ class Server:
         requests = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
-        ready: queue.Queue = queue.Queue(maxsize=2)
+        ready: queue.Queue = queue.Queue(maxsize=self.workers + 1)
         pending = list(requests)
         lock = threading.Lock()
 
@@ -130,25 +133,40 @@ class Server:
 
         th = threading.Thread(target=producer, daemon=True)
         th.start()
-        batches = 0
-        while True:
-            prep = ready.get()
-            if prep is None:
-                break
-            self.search(prep)
-            t = now()
-            with lock:
-                for r in prep.requests:
-                    r.done = t
-            batches += 1
+        batches = [0]
+        errors = []
+
+        def worker():
+            while True:
+                prep = ready.get()
+                if prep is None:
+                    ready.put(None)  # for the other workers
+                    return
+                try:
+                    self.search(prep)
+                except Exception as e:  # noqa: BLE001 - reported after the run
+                    errors.append(e)
+                t = now()
+                with lock:
+                    for r in prep.requests:
+                        r.done = t
+                    batches[0] += 1
+
+        ws = [threading.Thread(target=worker, daemon=True) for _ in range(self.workers)]
+        for w in ws:
+            w.start()
+        for w in ws:
+            w.join()
         th.join()
+        if errors:
+            raise errors[0]
         wall = now()
         lat = np.array([(r.done - r.arrival) * 1e3 for r in requests])
         mbp = sum(r.n_bases for r in requests) / 1e6
-        return {"requests": len(requests), "batches": batches, "wall_s": round(wall, 3),
+        return {"requests": len(requests), "batches": batches[0], "workers": self.workers, "wall_s": round(wall, 3),
                 "query_mbp": round(mbp, 3), "mbp_per_s": round(mbp / wall, 3),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "mean_batch": round(len(requests) / max(batches, 1), 2)}
+                "mean_batch": round(len(requests) / max(batches[0], 1), 2)}
 
 
 def poisson_requests(reads_bases: np.ndarray, reads_offsets: np.ndarray, rate: float, seed: int,
