@@ -751,8 +751,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
     rt.read_scalars(tot32 + 3, &nhit, 1);
     if (nhit)
-      k_hits_compact<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(q.nseq, soff, hcnt, hbase, ss, sgs, stt, sq, hs,
-                                                                   hgs, ht, hq);
+      k_hits_compact<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, q.nseq, soff, hbase, ss, sgs, stt, sq, hs, hgs,
+                                                                 ht, hq);
     rt.check_launch("k_hits_compact");
   }
   if (keep_stages) {

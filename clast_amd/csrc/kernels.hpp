@@ -998,10 +998,13 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __re
   hcnt[s] = h1 - h0;
 }
 
-// Dense hits in strand order: strand s's hcnt[s] hits move from the start of
-// its seed range (soff[s]) to hbase[s] (exclusive scan of hcnt).
-static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, const uint32_t* __restrict__ soff,
-                                                                const uint32_t* __restrict__ hcnt,
+// Dense hits in strand order: strand s's hits move from the start of its
+// seed range (soff[s]) to hbase[s] (exclusive scan of the per-strand hit
+// counts).  One thread per dense hit; its strand is the last one whose base
+// is at most the hit's index (strands without hits share their
+// successor's base, so the last such strand is the one that has the hit).
+static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nhit, uint32_t nseq,
+                                                                const uint32_t* __restrict__ soff,
                                                                 const uint32_t* __restrict__ hbase,
                                                                 const int32_t* __restrict__ hs,
                                                                 const int32_t* __restrict__ hgs,
@@ -1009,18 +1012,14 @@ static __global__ __launch_bounds__(kBlock) void k_hits_compact(uint32_t nseq, c
                                                                 const int32_t* __restrict__ hq,
                                                                 int32_t* __restrict__ ds, int32_t* __restrict__ dgs,
                                                                 int32_t* __restrict__ dt, int32_t* __restrict__ dq) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= nseq) return;
-  const uint32_t n = hcnt[s];
-  if (!n) return;
-  const uint32_t src = soff[s];
-  const uint32_t dst = hbase[s];
-  for (uint32_t j = 0; j < n; ++j) {
-    ds[dst + j] = hs[src + j];
-    dgs[dst + j] = hgs[src + j];
-    dt[dst + j] = ht[src + j];
-    dq[dst + j] = hq[src + j];
-  }
+  const uint32_t d = blockIdx.x * kBlock + threadIdx.x;
+  if (d >= nhit) return;
+  const uint32_t s = upper_bound_u(hbase, 0u, nseq, d) - 1u;
+  const uint32_t src = soff[s] + (d - hbase[s]);
+  ds[d] = hs[src];
+  dgs[d] = hgs[src];
+  dt[d] = ht[src];
+  dq[d] = hq[src];
 }
 
 // ---------------------------------------------------------------- banded DP
