@@ -413,48 +413,43 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
 
 // ---------------------------------------------------------------- query
 
-void build_query(const clast_params& p, const clast_window* wins, int nwin, clast_query_s& q) {
-  Runtime& rt = runtime();
-  std::lock_guard<std::mutex> lk(rt.mutex());
-  hipStream_t st = rt.stream();
-  q.device = rt.device();
-  q.lmer = p.lmer;
-  q.nwin = nwin;
-  std::vector<uint64_t> win_slot(nwin + 1, 0);
-  std::vector<uint32_t> qslot, pos_off(1, 0);
+// Per-strand tables of a query set as they are built (both builders).
+struct QueryLayout {
+  std::vector<uint64_t> win_slot{0};  // window w's slots: [win_slot[w], win_slot[w + 1])
+  std::vector<uint32_t> qslot, pos_off{0};
   uint64_t npos = 0;
-  for (int w = 0; w < nwin; ++w) {
-    check_window(wins[w]);
-    if (wins[w].n_seq % 2) throw ClastError(CLAST_ERR_ARG, "a query window holds '+'/'-' pairs (even n_seq)");
-    win_slot[w + 1] = win_slot[w] + static_cast<uint64_t>(wins[w].n_slots);
-    if (win_slot[w + 1] >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
-    for (int k = 0; k < wins[w].n_seq; ++k) {
-      const int len = wins[w].lengths[k];
-      qslot.push_back(static_cast<uint32_t>(win_slot[w] + wins[w].gateway[k]));
-      q.h_qlen.push_back(len);
-      q.h_qref.push_back(wins[w].id_begin + k);
-      q.h_qwin.push_back(w);
-      q.max_qlen = std::max(q.max_qlen, len);
-      npos += len >= p.lmer ? static_cast<uint64_t>(len - p.lmer + 1) : 0ull;
-      if (npos >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 k-mer positions");
-      pos_off.push_back(static_cast<uint32_t>(npos));
-    }
-    q.h_id_begin.push_back(wins[w].id_begin);
-  }
-  q.nseq = static_cast<uint32_t>(qslot.size());
-  q.npos = static_cast<uint32_t>(npos);
+};
+
+static void query_add_strand(const clast_params& p, clast_query_s& q, QueryLayout& L, uint64_t slot, int len,
+                             int ref, int w) {
+  if (slot >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
+  L.qslot.push_back(static_cast<uint32_t>(slot));
+  q.h_qlen.push_back(len);
+  q.h_qref.push_back(ref);
+  q.h_qwin.push_back(w);
+  q.max_qlen = std::max(q.max_qlen, len);
+  L.npos += len >= p.lmer ? static_cast<uint64_t>(len - p.lmer + 1) : 0ull;
+  if (L.npos >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 k-mer positions");
+  L.pos_off.push_back(static_cast<uint32_t>(L.npos));
+}
+
+static void query_alloc_bases(clast_query_s& q, const QueryLayout& L) {
+  const uint64_t slots = L.win_slot.back();
+  if (slots >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
+  q.qbase.alloc(padded_blocks(slots) * 16);
+  q.bmax = static_cast<int32_t>(padded_blocks(slots) - 1);
+}
+
+// The tables after q.qbase holds class codes: strands, k-mer positions,
+// 4-bit copy, clean flags.
+static void query_finish(Runtime& rt, clast_query_s& q, const QueryLayout& L) {
+  hipStream_t st = rt.stream();
+  q.nseq = static_cast<uint32_t>(L.qslot.size());
+  q.npos = static_cast<uint32_t>(L.npos);
   q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
-  q.qbase.alloc(padded_blocks(win_slot[nwin]) * 16);
-  q.bmax = static_cast<int32_t>(padded_blocks(win_slot[nwin]) - 1);
-  for (int w = 0; w < nwin; ++w)
-    if (wins[w].n_slots)
-      hip_check(hipMemcpyAsync(q.qbase.ptr + win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice, st),
-                "upload query bases");
-  if (win_slot[nwin])
-    k_classify<<<grid_for(win_slot[nwin], kBlock), kBlock, 0, st>>>(q.qbase.ptr, win_slot[nwin]);
-  q.qslot.upload(qslot.data(), qslot.size(), st);
-  q.pos_off.upload(pos_off.data(), pos_off.size(), st);
-  q.pos_strand.alloc(std::max<uint64_t>(npos, 1));
+  q.qslot.upload(L.qslot.data(), L.qslot.size(), st);
+  q.pos_off.upload(L.pos_off.data(), L.pos_off.size(), st);
+  q.pos_strand.alloc(std::max<uint64_t>(L.npos, 1));
   if (q.nseq) k_pos_strand<<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.pos_off.ptr, q.nseq, q.pos_strand.ptr);
   q.qlen.upload(q.h_qlen.data(), q.h_qlen.size(), st);
   q.qref.upload(q.h_qref.data(), q.h_qref.size(), st);
@@ -469,7 +464,95 @@ void build_query(const clast_params& p, const clast_window* wins, int nwin, clas
     k_clean_flags<uint32_t><<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.qbase.ptr, q.qslot.ptr, q.qlen.ptr,
                                                                               q.nseq, q.qclean.ptr);
   q.smin.alloc(std::max<uint64_t>(q.nseq, 1));
+  rt.check_launch("query tables");
   rt.sync();
+}
+
+void build_query(const clast_params& p, const clast_window* wins, int nwin, clast_query_s& q) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mutex());
+  hipStream_t st = rt.stream();
+  q.device = rt.device();
+  q.lmer = p.lmer;
+  q.nwin = nwin;
+  QueryLayout L;
+  for (int w = 0; w < nwin; ++w) {
+    check_window(wins[w]);
+    if (wins[w].n_seq % 2) throw ClastError(CLAST_ERR_ARG, "a query window holds '+'/'-' pairs (even n_seq)");
+    const uint64_t w0 = L.win_slot.back();
+    L.win_slot.push_back(w0 + static_cast<uint64_t>(wins[w].n_slots));
+    for (int k = 0; k < wins[w].n_seq; ++k)
+      query_add_strand(p, q, L, w0 + static_cast<uint64_t>(wins[w].gateway[k]), wins[w].lengths[k],
+                       wins[w].id_begin + k, w);
+    q.h_id_begin.push_back(wins[w].id_begin);
+  }
+  query_alloc_bases(q, L);
+  for (int w = 0; w < nwin; ++w)
+    if (wins[w].n_slots)
+      hip_check(hipMemcpyAsync(q.qbase.ptr + L.win_slot[w], wins[w].bases, wins[w].n_slots, hipMemcpyHostToDevice,
+                               st),
+                "upload query bases");
+  if (L.win_slot.back())
+    k_classify<<<grid_for(L.win_slot.back(), kBlock), kBlock, 0, st>>>(q.qbase.ptr, L.win_slot.back());
+  query_finish(rt, q, L);
+}
+
+// Query windows from the reads as given: window w holds reads
+// [win[2w], win[2w + 1]) (consecutive windows), each read as its '+' and
+// '-' strands (list IDs id_begin[w] + 2 (k - win[2w]) + {0, 1}); the strands
+// and joints are built on the device (k_encode_reads) from one upload of the
+// reads' bytes.
+void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_t* offsets, const int32_t* win,
+                       const int32_t* id_begin, int nwin, clast_query_s& q) {
+  Runtime& rt = runtime();
+  std::lock_guard<std::mutex> lk(rt.mutex());
+  hipStream_t st = rt.stream();
+  q.device = rt.device();
+  q.lmer = p.lmer;
+  q.nwin = nwin;
+  const int32_t r0 = win[0], r1 = win[2 * nwin - 1];
+  if (r0 < 0 || r1 < r0) throw ClastError(CLAST_ERR_ARG, "bad read windows");
+  for (int w = 0; w < nwin; ++w) {
+    if (win[2 * w] > win[2 * w + 1] || (w > 0 && win[2 * w] != win[2 * w - 1]))
+      throw ClastError(CLAST_ERR_ARG, "read windows must be consecutive ranges in order");
+  }
+  const int64_t joint = std::max(p.lmer - 1, 0);
+  QueryLayout L;
+  std::vector<uint64_t> rslot;  // '+' strand slot of each read
+  std::vector<int64_t> roff;    // byte offsets of the reads in the upload
+  rslot.reserve(static_cast<size_t>(r1 - r0));
+  roff.reserve(static_cast<size_t>(r1 - r0) + 1);
+  const int64_t b0 = offsets[r0];
+  uint64_t slot = 0;
+  for (int w = 0; w < nwin; ++w) {
+    for (int32_t k = win[2 * w]; k < win[2 * w + 1]; ++k) {
+      const int64_t len = offsets[k + 1] - offsets[k];
+      if (len < 0 || len > INT32_MAX - joint) throw ClastError(CLAST_ERR_ARG, "bad read offsets");
+      roff.push_back(offsets[k] - b0);
+      rslot.push_back(slot);
+      const int local = 2 * (k - win[2 * w]);
+      query_add_strand(p, q, L, slot, static_cast<int>(len), id_begin[w] + local, w);
+      query_add_strand(p, q, L, slot + static_cast<uint64_t>(len + joint), static_cast<int>(len),
+                       id_begin[w] + local + 1, w);
+      slot += 2 * static_cast<uint64_t>(len + joint);
+    }
+    L.win_slot.push_back(slot);
+    q.h_id_begin.push_back(id_begin[w]);
+  }
+  roff.push_back(offsets[r1] - b0);
+  query_alloc_bases(q, L);
+  const uint32_t nreads = static_cast<uint32_t>(r1 - r0);
+  DevArray<uint8_t> dreads;
+  DevArray<int64_t> droff;
+  DevArray<uint64_t> dslot;
+  if (nreads) {
+    dreads.upload(bases + b0, static_cast<uint64_t>(roff.back()), st);
+    droff.upload(roff.data(), roff.size(), st);
+    dslot.upload(rslot.data(), rslot.size(), st);
+    k_encode_reads<<<grid_for(nreads, kBlock / 64), kBlock, 0, st>>>(dreads.ptr, droff.ptr, nreads, dslot.ptr, p.lmer,
+                                                                     q.qbase.ptr);
+  }
+  query_finish(rt, q, L);  // ends with a sync: the uploads are done before they are freed
 }
 
 void ensure_smin(Runtime& rt, const clast_params& p, clast_query_s& q) {
@@ -1089,6 +1172,18 @@ int clast_query_create_multi(const clast_params* p, const clast_window* q, int32
     if (!q || n < 1 || !out) throw ClastError(CLAST_ERR_ARG, "bad query windows");
     auto qq = std::make_unique<clast_query_s>();
     build_query(*p, q, n, *qq);
+    *out = qq.release();
+  });
+}
+
+int clast_query_create_reads(const clast_params* p, const uint8_t* bases, const int64_t* offsets, const int32_t* win,
+                             const int32_t* id_begin, int32_t n_win, clast_query* out) {
+  return guard([&] {
+    check_params(p);
+    if (!bases || !offsets || !win || !id_begin || n_win < 1 || !out)
+      throw ClastError(CLAST_ERR_ARG, "bad read windows");
+    auto qq = std::make_unique<clast_query_s>();
+    build_query_reads(*p, bases, offsets, win, id_begin, n_win, *qq);
     *out = qq.release();
   });
 }

@@ -52,6 +52,40 @@ static __global__ __launch_bounds__(kBlock) void k_classify(uint8_t* __restrict_
   if (i < n) p[i] = static_cast<uint8_t>(dp_class(p[i]));
 }
 
+// Query strands built on the device from the reads as given
+// (addSequence, utilAddSequence.cu:6-47, as CHostSeqList_query uses it,
+// CHostSeqList_query.cu:11-54): read k becomes its '+' strand at slot
+// slot[k] and its reverse complement ('-': A<->T, C<->G, any other byte N)
+// right after, each followed by lMer-1 joint bases (the strand's own first
+// bases, N past a short strand's end); written as class codes.  One
+// wavefront per read; roff holds the reads' byte offsets in `reads`.
+static __global__ __launch_bounds__(kBlock) void k_encode_reads(const uint8_t* __restrict__ reads,
+                                                                const int64_t* __restrict__ roff, uint32_t nreads,
+                                                                const uint64_t* __restrict__ slot, int lmer,
+                                                                uint8_t* __restrict__ qbase) {
+  const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (k >= nreads) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint8_t* r = reads + roff[k];
+  const int64_t len = roff[k + 1] - roff[k];
+  const int64_t span = len + max(lmer - 1, 0);  // slots per strand
+  uint8_t* plus = qbase + slot[k];
+  uint8_t* minus = plus + span;
+  for (int64_t i = lane; i < span; i += 64) {
+    // the strand position whose base slot i holds (joint slots repeat the
+    // first bases); none past a short strand's end
+    const int64_t src = i < len ? i : i - len;
+    const bool has = src < len;
+    const uint32_t cp = has ? dp_class(r[src]) : kClsOther;
+    // '-' base at src is the complement of read base len - 1 - src; the
+    // complement keeps A C G T (as T G C A) and turns every other byte into N
+    const uint32_t cr = has ? dp_class(r[len - 1 - src]) : kClsOther;
+    const uint32_t cm = cr < 4u ? 3u - cr : kClsOther;
+    plus[i] = static_cast<uint8_t>(cp);
+    minus[i] = static_cast<uint8_t>(cm);
+  }
+}
+
 // clean[k] = 1 when the bytes [slot[k], slot[k] + len[k]) of a class array
 // hold no star or X class (scores and match counts of such a sequence's
 // cells stay within 16 bits; see k_dp16).  One wavefront per sequence, four

@@ -85,6 +85,7 @@ def hip() -> C.CDLL:
             "clast_index_destroy": (C.c_int, [vp]),
             "clast_query_create": (C.c_int, [pp, wp, C.POINTER(vp)]),
             "clast_query_create_multi": (C.c_int, [pp, wp, i32, C.POINTER(vp)]),
+            "clast_query_create_reads": (C.c_int, [pp, vp, vp, vp, vp, i32, C.POINTER(vp)]),
             "clast_query_destroy": (C.c_int, [vp]),
             "clast_search": (C.c_int, [pp, vp, vp, i32, i32, C.POINTER(vp)]),
             "clast_search_batch": (C.c_int, [pp, vp, vp, i32, C.POINTER(vp)]),
@@ -203,6 +204,18 @@ def windows(gateway: np.ndarray, vram_slots: float, query: bool) -> list[tuple[i
     return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(k)]
 
 
+def read_gateway(lengths: np.ndarray, lmer: int) -> np.ndarray:
+    """Slot gateway of the '+'/'-' strands of reads of the given lengths
+    (each strand followed by lMer-1 joint slots), as a query host list
+    would hold it: 2 n + 1 entries."""
+    span = np.repeat(np.asarray(lengths, dtype=np.int64) + max(lmer - 1, 0), 2)
+    g = np.zeros(len(span) + 1, dtype=np.int64)
+    np.cumsum(span, out=g[1:])
+    if g[-1] >= 2**31:
+        raise ValueError("query set over 2^31 slots")
+    return g.astype(np.int32)
+
+
 def chunks(lengths: np.ndarray, avail: int) -> tuple[list[tuple[int, int]], np.ndarray]:
     """The FASTA loader's chunk boundaries for records of these lengths."""
     ln = np.ascontiguousarray(lengths, dtype=np.int64)
@@ -296,6 +309,26 @@ class Query:
                "clast_query_create_multi")
         for lst, _ in lists_and_windows:
             lst.release_windows()
+        self.h = h
+        return self
+
+    @classmethod
+    def from_reads(cls, params: Params, bases: np.ndarray, offsets: np.ndarray, wins: list[tuple[int, int]],
+                   id_begin: list[int] | None = None) -> "Query":
+        """Query windows built on the device from the reads as given
+        (clast_query_create_reads): wins are consecutive read (pair) ranges;
+        window w's first strand has list ID id_begin[w] (default 2 x its
+        first read)."""
+        self = cls.__new__(cls)
+        self.wins = list(wins)
+        b = np.ascontiguousarray(bases, dtype=np.uint8)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        w = np.ascontiguousarray(np.asarray(wins, dtype=np.int32).reshape(-1))
+        ids = np.ascontiguousarray(np.asarray(id_begin if id_begin is not None else [2 * a for a, _ in wins],
+                                              dtype=np.int32))
+        h = C.c_void_p()
+        _check(hip().clast_query_create_reads(C.byref(params.c()), _ptr(b), _ptr(off), _ptr(w), _ptr(ids), len(wins),
+                                              C.byref(h)), "clast_query_create_reads")
         self.h = h
         return self
 

@@ -1,9 +1,11 @@
 """Serving loop: a resident target index in HBM and continuous batching of
 read requests.
 
-Each request is a set of reads (a query RAM chunk of its own).  Its host list
-and query windows are built exactly as the driver would build them for that
-request alone (qVRAM windows over '+'/'-' pairs).  Requests that have
+Each request is a set of reads (a query RAM chunk of its own).  Its query
+windows are the ones the driver would build for that request alone (qVRAM
+windows over '+'/'-' pairs, computed from the read lengths); the strands
+themselves are built on the device from one upload of the batch's reads
+(clast_query_create_reads).  Requests that have
 arrived while the device was busy are admitted in arrival order up to a
 per-launch budget of query k-mer positions and coalesced into one query set
 (one window list, request after request), searched in one launch chain, and
@@ -47,8 +49,7 @@ class Request:
 @dataclass
 class Prepared:
     requests: list
-    lists: list              # one HostList per request
-    wins: list               # query windows (pairs) per request
+    wins: list               # query windows (read pair ranges) per request
     positions: int
 
 
@@ -68,14 +69,12 @@ class Server:
         return int(2 * np.clip(ln - self.p.lmer + 1, 0, None).sum())
 
     def prepare(self, reqs: list) -> Prepared:
-        lists, wins, npos = [], [], 0
+        wins, npos = [], 0
         for r in reqs:
-            off = (r.offsets - r.offsets[0]).astype(np.int64)
-            lst = native.build_list(r.bases[r.offsets[0]:r.offsets[-1]], off, self.p.lmer, True)
-            lists.append(lst)
-            wins.append(native.windows(lst.gateway, self.q_vram, True))
+            g = native.read_gateway(np.diff(r.offsets), self.p.lmer)
+            wins.append(native.windows(g, self.q_vram, True))
             npos += self.positions(r)
-        return Prepared(reqs, lists, wins, npos)
+        return Prepared(reqs, wins, npos)
 
     def search(self, prep: Prepared) -> None:
         """One launch chain for the whole batch; rows split per request."""
@@ -84,7 +83,17 @@ class Server:
             for r in prep.requests:
                 r.rows = {}
             return
-        q = native.Query.merged(self.p, list(zip(prep.lists, prep.wins)))
+        # one upload of the batch's reads; each request's windows shifted to
+        # its reads' place in the batch, list IDs numbered per request
+        bases = np.concatenate([r.bases[r.offsets[0]:r.offsets[-1]] for r in prep.requests])
+        offs = [np.zeros(1, np.int64)]
+        bwins, ids, first = [], [], 0
+        for r, wl in zip(prep.requests, prep.wins):
+            offs.append((r.offsets[1:] - r.offsets[0]).astype(np.int64) + offs[-1][-1])
+            bwins += [(first + a, first + b) for a, b in wl]
+            ids += [2 * a for a, _ in wl]
+            first += len(r.offsets) - 1
+        q = native.Query.from_reads(self.p, bases, np.concatenate(offs), bwins, ids)
         try:
             hits = native.search_batch(self.p, self.index, q)
             rows = hits.arrays()

@@ -121,6 +121,18 @@ int clast_index_destroy(clast_index idx);
 int clast_query_create(const clast_params* p, const clast_window* query, clast_query* out);
 int clast_query_create_multi(const clast_params* p, const clast_window* queries, int32_t n_windows,
                              clast_query* out);
+/* The reads as given, encoded on the device (replaces the host side of
+ * CHostSeqList_query, CHostSeqList_query.cu:11-54 with addSequence,
+ * utilAddSequence.cu:6-47, and the upload of CDeviceSeqList_query,
+ * CDeviceSeqList_query.cuh:14-18): read k (bytes [offsets[k],
+ * offsets[k+1]) of `bases`) becomes its '+' strand and its reverse
+ * complement ('-': A<->T, C<->G, any other byte N), each followed by lMer-1
+ * joint bases (the strand's first bases, N past a short strand's end).
+ * Window w holds reads [win[2w], win[2w+1]); windows are consecutive and in
+ * order; the list ID of window w's first strand is id_begin[w].  Equal to
+ * clast_query_create_multi over the same strands built on the host. */
+int clast_query_create_reads(const clast_params* p, const uint8_t* bases, const int64_t* offsets, const int32_t* win,
+                             const int32_t* id_begin, int32_t n_win, clast_query* out);
 int clast_query_destroy(clast_query q);
 
 /* CDeviceHitList ctor (CDeviceHitList.cu:228-423): the whole seed -> filter

@@ -298,6 +298,47 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     assert_rows_equal(got0, o["final"], what=f"no band local={local} gap={gap}")
 
 
+@pytest.mark.parametrize("lmer", [12, 15])
+def test_query_from_reads(lmer):
+    # Query strands and joints built on the device from the reads as given
+    # equal the host-built list (reverse complement: A<->T, C<->G, any other
+    # byte N; joints: the strand's first bases, N past a short strand): the
+    # same rows as the host path and the oracle, over several windows, with
+    # lowercase, X, star and N letters, reads shorter than lMer and an empty
+    # read.
+    t, r = workload(n_t=4, t_len=20_000, n_r=220, len_mode=gen.LEN_LOGUNIFORM, read_len=60, len_b=900, seed=17)
+    seqs = [r.seq(i) for i in range(len(r))]
+    seqs[3] = seqs[3].lower()
+    seqs[5] = seqs[5][:40] + b"X" + seqs[5][41:]
+    seqs[7] = seqs[7][:30] + b"NNNNNNNN" + seqs[7][38:]
+    seqs[9] = seqs[9][:50] + b"EJ" + seqs[9][52:]
+    seqs += [b"ACG", b"", t.seq(1)[200:210], t.seq(2)[300:600].lower().upper()]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    p = native.Params(total_db_size=db_size(t), lmer=lmer)
+    tl = native.build_list(t.bases, t.offsets, lmer, False)
+    ql = native.build_list(bases, off, lmer, True)
+    twins = native.windows(tl.gateway, 40_000, query=False)
+    qwins = native.windows(ql.gateway, 30_000, query=True)
+    g = native.read_gateway(np.diff(off), lmer)
+    np.testing.assert_array_equal(g, ql.gateway)
+    assert native.windows(g, 30_000, query=True) == qwins
+    assert len(qwins) > 1
+    ix = native.Index(p, tl, twins)
+    q_host = native.Query(p, ql, qwins)
+    q_dev = native.Query.from_reads(p, bases, off, qwins)
+    print(f"query from reads lmer={lmer}: {len(qwins)} windows", flush=True)
+    got_host = native.search_batch(p, ix, q_host).arrays()
+    got_dev = native.search_batch(p, ix, q_dev).arrays()
+    for q in (q_host, q_dev):
+        q.close()
+    ix.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(got_dev, got_host, what=f"device vs host encoding lmer {lmer}")
+    assert_rows_equal(got_dev, o["final"], what=f"device encoding vs oracle lmer {lmer}")
+
+
 @pytest.mark.parametrize("form", ["direct", "bitmap"])
 @pytest.mark.parametrize("lmer", [12, 15, 17])
 def test_index_forms(form, lmer, monkeypatch):
