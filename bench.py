@@ -207,8 +207,8 @@ def main() -> int:
         log(f"copied {args.calibrate_copy} MiB")
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (the search shards by query "
-              "with no exchange; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one MI355X: there is no multi-GPU path (the search would shard by query "
+              "chunk with no exchange during compute).", file=sys.stderr)
         return 2
 
     if args.serve:

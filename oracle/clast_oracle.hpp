@@ -3,8 +3,8 @@
 // A host C++ restatement of the reference's search path, written from the
 // reference source read as text.  Each function cites the reference file:line
 // it follows.  Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load it, and only as the checker: the product
-// (libclast_hip + the clast driver) never links or calls it.
+// cpu_baseline leg may load it, and only to check results against: the
+// product (libclast_hip + the clast driver) never links or calls it.
 //
 // Conventions for behaviour the reference leaves undefined (parity unpinned,
 // fixed by this oracle's documented choice):

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.
+bench.py's cpu_baseline leg, only to check results against (and to time the
+CPU baseline), never as part of the product path.
 """
 from __future__ import annotations
 
