@@ -1,6 +1,7 @@
 """Host-side code without a GPU: the C-ABI library loads and exports every
-declared entry point, the host list layout, windows, FASTA chunking and the
-driver's flag handling."""
+declared entry point, the host list layout (and its gateway computed from
+read lengths alone), windows, FASTA chunking and the driver's flag
+handling."""
 from __future__ import annotations
 
 import ctypes
@@ -60,6 +61,21 @@ def test_host_list_layout(query):
     assert lst.base.tobytes() == b
     assert lst.length.tolist() == lens
     assert lst.gateway.tolist() == gw
+
+
+@pytest.mark.parametrize("lmer", [1, 11, 15, 21])
+def test_read_gateway_equals_host_list(lmer):
+    # the strand gateway computed from read lengths alone (device-built query
+    # sets) equals the host-built list's, short and empty reads included
+    rng = np.random.default_rng(lmer)
+    seqs = [bytes(rng.choice(list(b"ACGTN"), int(n))) for n in rng.integers(0, 400, 60)] + [b"", b"AC"]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    lst = native.build_list(bases, off, lmer, True)
+    g = native.read_gateway(np.diff(off), lmer)
+    assert g.tolist() == lst.gateway.tolist()
+    for vram in (300, 5_000, 1e9):
+        assert native.windows(g, vram, True) == native.windows(lst.gateway, vram, True)
 
 
 def py_windows(gateway, vram, query):
