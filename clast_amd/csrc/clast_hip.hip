@@ -436,7 +436,9 @@ static void query_add_strand(const clast_params& p, clast_query_s& q, QueryLayou
 static void query_alloc_bases(clast_query_s& q, const QueryLayout& L) {
   const uint64_t slots = L.win_slot.back();
   if (slots >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
-  q.qbase.alloc(padded_blocks(slots) * 16);
+  // 48 bytes of slack: k-mer lookups read the 9 aligned words that hold
+  // each k-mer (kmer_words)
+  q.qbase.alloc((padded_blocks(slots) + 3) * 16);
   q.bmax = static_cast<int32_t>(padded_blocks(slots) - 1);
 }
 
@@ -1122,6 +1124,18 @@ int clast_index_create_multi(const clast_params* p, const clast_window* t, int32
   });
 }
 
+// Inverse of index_code: the k-mer code of an index code.
+static uint64_t kmer_of_index_code(uint64_t f, int lmer) {
+  if (!(lmer & 1)) return f;
+  const uint64_t hb = 1ull << lmer;
+  const uint64_t c0 = f >> 1;
+  const uint64_t c = (c0 & (hb - 1ull)) | ((c0 & ~(hb - 1ull)) << 1);
+  if (!(f & 1ull)) return c;
+  uint64_t x = c, rc = 0;
+  for (int j = 0; j < lmer; ++j, x >>= 2) rc = (rc << 2) | (3ull - (x & 3ull));
+  return rc;
+}
+
 int64_t clast_index_entry_count(clast_index idx) { return idx ? static_cast<int64_t>(idx->nentries) : -1; }
 
 int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* tID, int32_t* tIdx) {
@@ -1145,11 +1159,18 @@ int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* 
       rt.sync();
       for (uint64_t i = 0; i < n; ++i) k[i] = static_cast<int64_t>(kk[i]);
     }
-    for (uint64_t i = 0; i < n; ++i) {
-      if (key) key[i] = k[i];
-      if (window) window[i] = ix->h_seq_win[ep[i].x];
-      if (tID) tID[i] = ix->h_seq_ref[ep[i].x];
-      if (tIdx) tIdx[i] = ep[i].y;
+    // entries are stored in index-code order; report k-mer codes in k-mer
+    // order (a stable sort keeps each k-mer's run as it is)
+    for (uint64_t i = 0; i < n; ++i) k[i] = static_cast<int64_t>(kmer_of_index_code(static_cast<uint64_t>(k[i]), ix->lmer));
+    std::vector<uint64_t> ord(n);
+    for (uint64_t i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint64_t x, uint64_t y) { return k[x] < k[y]; });
+    for (uint64_t r = 0; r < n; ++r) {
+      const uint64_t i = ord[r];
+      if (key) key[r] = k[i];
+      if (window) window[r] = ix->h_seq_win[ep[i].x];
+      if (tID) tID[r] = ix->h_seq_ref[ep[i].x];
+      if (tIdx) tIdx[r] = ep[i].y;
     }
   });
 }

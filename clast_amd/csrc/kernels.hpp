@@ -148,10 +148,69 @@ __device__ __forceinline__ uint32_t lower_bound_u(const T* a, uint32_t lo, uint3
 
 // ---------------------------------------------------------------- index build
 
+// Index code of a valid k-mer x (2-bit MSB-first code, rc its reverse
+// complement).  For odd lMer a bijection of the 2 lMer-bit codes that puts
+// x and rc side by side: of the two, take c, the one whose middle base is A
+// or C (exactly one is, the complement of A/C being T/G), drop that base's
+// high bit (bit lMer, always 0 in c) and append which of the two x is.  A
+// query strand and its reverse complement then look up the same bitmap word
+// and the same cell line at mirrored positions.  Even lMer keeps the plain
+// code (a k-mer can be its own reverse complement).  Cells, entry order and
+// the repeat cut only need some bijection: entries of one k-mer still form
+// one run, in position order.
+__host__ __device__ __forceinline__ uint64_t index_code(uint64_t x, uint64_t rc, int lmer) {
+  if (!(lmer & 1)) return x;
+  const uint64_t hb = 1ull << lmer;
+  const uint64_t o = (x >> lmer) & 1ull;
+  const uint64_t c = o ? rc : x;
+  return (((c & (hb - 1ull)) | ((c >> 1) & ~(hb - 1ull))) << 1) | o;
+}
+
+// 2-bit codes of the 4 class bytes of a little-endian word, first byte in
+// the top bits (the MSB-first order of SRead2bit.cuh:12-18).
+__device__ __forceinline__ uint32_t pack_codes4(uint32_t x) {
+  const uint32_t y = x & 0x03030303u;
+  return ((y << 6) & 0xC0u) | ((y >> 4) & 0x30u) | ((y >> 14) & 0x0Cu) | (y >> 24);
+}
+// Bit k set when byte k of the word is not A, C, G or T (classes 4..6 have
+// bit 2 set, 0..3 do not).
+__device__ __forceinline__ uint32_t bad_bytes4(uint32_t x) {
+  const uint32_t b = (x >> 2) & 0x01010101u;
+  return (b | (b >> 7) | (b >> 14) | (b >> 21)) & 0xFu;
+}
+
+// K-mer of lmer (1..31) class bytes at byte offset a of base, read as the 9
+// aligned words that hold it (the buffer extends 36 bytes past the last
+// k-mer): its code and reverse-complement code; false if a byte is not
+// A, C, G or T (SRepelHasOddBase.cuh:7-13).
+__device__ __forceinline__ bool kmer_words(const uint8_t* __restrict__ base, uint64_t a, int lmer, uint64_t& key,
+                                           uint64_t& rc) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + (a >> 2);
+  uint32_t x[9];
+#pragma unroll
+  for (int d = 0; d < 9; ++d) x[d] = w[d];
+  uint64_t h = 0, bad = 0;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    h = (h << 8) | pack_codes4(x[d]);
+    bad |= static_cast<uint64_t>(bad_bytes4(x[d])) << (4 * d);
+  }
+  bad |= static_cast<uint64_t>(bad_bytes4(x[8])) << 32;
+  const int o = static_cast<int>(a & 3u);
+  h = (h << (2 * o)) | (pack_codes4(x[8]) >> (8 - 2 * o));  // bases o .. o + 31 of the 36 read
+  key = h >> (64 - 2 * lmer);
+  // reverse the 2-bit groups, then complement (A<->T, C<->G)
+  uint64_t r = __builtin_bitreverse64(key);
+  r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+  const uint64_t mask = (1ull << (2 * lmer)) - 1ull;
+  rc = (r >> (64 - 2 * lmer)) ^ mask;
+  return ((bad >> o) & ((1ull << lmer) - 1ull)) == 0;
+}
+
 // Strided target k-mers (CDeviceHashTable.cu:20-85): element g of window w
-// is position p = (g - win_stride_off[w]) * stride of that window; invalid
-// k-mers (non-ACGT or running past the window end) get `invalid`, which
-// sorts after every valid code.
+// is position p = (g - win_stride_off[w]) * stride of that window; a valid
+// k-mer gets its index code, an invalid one (non-ACGT or running past the
+// window end) `invalid`, which sorts after every valid code.
 static __global__ __launch_bounds__(kBlock) void k_target_keys(const uint8_t* __restrict__ tbase,
                                                         const uint64_t* __restrict__ win_slot_off,
                                                         const uint32_t* __restrict__ win_stride_off, int nwin,
@@ -163,14 +222,16 @@ static __global__ __launch_bounds__(kBlock) void k_target_keys(const uint8_t* __
   const uint64_t p = static_cast<uint64_t>(g - win_stride_off[w]) * stride;
   const uint64_t nw = win_slot_off[w + 1] - win_slot_off[w];
   const uint8_t* b = tbase + win_slot_off[w] + p;
-  uint64_t key = 0;
+  const int top = 2 * (lmer - 1);
+  uint64_t key = 0, rc = 0;
   bool ok = true;
   for (int j = 0; j < lmer; ++j) {
     const uint32_t c = (p + j < nw) ? base_code(b[j]) : 4u;
     ok = ok && c < 4u;
     key = (key << 2) | (c & 3u);
+    rc = (rc >> 2) | (static_cast<uint64_t>(3u - (c & 3u)) << top);
   }
-  keys[g] = ok ? key : invalid;
+  keys[g] = ok ? index_code(key, rc, lmer) : invalid;
   vals[g] = g;
 }
 
@@ -422,45 +483,57 @@ __device__ __forceinline__ void note_unit_first(const SeedArgs& a, int tw, int q
 // filter).  Only positions with qIdx + lMer <= qLen exist here: the
 // query-boundary filter (…deleteSeedsOnSequenceBoundary.cu:19-72) removes
 // every other one.
+//
+// Thread t takes the t-th position of its strand pair (strands 2k, 2k + 1).
+// A pair of equal lengths is walked interleaved: '+' position j, then the
+// '-' position m - 1 - j, which reads the reverse complement of the same
+// bases, so with index codes (index_code) the two lanes look up one bitmap
+// word and one cell line and the wavefront issues a single request for
+// both.  Any other pair is walked in order.
 static __global__ __launch_bounds__(kBlock) void k_seed_count(SeedArgs a, uint4* __restrict__ cellp,
                                                               uint32_t* __restrict__ scnt) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = p < a.npos;
-  const uint32_t s = valid ? __builtin_nontemporal_load(&a.pos_strand[p]) : 0xFFFFFFFFu;
-  uint32_t n = 0;
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = t < a.npos;
+  uint32_t s = 0xFFFFFFFFu, n = 0;
   if (valid) {
-    const int qIdx = static_cast<int>(p - a.pos_off[s]);
-    const uint8_t* b = a.qbase + a.qslot[s] + qIdx;
-    uint64_t key = 0;
-    bool ok = true;
-    for (int j = 0; j < a.lmer; ++j) {
-      const uint32_t c = base_code(b[j]);
-      ok = ok && c < 4u;
-      key = (key << 2) | (c & 3u);
+    s = __builtin_nontemporal_load(&a.pos_strand[t]);
+    uint32_t qIdx = t - a.pos_off[s];
+    const uint32_t s0 = s & ~1u;
+    if (s0 + 1u < a.nseq) {
+      const uint32_t p0 = a.pos_off[s0], p1 = a.pos_off[s0 + 1], p2 = a.pos_off[s0 + 2];
+      if (p1 - p0 == p2 - p1) {
+        const uint32_t u = t - p0, j = u >> 1;
+        s = s0 + (u & 1u);
+        qIdx = (u & 1u) ? p2 - p1 - 1u - j : j;
+      }
     }
+    uint64_t key, rc;
+    const bool ok = kmer_words(a.qbase, static_cast<uint64_t>(a.qslot[s]) + qIdx, a.lmer, key, rc);
     uint32_t lo = 0;
     int2 first = make_int2(0, 0);
-    if (ok) n = lookup(a.lk, key, lo, first);
+    if (ok) n = lookup(a.lk, index_code(key, rc, a.lmer), lo, first);
     u32x4 v;
     v.x = lo;
     v.y = n;
     v.z = static_cast<uint32_t>(first.x);
     v.w = static_cast<uint32_t>(first.y);
-    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(&cellp[p]));  // read again only by the next kernel
+    // read again only by the next kernel
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(&cellp[a.pos_off[s] + qIdx]));
   }
-  // strand totals: positions are strand-ordered, so each strand's lanes of a
-  // wavefront are contiguous; segmented inclusive scan, the segment's last
-  // lane adds the sum
+  // strand totals: among the even lanes, and among the odd ones, each
+  // strand's lanes are contiguous (pairs alternate, other strands run in
+  // order); segmented inclusive scan over each half, whose last lane of a
+  // strand adds the sum
   const int lane = lane_id();
   uint32_t v = n;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
+  for (int d = 2; d < 64; d <<= 1) {
     const uint32_t up = __shfl_up(v, d, 64);
     const uint32_t su = __shfl_up(s, d, 64);
     if (lane >= d && su == s) v += up;
   }
-  const uint32_t sn = __shfl_down(s, 1, 64);
-  if (valid && v && (lane == 63 || sn != s)) atomicAdd(&scnt[s], v);
+  const uint32_t sn = __shfl_down(s, 2, 64);
+  if (valid && v && (lane >= 62 || sn != s)) atomicAdd(&scnt[s], v);
 }
 
 // hasNearPair core (…deleteDuplicateSeeds.cuh:29-38) for seeds of one
