@@ -319,7 +319,8 @@ def main() -> int:
     # 16 B per query k-mer; segments 24 B per raw seed
     # (expansion) + 32 B (sort) + 16 B (isolate/duplicate pass) + 16 B per hit
     # out; DP 14 int32 ops per cell; results 40 B per hit in + 48 B per row out
-    sec = lambda col: float(times[:, col].sum()) / 1e3  # noqa: E731
+    # (a stage that did not run, e.g. no seeds, reports a rate of 0)
+    sec = lambda col: max(float(times[:, col].sum()) / 1e3, 1e-12)  # noqa: E731
     seeds_raw, dp_hits, cells, out_rows = (float(counts[:, c].sum()) for c in range(4))
     seg_bytes = 72.0 * seeds_raw + 16.0 * dp_hits
     res_bytes = 40.0 * dp_hits + 48.0 * out_rows

@@ -339,6 +339,38 @@ def test_query_from_reads(lmer):
     assert_rows_equal(got_dev, o["final"], what=f"device encoding vs oracle lmer {lmer}")
 
 
+@pytest.mark.parametrize("lmer", [12, 15])
+def test_query_strands_as_given(lmer):
+    # Query strands taken as given rather than a read and its reverse
+    # complement: pairs of unequal lengths (the lookup walks them in order)
+    # and equal-length pairs of unrelated reads (walked interleaved, but
+    # mirrored positions are not each other's reverse complement), against
+    # the oracle over the same list.
+    t, r = workload(n_t=4, t_len=20_000, n_r=200, len_mode=gen.LEN_LOGUNIFORM, read_len=60, len_b=600, seed=29)
+    seqs = [r.seq(i) for i in range(len(r))]
+    for i in range(0, 80, 2):
+        n = min(len(seqs[i]), len(seqs[i + 1]))
+        seqs[i], seqs[i + 1] = seqs[i][:n], seqs[i + 1][:n]
+    lens = np.array([len(x) for x in seqs])
+    assert np.sum(lens[0::2] == lens[1::2]) >= 40 and np.sum(lens[0::2] != lens[1::2]) >= 40
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    p = native.Params(total_db_size=db_size(t), lmer=lmer)
+    tl = native.build_list(t.bases, t.offsets, lmer, False)
+    ql = native.build_list(bases, off, lmer, False)
+    twins = [(0, tl.n_seq)]
+    qwins = native.windows(ql.gateway, 20_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    print(f"strands as given lmer={lmer}: {len(qwins)} windows", flush=True)
+    got = native.search_batch(p, ix, q).arrays()
+    q.close()
+    ix.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(got, o["final"], what=f"strands as given lmer {lmer}")
+
+
 @pytest.mark.parametrize("form", ["direct", "bitmap"])
 @pytest.mark.parametrize("lmer", [12, 15, 17])
 def test_index_forms(form, lmer, monkeypatch):
