@@ -727,9 +727,20 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     mark(EV_WRITE0);
     if (!keep_stages) {
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
+      // unit candidates as per-strand window masks when the windows fit 64 bits
+      if (ix.nwin <= 64) {
+        a.wmask = ws.get<uint64_t>("s.wmask", q.nseq);
+        k_fill<uint64_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(a.wmask, q.nseq, 0ull);
+      }
       segments(true, q.nseq, nullptr, nullptr, kSegFirst);
       // the big list's length stays on the device: a bounded grid strides over it
       segments(false, std::min<uint32_t>(q.nseq, 4096u * kSegWaves), big_list, tot32 + 6, kSegBigPass);
+      if (a.wmask) {
+        k_unit_first<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(a.wmask, q.qwin.ptr, q.nseq, ix.nwin, q.nwin,
+                                                                 unit_first);
+        rt.check_launch("k_unit_first");
+        a.wmask = nullptr;  // the later passes (global path, rerun) read unit_first
+      }
       rt.read_scalars(tot32 + 1, &novf, 1);
     } else {
       // every strand on the global path: list them all

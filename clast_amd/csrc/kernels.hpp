@@ -451,6 +451,7 @@ struct SeedArgs {
   int nqwin;
   int lmer, gap, width, local;
   uint32_t* unit_first;       // per (target window, query window): first strand with a seed
+  uint64_t* wmask;            // per strand: target windows it keeps seeds in (at most 64 windows), or null
 };
 
 // Strand of every k-mer position of a query set, written once when the set
@@ -474,6 +475,46 @@ __device__ __forceinline__ bool corner_ok(const SeedArgs& a, int d, int qlen, in
 __device__ __forceinline__ void note_unit_first(const SeedArgs& a, int tw, int qw, uint32_t s) {
   uint32_t* uf = &a.unit_first[tw * a.nqwin + qw];
   if (s < *uf) atomicMin(uf, s);
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x |= __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Each unit's first strand from the strands' window masks (k_seed_segments
+// with SeedArgs::wmask): thread per strand; a wavefront whose strands share
+// one query window takes, per window bit, its first lane with the bit and
+// makes one candidate of it, so candidates come per wavefront, not per
+// strand (a read of unit_first before each atomic per strand costs a
+// dependent load on the strand's path, and atomics on these few addresses
+// contend).  Strands ascend with the lanes.
+static __global__ __launch_bounds__(kBlock) void k_unit_first(const uint64_t* __restrict__ wmask,
+                                                              const int32_t* __restrict__ qwin, uint32_t nseq,
+                                                              int nwin, int nqwin, uint32_t* __restrict__ unit_first) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = s < nseq;
+  const uint64_t m = valid ? wmask[s] : 0ull;
+  const int qw = valid ? qwin[s] : -1;
+  const int lane = lane_id();
+  const int qw0 = __shfl(qw, 0, 64);
+  if (__all(!valid || qw == qw0)) {
+    uint32_t cand = 0xFFFFFFFFu;
+    for (int b = 0; b < nwin; ++b) {
+      const unsigned long long bm = __ballot(((m >> b) & 1ull) != 0ull);
+      if (bm && lane == b) cand = s - static_cast<uint32_t>(lane) + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(bm)) - 1);
+    }
+    if (cand != 0xFFFFFFFFu) {
+      uint32_t* uf = &unit_first[lane * nqwin + qw0];
+      if (cand < *uf) atomicMin(uf, cand);
+    }
+  } else {
+    for (uint64_t r = m; r; r &= r - 1ull) {
+      uint32_t* uf = &unit_first[(__ffsll(static_cast<unsigned long long>(r)) - 1) * nqwin + qw];
+      if (s < *uf) atomicMin(uf, s);
+    }
+  }
 }
 
 // Query k-mer lookup per position (krnlBinarySearch.cu:3-33 + the cell
@@ -713,9 +754,11 @@ __device__ __forceinline__ SeedMarks seed_marks(const SeedArgs& a, const SeedV& 
 // Element 0 of a unit is the first seed of the unit's first strand with a
 // seed in that target window.  The first pass (every strand; strands over
 // CAP go to the big list) and the big pass (the big list; strands over CAP
-// go to the global path) record each strand as a candidate (unit_first; the
-// value is read first so the atomic is rare) and take no seed as element 0; the strands that end up first in
-// some unit are then run again (kSegRerun) with unit_first final.
+// go to the global path) take no seed as element 0 and record the strand as
+// a candidate: its window mask (SeedArgs::wmask, reduced by k_unit_first)
+// or, with more than 64 target windows, unit_first itself (read first so
+// the atomic is rare); the strands that end up first in some unit are then
+// run again (kSegRerun) with unit_first final.
 template <bool PACKED, int CAP>
 static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, const uint4* __restrict__ cellp,
                                                                  const uint32_t* __restrict__ scnt,
@@ -755,27 +798,42 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     const uint32_t hbase = soff[s];
     // generate the strand's seeds into LDS in any order (they are sorted
     // next): entry j of every lane's cell per round, appended by ballot;
-    // seeds the corner filter removes are not written
+    // seeds the corner filter removes are not written.  The cell records of
+    // up to 256 positions, and the target sequence of each first entry, are
+    // requested before any is used (addresses clamped, so no load sits in a
+    // branch).
     uint32_t m = 0;
-    for (uint32_t c0 = p0; c0 < p1; c0 += 64) {
-      const uint32_t p = c0 + lane;
-      uint4 cp = make_uint4(0, 0, 0, 0);
-      if (p < p1) cp = cellp[p];
-      const int qIdx = static_cast<int>(p - p0);
-      for (uint32_t j = 0; __any(j < cp.y); ++j) {
-        bool ok = false;
-        SeedV v{0, 0, 0, qIdx};
-        if (j < cp.y) {
-          const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
-          const int2 ti = a.tinfo[e.x];
-          v.gs = e.x;
-          v.tw = ti.y;
-          v.d = e.y - qIdx;
-          ok = corner_ok(a, v.d, qLen, ti.x);
+    for (uint32_t b0 = p0; b0 < p1; b0 += 4 * 64) {
+      uint4 cps[4];
+      int2 t0s[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = b0 + 64u * u + lane;
+        cps[u] = cellp[p < p1 ? p : p1 - 1u];  // p1 > p0: the strand has seeds
+        if (p >= p1) cps[u].y = 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t0s[u] = a.tinfo[cps[u].y ? cps[u].z : 0u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (b0 + 64u * u >= p1) break;
+        const uint4 cp = cps[u];
+        const int qIdx = static_cast<int>(b0 + 64u * u + lane - p0);
+        for (uint32_t j = 0; __any(j < cp.y); ++j) {
+          bool ok = false;
+          SeedV v{0, 0, 0, qIdx};
+          if (j < cp.y) {
+            const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
+            const int2 ti = j == 0 ? t0s[u] : a.tinfo[e.x];
+            v.gs = e.x;
+            v.tw = ti.y;
+            v.d = e.y - qIdx;
+            ok = corner_ok(a, v.d, qLen, ti.x);
+          }
+          const unsigned long long bm = __ballot(ok);
+          if (ok) seg_put(L, w, m + static_cast<uint32_t>(__popcll(bm & below)), pk, v);
+          m += static_cast<uint32_t>(__popcll(bm));
         }
-        const unsigned long long bm = __ballot(ok);
-        if (ok) seg_put(L, w, m + static_cast<uint32_t>(__popcll(bm & below)), pk, v);
-        m += static_cast<uint32_t>(__popcll(bm));
       }
     }
     if (m == 0) {
@@ -798,11 +856,14 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
           mk = seed_marks(a, v, pv, lane > 0);
           bool elem0 = false;
           if (mk.twhead) {
-            uint32_t* uf = &a.unit_first[v.tw * a.nqwin + qw];
-            if (rerun) elem0 = *uf == s;
-            else if (s < *uf) atomicMin(uf, s);
+            if (rerun) elem0 = a.unit_first[v.tw * a.nqwin + qw] == s;
+            else if (!a.wmask) note_unit_first(a, v.tw, qw, s);
           }
           kk = mk.nr || elem0;
+        }
+        if (!rerun && a.wmask) {
+          const uint64_t wm = wave_or(valid && mk.twhead ? 1ull << (v.tw & 63) : 0ull);
+          if (lane == 0 && wm) a.wmask[s] = wm;
         }
         const int kin = wave_max_scan_incl<int>(kk ? lane : -1);
         const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? lane : -1);
@@ -860,6 +921,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     // filters over the sorted seeds, 64 at a time
     int carry_k = -1, carry_g = -1;
     uint32_t nh = 0;
+    uint64_t wm = 0;  // target windows of the strand's seeds (with SeedArgs::wmask)
     for (uint32_t c0 = 0; c0 < m; c0 += 64) {
       const uint32_t i = c0 + lane;
       const bool valid = i < m;
@@ -872,12 +934,12 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         mk = seed_marks(a, v, pv, i > 0);
         bool elem0 = false;
         if (mk.twhead) {
-          uint32_t* uf = &a.unit_first[v.tw * a.nqwin + qw];
-          if (rerun) elem0 = *uf == s;
-          else if (s < *uf) atomicMin(uf, s);
+          if (rerun) elem0 = a.unit_first[v.tw * a.nqwin + qw] == s;
+          else if (!a.wmask) note_unit_first(a, v.tw, qw, s);
         }
         kk = mk.nr || elem0;
       }
+      if (!rerun && a.wmask) wm |= wave_or(valid && mk.twhead ? 1ull << (v.tw & 63) : 0ull);
       const int kin = wave_max_scan_incl<int>(kk ? static_cast<int>(i) : -1);
       const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? static_cast<int>(i) : -1);
       int kex = __shfl_up(kin, 1, 64);
@@ -905,7 +967,10 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
       carry_k = lk > carry_k ? lk : carry_k;
       carry_g = lg > carry_g ? lg : carry_g;
     }
-    if (lane == 0) o.hcnt[s] = nh;
+    if (lane == 0) {
+      o.hcnt[s] = nh;
+      if (wm) a.wmask[s] = wm;
+    }
     wave_lds_sync();
   }
 }

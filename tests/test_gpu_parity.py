@@ -136,6 +136,31 @@ def test_batch_many_windows():
             assert_rows_equal(one, ou["final"], what=f"unit {w},{qa}")
 
 
+@pytest.mark.parametrize("n_t", [40, 70])
+def test_element0_many_units(n_t):
+    # Element 0 of every (target window, query window) unit: target windows
+    # of one sequence each (40: unit candidates as per-strand window masks;
+    # 70: over 64 windows, candidates through unit_first directly) and query
+    # windows of a few reads, so one wavefront of strands spans several query
+    # windows.
+    t, r = workload(n_t=n_t, t_len=3_000, n_r=300, repeat_frac=0.1, random_frac=0.1, seed=31)
+    p = native.Params(total_db_size=db_size(t))
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 3_100, query=False)
+    qwins = native.windows(ql.gateway, 3_000, query=True)
+    assert len(twins) == n_t and len(qwins) > 20
+    print(f"element 0: {len(twins)}x{len(qwins)} units", flush=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    got = native.search_batch(p, ix, q).arrays()
+    q.close()
+    ix.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(got, o["final"], what=f"{n_t} target windows")
+    np.testing.assert_array_equal(got["twin"], o["final"]["twin"])
+
+
 def test_uneven_lengths_and_local():
     t, r = workload(n_t=4, t_len=40_000, n_r=60, len_mode=gen.LEN_LOGUNIFORM, read_len=100, len_b=3000,
                     random_frac=0.2, indel=0.05)
