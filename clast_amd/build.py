@@ -74,7 +74,8 @@ def build_gen() -> str:
 
 
 def build_host() -> str:
-    """libclast_host.so (driver code as a library) and the clast executable."""
+    """libclast_host.so (driver code as a library), the clast executable and
+    the preprocessDB tool."""
     os.makedirs(BIN, exist_ok=True)
     srcs = [os.path.join(HOST, u) for u in HOST_UNITS]
     lib = os.path.join(LIB, "libclast_host.so")
@@ -85,6 +86,9 @@ def build_host() -> str:
     _run([*common, os.path.join(HOST, "main.cpp"), f"-L{LIB}", "-lclast_host", "-lclast_hip",
           "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
     os.replace(exe + ".tmp", exe)
+    pre = os.path.join(BIN, "preprocessDB")
+    _run(["g++", "-O3", "-std=c++17", os.path.join(HOST, "preprocess_db.cpp"), "-o", pre + ".tmp"])
+    os.replace(pre + ".tmp", pre)
     return exe
 
 

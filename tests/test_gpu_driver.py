@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLAST = os.path.join(ROOT, "clast_amd", "bin", "clast")
+PRE = os.path.join(ROOT, "clast_amd", "bin", "preprocessDB")
 
 
 def write_inputs(d):
@@ -77,5 +78,27 @@ def test_driver_tsv_matches_oracle(tmp_path, flags):
     run([CLAST, *inputs, "-o", "batch.tsv", "-batch", "1", *flags], d)
     exp = open(os.path.join(d, "exp.tsv")).read()
     assert len(exp.splitlines()) > 100
+    same_tsv(open(os.path.join(d, "plain.tsv")).read(), exp)
+    same_tsv(open(os.path.join(d, "batch.tsv")).read(), exp)
+
+
+def test_driver_split_target_matches_oracle(tmp_path):
+    # targets split by preprocessDB into overlapping '@offset' parts shorter
+    # than the sequences (reads fall inside parts and across part
+    # boundaries): the driver's rows on the split file equal the oracle
+    # CLI's, with target positions in the whole sequence
+    d = str(tmp_path)
+    t, r = workload(n_t=3, t_len=60_000, n_r=400, repeat_frac=0.1, seed=41)
+    gen.write_fasta(os.path.join(d, "t.fa"), t, "chr", width=70)
+    gen.write_fasta(os.path.join(d, "q.fa"), r, "read", width=80)
+    run([PRE, "t.fa", "parts.fa", "20000", "1000"], d)
+    inputs = ["-t", "parts.fa", "-q", "q.fa"]
+    print("driver on a split target", flush=True)
+    run([oracle.EXE, *inputs, "-o", "exp.tsv"], d)
+    run([CLAST, *inputs, "-o", "plain.tsv"], d)
+    run([CLAST, *inputs, "-o", "batch.tsv", "-batch", "1"], d)
+    exp = open(os.path.join(d, "exp.tsv")).read()
+    assert len(exp.splitlines()) > 100
+    assert max(int(x.split("\t")[5]) for x in exp.splitlines()) > 40_000
     same_tsv(open(os.path.join(d, "plain.tsv")).read(), exp)
     same_tsv(open(os.path.join(d, "batch.tsv")).read(), exp)

@@ -1,7 +1,7 @@
 """Host-side code without a GPU: the C-ABI library loads and exports every
 declared entry point, the host list layout (and its gateway computed from
-read lengths alone), windows, FASTA chunking and the driver's flag
-handling."""
+read lengths alone), windows, FASTA chunking, the driver's flag handling
+and the preprocessDB splitter."""
 from __future__ import annotations
 
 import ctypes
@@ -139,6 +139,85 @@ def test_driver_prints_settings_before_device(tmp_path):
     assert "allowable seed-seed width: 50" in r.stdout
     assert "RAM usage for target (MB): 1.5" in r.stdout
     assert "No.2 b.fa" in r.stdout
+
+
+PRE = os.path.join(ROOT, "clast_amd", "bin", "preprocessDB")
+
+
+def run_pre(args, cwd):
+    return subprocess.run([PRE, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                          timeout=60)
+
+
+def fasta_parts(text):
+    """(label, start, sequence) of every part of a preprocessDB output."""
+    parts = []
+    for block in text.split("\n\n"):
+        lines = [x for x in block.split("\n") if x]
+        if lines:
+            assert lines[1].startswith("@")
+            parts.append((lines[0], int(lines[1][1:]), "".join(lines[2:])))
+    return parts
+
+
+def test_preprocess_db_known_parts(tmp_path):
+    # preprocessDB.cpp's loops worked by hand: parts start at 0 and step
+    # part - overlap while a whole part fits, then one part holds the rest
+    # from the last start; 60-column lines, a part of exactly 60 bases gets
+    # an empty last line; spaces are removed; an empty line ends a record
+    # (lines after it until the next '>' are ignored); output is appended.
+    (tmp_path / "in.fa").write_text(">x desc\nACGT ACGTAC\n>v\n" + "A" * 60 + "C" * 40 + "\n" + "G" * 30 +
+                                    "\n\nTTTT\n>z\nGG\n")
+    r = run_pre(["in.fa", "out.fa", "100", "30"], tmp_path)
+    assert r.returncode == 0, r.stdout
+    exp = (">x desc\n@0\nACGTACGTAC\n\n"
+           ">v\n@0\n" + "A" * 60 + "\n" + "C" * 40 + "\n\n"
+           ">v\n@70\n" + "C" * 30 + "G" * 30 + "\n\n\n"
+           ">z\n@0\nGG\n\n")
+    assert (tmp_path / "out.fa").read_text() == exp
+    (tmp_path / "in2.fa").write_text(">x\nACGTACGTAC\n")
+    assert run_pre(["in2.fa", "out.fa", "4", "1"], tmp_path).returncode == 0
+    exp += ">x\n@0\nACGT\n\n>x\n@3\nTACG\n\n>x\n@6\nGTAC\n\n>x\n@9\nC\n\n"
+    assert (tmp_path / "out.fa").read_text() == exp
+
+
+def test_preprocess_db_round_trip(tmp_path):
+    # every part is the input at its offset; parts start every part - overlap
+    # bases, all whole but the last, which reaches the end (a 700-base
+    # sequence gives [0, 700) and the 150 bases from 550)
+    rng = np.random.default_rng(5)
+    seqs = ["".join(rng.choice(list("ACGTN"), size=n)) for n in (5000, 700, 699, 1)]
+    with open(tmp_path / "in.fa", "w") as f:
+        for i, s in enumerate(seqs):
+            f.write(f">s{i}\n")
+            for k in range(0, len(s), 77):
+                f.write(s[k:k + 77] + "\n")
+    assert run_pre(["in.fa", "out.fa", "700", "150"], tmp_path).returncode == 0
+    parts = fasta_parts((tmp_path / "out.fa").read_text())
+    for i, s in enumerate(seqs):
+        mine = [(st, q) for lab, st, q in parts if lab == f">s{i}"]
+        starts = [st for st, _ in mine]
+        assert starts == list(range(0, 550 * (len(starts) - 1) + 1, 550))
+        for st, q in mine:
+            assert q == s[st:st + len(q)]
+        # whole parts while one fits, then the rest from the last start
+        assert all(len(q) == 700 for _, q in mine[:-1]) and len(mine[-1][1]) <= 700
+        assert mine[-1][0] + len(mine[-1][1]) == len(s)
+
+
+def test_preprocess_db_arguments(tmp_path):
+    (tmp_path / "in.fa").write_text(">x\nACGT\n")
+    r = run_pre([], tmp_path)
+    assert r.returncode == 0 and "usage" in r.stdout
+    r = run_pre(["in.fa", "o.fa", "10", "10"], tmp_path)
+    assert r.returncode != 0 and "larger than the overlap" in r.stdout
+    r = run_pre(["in.fa", "o.fa", "10x", "2"], tmp_path)
+    assert r.returncode != 0 and "not an integer" in r.stdout
+    assert not (tmp_path / "o.fa").exists()
+    # no records: no output file
+    (tmp_path / "empty.fa").write_text("ACGT\n")
+    assert run_pre(["empty.fa", "o.fa", "10", "2"], tmp_path).returncode == 0
+    assert not (tmp_path / "o.fa").exists()
 
 
 def test_poisson_requests_partition_reads():
