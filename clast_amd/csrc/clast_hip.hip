@@ -231,11 +231,15 @@ static bool dp16_disabled() {
   const char* f = std::getenv("CLAST_DP16");
   return f && !std::strcmp(f, "0");
 }
-// CLAST_DP_BAND=0 keeps long extensions (at least kBandRows rows) off the
-// lane-per-band kernel (k_dp_band); same results, for comparison.
-static bool band_disabled() {
+// CLAST_DP_BAND=1 sends the longest extensions (at least kBandRows rows and
+// over twice one SIMD's average share) to the lane-per-band kernel
+// (k_dp_band); same results.  Off by default: it spends about five times
+// the instructions per cell to finish single long tasks sooner, which lost
+// when searches share the GPU (serving latency p50 and p99 both 16% worse)
+// and was even on long-read batches.
+static bool band_enabled() {
   const char* f = std::getenv("CLAST_DP_BAND");
-  return f && !std::strcmp(f, "0");
+  return f && !std::strcmp(f, "1");
 }
 
 // Cell lookup over the top B key bits of the sorted entries (keys in ek,
@@ -879,7 +883,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     // class bits on top of the rows (two with long tasks possible, else one:
     // classes 1 and 2 only), within the radix passes the rows need
     const bool packed = !p.local && !dp16_disabled();
-    const bool band = maxrows >= kBandRows && !band_disabled();
+    const bool band = maxrows >= kBandRows && band_enabled();
     const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
     const int classbit = sortbits - (band ? 2 : 1);
     const int rowshift = std::max(0, rowbits - classbit);

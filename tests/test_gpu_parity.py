@@ -287,7 +287,8 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     # lane group, interleaved a step apart); shorter ones take the other DP
     # kernels.  Reads of 1-6 kbp, some with star or X letters, some from
     # sequence ends (bands that leave the target), in one set.  Rows equal
-    # the oracle's with the lane-per-band path on and off.
+    # the oracle's with the lane-per-band path on (CLAST_DP_BAND=1) and off
+    # (the default).
     t, _ = workload(n_t=3, t_len=50_000, n_r=1, seed=13)
     long_r = gen.make_reads(gen.ReadSpec(31, 16, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=6000,
                                          sub_rate=0.02, indel_rate=0.01), t)
@@ -309,9 +310,10 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     ix = native.Index(p, tl, twins)
     q = native.Query(p, ql, qwins)
     print(f"band local={local} gap={gap}: {len(qwins)} query windows", flush=True)
+    monkeypatch.setenv("CLAST_DP_BAND", "1")
     h = native.search_batch(p, ix, q)
     got, cnt = h.arrays(), h.counts()
-    monkeypatch.setenv("CLAST_DP_BAND", "0")
+    monkeypatch.delenv("CLAST_DP_BAND")
     h0 = native.search_batch(p, ix, q)
     got0, cnt0 = h0.arrays(), h0.counts()
     ix.close()
