@@ -41,7 +41,7 @@ def expected_entries(p, tl, twins):
     return np.array(rows, dtype=np.int64).reshape(-1, 4)
 
 
-@pytest.mark.parametrize("lmer,stride,cut", [(15, 5, 20), (11, 3, 4), (20, 4, -1)])
+@pytest.mark.parametrize("lmer,stride,cut", [(15, 5, 20), (11, 3, 4), (20, 4, -1), (31, 7, 20), (13, 2, 6)])
 def test_index_entries(lmer, stride, cut):
     t, r = workload(n_t=5, t_len=12_000, n_r=10, repeat_frac=0.2, lowc=0.02)
     p = native.Params(lmer=lmer, stride=stride, cut_repeat=cut, total_db_size=db_size(t))
@@ -77,6 +77,8 @@ SETTINGS = [
     dict(gap=2),
     dict(gap=16, lmer=12),
     dict(lmer=21, stride=3),
+    dict(lmer=31, stride=7),
+    dict(lmer=13, stride=2, cut_repeat=6),
     dict(cut_repeat=-1, width=30),
     dict(cut_off=-1),
 ]
