@@ -20,6 +20,85 @@
 
 namespace clast {
 
+// ---------------------------------------------------------------- lanes
+
+// Cross-lane moves as DPP operand modifiers (no LDS round trip).  gfx9
+// controls: quad_perm selects within groups of 4 lanes; row_shl:n / row_shr:n
+// read lane i + n / i - n of the same row of 16; wave_shr:1 reads lane i - 1;
+// row_bcast:15 / :31 broadcast lane 15 of each row to the next row / lane 31
+// to rows 2 and 3 (rows picked by the row mask).  A lane whose source lies
+// outside its row, or whose row is masked off, keeps `old` (bound_ctrl off).
+constexpr int kDppQuadXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]: lane i ^ 1
+constexpr int kDppQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]: lane i ^ 2
+constexpr int kDppRowShl = 0x100;   // + n
+constexpr int kDppRowShr = 0x110;   // + n
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(v), CTRL, ROWS, 0xF, false));
+}
+
+// Value of lane i ^ J (J a power of two): DPP for J <= 8, ds_bpermute above.
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+  if constexpr (J == 1) {
+    return dpp_u32<kDppQuadXor1>(v, v);
+  } else if constexpr (J == 2) {
+    return dpp_u32<kDppQuadXor2>(v, v);
+  } else if constexpr (J == 4 || J == 8) {
+    const uint32_t up = dpp_u32<kDppRowShl + J>(v, v);  // lanes with bit J clear read i + J
+    const uint32_t dn = dpp_u32<kDppRowShr + J>(v, v);  // lanes with bit J set read i - J
+    return (lane_id() & J) ? dn : up;
+  } else {
+    return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), J, 64));
+  }
+}
+
+template <int J>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v) {
+  return (static_cast<uint64_t>(xor_lane<J>(static_cast<uint32_t>(v >> 32))) << 32) |
+         xor_lane<J>(static_cast<uint32_t>(v));
+}
+
+// Value of lane i - 1 (lane 0: `old`).
+__device__ __forceinline__ uint32_t lane_up1(uint32_t old, uint32_t v) { return dpp_u32<kDppWaveShr1>(old, v); }
+__device__ __forceinline__ uint64_t lane_up1_64(uint64_t v) {
+  const uint32_t hi = static_cast<uint32_t>(v >> 32), lo = static_cast<uint32_t>(v);
+  return (static_cast<uint64_t>(lane_up1(hi, hi)) << 32) | lane_up1(lo, lo);
+}
+
+// Inclusive max scan over the wavefront: within rows of 16, then across
+// rows by broadcasts.
+__device__ __forceinline__ int wave_max_scan_incl(int v) {
+  auto mx = [](int a, uint32_t b) { return static_cast<int>(b) > a ? static_cast<int>(b) : a; };
+  const uint32_t lo = static_cast<uint32_t>(INT_MIN);
+  v = mx(v, dpp_u32<kDppRowShr + 1>(lo, static_cast<uint32_t>(v)));
+  v = mx(v, dpp_u32<kDppRowShr + 2>(lo, static_cast<uint32_t>(v)));
+  v = mx(v, dpp_u32<kDppRowShr + 4>(lo, static_cast<uint32_t>(v)));
+  v = mx(v, dpp_u32<kDppRowShr + 8>(lo, static_cast<uint32_t>(v)));
+  v = mx(v, dpp_u32<kDppRowBcast15, 0xA>(lo, static_cast<uint32_t>(v)));
+  v = mx(v, dpp_u32<kDppRowBcast31, 0xC>(lo, static_cast<uint32_t>(v)));
+  return v;
+}
+
+// OR over the wavefront (the same value in every lane).
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
+  v |= dpp_u32<kDppRowShr + 1>(0u, v);
+  v |= dpp_u32<kDppRowShr + 2>(0u, v);
+  v |= dpp_u32<kDppRowShr + 4>(0u, v);
+  v |= dpp_u32<kDppRowShr + 8>(0u, v);
+  v |= dpp_u32<kDppRowBcast15, 0xA>(0u, v);
+  v |= dpp_u32<kDppRowBcast31, 0xC>(0u, v);
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+  return (static_cast<uint64_t>(wave_or32(static_cast<uint32_t>(x >> 32))) << 32) | wave_or32(static_cast<uint32_t>(x));
+}
+
 // ---------------------------------------------------------------- bases
 
 // Base classes.  Every base byte is stored on the device as a class index:
@@ -477,12 +556,6 @@ __device__ __forceinline__ void note_unit_first(const SeedArgs& a, int tw, int q
   if (s < *uf) atomicMin(uf, s);
 }
 
-__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) x |= __shfl_xor(x, d, 64);
-  return x;
-}
-
 // Each unit's first strand from the strands' window masks (k_seed_segments
 // with SeedArgs::wmask): thread per strand; a wavefront whose strands share
 // one query window takes, per window bit, its first lane with the bit and
@@ -586,17 +659,6 @@ __device__ __forceinline__ bool near_pair(int t, int d, int pt, int pd, int widt
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-}
-
-template <class T>
-__device__ __forceinline__ T wave_max_scan_incl(T v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const T o = __shfl_up(v, d, 64);
-    if (l >= d) v = o > v ? o : v;
-  }
-  return v;
 }
 
 __device__ __forceinline__ uint32_t wave_sum_incl(uint32_t v) {
@@ -706,19 +768,19 @@ enum SegMode { kSegFirst = 0, kSegBigPass = 1, kSegRerun = 2 };
 
 // Bitonic sort of one u64 per lane, lanes [0, N) ascending.  Lanes from N
 // on must hold ~0 (with N < 64 they never exchange with the lower lanes).
-template <int N>
+template <int KK, int J>
+__device__ __forceinline__ uint64_t bitonic_merge(uint64_t k, int lane) {
+  const uint64_t o = xor_lane64<J>(k);
+  const bool keep_min = ((lane & KK) == 0) == ((lane & J) == 0);
+  k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+  if constexpr (J > 1) return bitonic_merge<KK, J / 2>(k, lane);
+  else return k;
+}
+template <int N, int KK = 2>
 __device__ __forceinline__ uint64_t wave_sort(uint64_t k) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int kk = 2; kk <= N; kk <<= 1) {
-#pragma unroll
-    for (int j = kk >> 1; j > 0; j >>= 1) {
-      const uint64_t o = __shfl_xor(k, j, 64);
-      const bool keep_min = ((lane & kk) == 0) == ((lane & j) == 0);
-      k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
-    }
-  }
-  return k;
+  k = bitonic_merge<KK, KK / 2>(k, lane_id());
+  if constexpr (KK < N) return wave_sort<N, KK * 2>(k);
+  else return k;
 }
 
 // Isolate-filter flags of one sorted seed v with predecessor pv (has_pred:
@@ -849,7 +911,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
                              : m <= 32 ? wave_sort<32>(k0) : wave_sort<64>(k0);
         const bool valid = lane < static_cast<int>(m);
         const SeedV v = seg_decode(key, pk);
-        const SeedV pv = seg_decode(__shfl_up(key, 1, 64), pk);
+        const SeedV pv = seg_decode(lane_up1_64(key), pk);
         SeedMarks mk{false, false, false};
         bool kk = false;
         if (valid) {
@@ -865,10 +927,9 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
           const uint64_t wm = wave_or(valid && mk.twhead ? 1ull << (v.tw & 63) : 0ull);
           if (lane == 0 && wm) a.wmask[s] = wm;
         }
-        const int kin = wave_max_scan_incl<int>(kk ? lane : -1);
-        const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? lane : -1);
-        int prevk = __shfl_up(kin, 1, 64);
-        if (lane == 0) prevk = -1;
+        const int kin = wave_max_scan_incl(kk ? lane : -1);
+        const int gin = wave_max_scan_incl(valid && mk.ghead ? lane : -1);
+        const int prevk = static_cast<int>(lane_up1(0xFFFFFFFFu, static_cast<uint32_t>(kin)));
         const SeedV kv = seg_decode(__shfl(key, prevk < 0 ? 0 : prevk, 64), pk);
         bool dd = false;
         if (valid && kk) {
@@ -940,10 +1001,9 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         kk = mk.nr || elem0;
       }
       if (!rerun && a.wmask) wm |= wave_or(valid && mk.twhead ? 1ull << (v.tw & 63) : 0ull);
-      const int kin = wave_max_scan_incl<int>(kk ? static_cast<int>(i) : -1);
-      const int gin = wave_max_scan_incl<int>(valid && mk.ghead ? static_cast<int>(i) : -1);
-      int kex = __shfl_up(kin, 1, 64);
-      if (lane == 0) kex = -1;
+      const int kin = wave_max_scan_incl(kk ? static_cast<int>(i) : -1);
+      const int gin = wave_max_scan_incl(valid && mk.ghead ? static_cast<int>(i) : -1);
+      const int kex = static_cast<int>(lane_up1(0xFFFFFFFFu, static_cast<uint32_t>(kin)));
       const int prevk = kex > carry_k ? kex : carry_k;
       const int gstart = gin > carry_g ? gin : carry_g;
       bool dd = false;
@@ -963,7 +1023,7 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
         o.hq[at] = v.q;
       }
       nh += static_cast<uint32_t>(__popcll(hm));
-      const int lk = __shfl(kin, 63, 64), lg = __shfl(gin, 63, 64);
+      const int lk = __builtin_amdgcn_readlane(kin, 63), lg = __builtin_amdgcn_readlane(gin, 63);
       carry_k = lk > carry_k ? lk : carry_k;
       carry_g = lg > carry_g ? lg : carry_g;
     }
