@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -302,6 +303,34 @@ void build_cells(Runtime& rt, clast_index_s& ix, const uint64_t* ek, uint32_t ne
   rt.check_launch("rank");
 }
 
+// Device tables of a target set derived from its host tables and class
+// bytes (ix.tbase, filled): window and sequence tables, the 4-bit copy and
+// the clean flags of the 16-bit DP.
+static void target_tables(Runtime& rt, clast_index_s& ix, const std::vector<uint64_t>& seq_slot) {
+  hipStream_t st = rt.stream();
+  const std::vector<uint64_t>& win_slot = ix.h_win_slot;
+  ix.win_slot.upload(win_slot.data(), win_slot.size(), st);
+  ix.seq_slot.upload(seq_slot.data(), seq_slot.size(), st);
+  ix.seq_len.upload(ix.h_seq_len.data(), ix.h_seq_len.size(), st);
+  ix.seq_win.upload(ix.h_seq_win.data(), ix.h_seq_win.size(), st);
+  {
+    std::vector<int2> info(ix.h_seq_len.size());
+    for (size_t i = 0; i < info.size(); ++i) info[i] = make_int2(ix.h_seq_len[i], ix.h_seq_win[i]);
+    ix.seq_info.upload(info.data(), info.size(), st);
+  }
+  ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
+  {
+    const uint64_t ndw = 2 * (static_cast<uint64_t>(ix.bmax) + 1);
+    ix.tnib.alloc(ndw);
+    k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(ix.tbase.ptr, ndw, 0, ix.tnib.ptr);
+  }
+  ix.seq_clean.alloc(std::max(ix.nseq, 1));
+  if (ix.nseq)
+    k_clean_flags<uint64_t><<<grid_for(ix.nseq, kBlock / 64), kBlock, 0, st>>>(ix.tbase.ptr, ix.seq_slot.ptr,
+                                                                               ix.seq_len.ptr, ix.nseq,
+                                                                               ix.seq_clean.ptr);
+}
+
 void build_index(const clast_params& p, const clast_window* wins, int nwin, clast_index_s& ix) {
   Runtime& rt = runtime();
   std::lock_guard<std::mutex> lk(rt.mutex());
@@ -345,26 +374,7 @@ void build_index(const clast_params& p, const clast_window* wins, int nwin, clas
                 "upload target bases");
   if (win_slot[nwin])
     k_classify<<<grid_for(win_slot[nwin], kBlock), kBlock, 0, st>>>(ix.tbase.ptr, win_slot[nwin]);
-  ix.win_slot.upload(win_slot.data(), win_slot.size(), st);
-  ix.seq_slot.upload(seq_slot.data(), seq_slot.size(), st);
-  ix.seq_len.upload(ix.h_seq_len.data(), ix.h_seq_len.size(), st);
-  ix.seq_win.upload(ix.h_seq_win.data(), ix.h_seq_win.size(), st);
-  {
-    std::vector<int2> info(ix.h_seq_len.size());
-    for (size_t i = 0; i < info.size(); ++i) info[i] = make_int2(ix.h_seq_len[i], ix.h_seq_win[i]);
-    ix.seq_info.upload(info.data(), info.size(), st);
-  }
-  ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
-  {
-    const uint64_t ndw = 2 * (static_cast<uint64_t>(ix.bmax) + 1);
-    ix.tnib.alloc(ndw);
-    k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(ix.tbase.ptr, ndw, 0, ix.tnib.ptr);
-  }
-  ix.seq_clean.alloc(std::max(ix.nseq, 1));
-  if (ix.nseq)
-    k_clean_flags<uint64_t><<<grid_for(ix.nseq, kBlock / 64), kBlock, 0, st>>>(ix.tbase.ptr, ix.seq_slot.ptr,
-                                                                               ix.seq_len.ptr, ix.nseq,
-                                                                               ix.seq_clean.ptr);
+  target_tables(rt, ix, seq_slot);
   DevArray<uint32_t> d_win_stride, d_win_seq;
   d_win_stride.upload(win_stride.data(), win_stride.size(), st);
   d_win_seq.upload(win_seq.data(), win_seq.size(), st);
@@ -1109,6 +1119,43 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
 
 // ---------------------------------------------------------------- C ABI
 
+// Persisted index file helpers (clast_index_save / clast_index_load).
+namespace {
+
+constexpr char kIndexMagic[8] = {'C', 'L', 'A', 'S', 'T', 'I', 'X', '1'};
+constexpr uint32_t kIndexVersion = 1;
+
+struct IndexFileHead {
+  char magic[8];
+  uint32_t version;
+  int32_t lmer, stride, cut_repeat, nwin, nseq;
+  uint32_t nentries;
+  uint64_t nslots;
+};
+
+struct FileCloser {
+  void operator()(std::FILE* f) const { std::fclose(f); }
+};
+using File = std::unique_ptr<std::FILE, FileCloser>;
+
+void file_write(std::FILE* f, const void* p, size_t n) {
+  if (n && std::fwrite(p, 1, n, f) != n) throw ClastError(CLAST_ERR_IO, "index file: write failed");
+}
+void file_read(std::FILE* f, void* p, size_t n) {
+  if (n && std::fread(p, 1, n, f) != n) throw ClastError(CLAST_ERR_IO, "index file: truncated or unreadable");
+}
+template <class T>
+void file_write_vec(std::FILE* f, const std::vector<T>& v) { file_write(f, v.data(), sizeof(T) * v.size()); }
+template <class T>
+std::vector<T> file_read_vec(std::FILE* f, uint64_t n) {
+  std::vector<T> v(n);
+  file_read(f, v.data(), sizeof(T) * n);
+  return v;
+}
+void bad_file(const char* what) { throw ClastError(CLAST_ERR_IO, std::string("index file: ") + what); }
+
+}  // namespace
+
 extern "C" {
 
 int clast_init(int32_t device) {
@@ -1187,6 +1234,137 @@ int clast_index_entries(clast_index ix, int64_t* key, int32_t* window, int32_t* 
       if (tID) tID[r] = ix->h_seq_ref[ep[i].x];
       if (tIdx) tIdx[r] = ep[i].y;
     }
+  });
+}
+
+// ---------------------------------------------------------------- persisted index
+//
+// File layout (host byte order): IndexFileHead, then
+//   window slot offsets  u64 x (nwin + 1)    window list IDs  i32 x nwin
+//   sequence first slots u64 x nseq          lengths, windows, list IDs  i32 x nseq each
+//   class bytes          u8  x nslots        entries (seq, tIdx) i32 x 2 x nentries
+//   entry index codes    u64 x nentries      (ascending)
+
+
+int clast_index_save(clast_index ix, const char* path) {
+  return guard([&] {
+    if (!ix || !path) throw ClastError(CLAST_ERR_ARG, "null index or path");
+    set_device(ix->device);
+    Runtime& rt = runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    hipStream_t st = rt.stream();
+    const uint64_t nslots = ix->h_win_slot.back();
+    const uint32_t n = ix->nentries;
+    std::vector<uint8_t> tb;
+    std::vector<uint64_t> seq_slot, keys(n);
+    std::vector<int2> ep;
+    std::vector<uint32_t> k32;
+    d2h(tb, ix->tbase.ptr, nslots, st);
+    d2h(seq_slot, ix->seq_slot.ptr, static_cast<uint64_t>(ix->nseq), st);
+    d2h(ep, ix->epos.ptr, n, st);
+    if (ix->key64) d2h(keys, ix->ekey64.ptr, n, st);
+    else d2h(k32, ix->ekey32.ptr, n, st);
+    rt.sync();
+    if (!ix->key64)
+      for (uint32_t i = 0; i < n; ++i) keys[i] = k32[i];
+    File f(std::fopen(path, "wb"));
+    if (!f) throw ClastError(CLAST_ERR_IO, std::string("index file: cannot create ") + path);
+    IndexFileHead h{};
+    std::memcpy(h.magic, kIndexMagic, sizeof(h.magic));
+    h.version = kIndexVersion;
+    h.lmer = ix->lmer;
+    h.stride = ix->stride;
+    h.cut_repeat = ix->cut_repeat;
+    h.nwin = ix->nwin;
+    h.nseq = ix->nseq;
+    h.nentries = n;
+    h.nslots = nslots;
+    file_write(f.get(), &h, sizeof(h));
+    file_write_vec(f.get(), ix->h_win_slot);
+    file_write_vec(f.get(), ix->h_id_begin);
+    file_write_vec(f.get(), seq_slot);
+    file_write_vec(f.get(), ix->h_seq_len);
+    file_write_vec(f.get(), ix->h_seq_win);
+    file_write_vec(f.get(), ix->h_seq_ref);
+    file_write_vec(f.get(), tb);
+    file_write_vec(f.get(), ep);
+    file_write_vec(f.get(), keys);
+    if (std::fflush(f.get()) != 0) throw ClastError(CLAST_ERR_IO, "index file: write failed");
+  });
+}
+
+int clast_index_load(const char* path, clast_index* out) {
+  return guard([&] {
+    if (!path || !out) throw ClastError(CLAST_ERR_ARG, "null path or output");
+    // everything is read and checked on the host before any device work
+    File f(std::fopen(path, "rb"));
+    if (!f) throw ClastError(CLAST_ERR_IO, std::string("index file: cannot open ") + path);
+    IndexFileHead h{};
+    file_read(f.get(), &h, sizeof(h));
+    if (std::memcmp(h.magic, kIndexMagic, sizeof(h.magic)) != 0) bad_file("not an index file");
+    if (h.version != kIndexVersion) bad_file("unsupported version");
+    if (h.lmer < 1 || h.lmer > 31 || h.stride < 1 || h.nwin < 1 || h.nseq < 0 || h.nslots >= (1ull << 35))
+      bad_file("header out of range");
+    const uint64_t nw = static_cast<uint64_t>(h.nwin), nq = static_cast<uint64_t>(h.nseq), n = h.nentries;
+    const uint64_t expect = sizeof(h) + 8 * (nw + 1) + 4 * nw + 8 * nq + 12 * nq + h.nslots + 16 * n;
+    if (std::fseek(f.get(), 0, SEEK_END) != 0) bad_file("not seekable");
+    const long size = std::ftell(f.get());
+    if (size < 0 || static_cast<uint64_t>(size) != expect) bad_file("size does not match its header");
+    if (std::fseek(f.get(), static_cast<long>(sizeof(h)), SEEK_SET) != 0) bad_file("not seekable");
+    auto ix = std::make_unique<clast_index_s>();
+    ix->lmer = h.lmer;
+    ix->stride = h.stride;
+    ix->cut_repeat = h.cut_repeat;
+    ix->nwin = h.nwin;
+    ix->nseq = h.nseq;
+    ix->key64 = h.lmer > 16;
+    ix->h_win_slot = file_read_vec<uint64_t>(f.get(), nw + 1);
+    ix->h_id_begin = file_read_vec<int32_t>(f.get(), nw);
+    const std::vector<uint64_t> seq_slot = file_read_vec<uint64_t>(f.get(), nq);
+    ix->h_seq_len = file_read_vec<int32_t>(f.get(), nq);
+    ix->h_seq_win = file_read_vec<int32_t>(f.get(), nq);
+    ix->h_seq_ref = file_read_vec<int32_t>(f.get(), nq);
+    const std::vector<uint8_t> tb = file_read_vec<uint8_t>(f.get(), h.nslots);
+    const std::vector<int2> ep = file_read_vec<int2>(f.get(), n);
+    const std::vector<uint64_t> keys = file_read_vec<uint64_t>(f.get(), n);
+    // the tables must describe a layout the kernels can index
+    if (ix->h_win_slot[0] != 0 || ix->h_win_slot[nw] != h.nslots) bad_file("window table");
+    for (uint64_t w = 0; w < nw; ++w)
+      if (ix->h_win_slot[w + 1] < ix->h_win_slot[w]) bad_file("window table");
+    for (uint64_t q = 0; q < nq; ++q) {
+      const int32_t w = ix->h_seq_win[q], len = ix->h_seq_len[q];
+      if (w < 0 || w >= h.nwin || (q > 0 && w < ix->h_seq_win[q - 1]) || len < 0 ||
+          seq_slot[q] < ix->h_win_slot[w] || seq_slot[q] + static_cast<uint64_t>(len) > ix->h_win_slot[w + 1])
+        bad_file("sequence table");
+      ix->max_tlen = std::max(ix->max_tlen, len);
+    }
+    for (uint8_t c : tb)
+      if (c > kClsX) bad_file("class bytes");
+    const uint64_t kmax = 1ull << (2 * h.lmer);
+    for (uint64_t i = 0; i < n; ++i) {
+      if (keys[i] >= kmax || (i > 0 && keys[i] < keys[i - 1])) bad_file("entry codes");
+      if (ep[i].x < 0 || ep[i].x >= h.nseq || ep[i].y < 0 ||
+          static_cast<int64_t>(ep[i].y) + h.lmer > ix->h_seq_len[static_cast<size_t>(ep[i].x)])
+        bad_file("entries");
+    }
+    f.reset();
+
+    Runtime& rt = runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    hipStream_t st = rt.stream();
+    ix->device = rt.device();
+    ix->tbase.alloc(padded_blocks(h.nslots) * 16);
+    ix->bmax = static_cast<int32_t>(padded_blocks(h.nslots) - 1);
+    if (h.nslots) hip_check(hipMemcpyAsync(ix->tbase.ptr, tb.data(), h.nslots, hipMemcpyHostToDevice, st), "upload");
+    target_tables(rt, *ix, seq_slot);
+    ix->epos.upload(ep.data(), n, st);
+    uint64_t* ek = n ? rt.ws().get<uint64_t>("ix.ekey", n) : nullptr;
+    if (n) hip_check(hipMemcpyAsync(ek, keys.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, st), "upload");
+    build_cells(rt, *ix, ek, static_cast<uint32_t>(n), 2 * h.lmer);
+    ix->nentries = static_cast<uint32_t>(n);
+    rt.check_launch("clast_index_load");
+    rt.sync();
+    *out = ix.release();
   });
 }
 

@@ -83,6 +83,8 @@ def hip() -> C.CDLL:
             "clast_index_entry_count": (i64, [vp]),
             "clast_index_entries": (C.c_int, [vp, vp, vp, vp, vp]),
             "clast_index_destroy": (C.c_int, [vp]),
+            "clast_index_save": (C.c_int, [vp, C.c_char_p]),
+            "clast_index_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
             "clast_query_create": (C.c_int, [pp, wp, C.POINTER(vp)]),
             "clast_query_create_multi": (C.c_int, [pp, wp, i32, C.POINTER(vp)]),
             "clast_query_create_reads": (C.c_int, [pp, vp, vp, vp, vp, i32, C.POINTER(vp)]),
@@ -257,6 +259,22 @@ class Index:
                "clast_index_create_multi")
         for lst, _ in lists_and_windows:
             lst.release_windows()
+        self.h = h
+        return self
+
+    def save(self, path: str) -> None:
+        """Writes the index to a file (clast_index_save)."""
+        _check(hip().clast_index_save(self.h, os.fsencode(path)), "clast_index_save")
+
+    @classmethod
+    def load(cls, params: Params, path: str) -> "Index":
+        """An index read back from a file written by save (clast_index_load);
+        params must be the ones it was built with."""
+        self = cls.__new__(cls)
+        self.params = params
+        self.wins = []
+        h = C.c_void_p()
+        _check(hip().clast_index_load(os.fsencode(path), C.byref(h)), "clast_index_load")
         self.h = h
         return self
 

@@ -43,7 +43,8 @@ enum {
   CLAST_ERR_ARG = 1,    /* invalid argument or handle */
   CLAST_ERR_HIP = 2,    /* a HIP runtime call failed */
   CLAST_ERR_NOMEM = 3,  /* device allocation failed */
-  CLAST_ERR_LIMIT = 4   /* input exceeds a representable size (split it) */
+  CLAST_ERR_LIMIT = 4,  /* input exceeds a representable size (split it) */
+  CLAST_ERR_IO = 5      /* an index file could not be written, read or is not one */
 };
 
 /* Stage-retention flag for clast_search_batch (stage-level parity checks). */
@@ -115,6 +116,16 @@ int clast_index_create_multi(const clast_params* p, const clast_window* targets,
 int64_t clast_index_entry_count(clast_index idx);
 int clast_index_entries(clast_index idx, int64_t* key, int32_t* window, int32_t* tID, int32_t* tIdx);
 int clast_index_destroy(clast_index idx);
+/* Persisted index, no counterpart in the reference (which rebuilds the table
+ * for every query chunk, main.cu:63-88, CHostSchedular.cu:70): for serving
+ * start-up.  clast_index_save writes the target windows (class bytes,
+ * sequence tables) and the index entries to a file; clast_index_load reads
+ * one back on the current device (clast_init) and rebuilds the cell tables,
+ * giving an index equal to the saved one.  A file that is not a complete
+ * index of this format is rejected with CLAST_ERR_IO before any device
+ * work. */
+int clast_index_save(clast_index idx, const char* path);
+int clast_index_load(const char* path, clast_index* out);
 
 /* CDeviceSeqList_query ctor (CDeviceSeqList_query.cu:61-71): uploads one
  * query window ('+' and '-' strand of each query, IDs 2k and 2k+1). */

@@ -58,6 +58,36 @@ def test_index_entries(lmer, stride, cut):
     np.testing.assert_array_equal(got["tIdx"], exp[:, 3])
 
 
+@pytest.mark.parametrize("lmer,form", [(15, "auto"), (12, "bitmap"), (17, "auto")])
+def test_index_save_load(tmp_path, lmer, form, monkeypatch):
+    # an index written to a file and read back (its cell tables rebuilt, in
+    # the form the environment asks for) holds the same entries and gives
+    # the same rows as the index it was saved from, and the oracle's
+    t, r = workload(n_t=5, t_len=12_000, n_r=200, repeat_frac=0.2, seed=19)
+    p = native.Params(lmer=lmer, total_db_size=db_size(t))
+    tl, ql = lists(t, r, lmer)
+    twins = [(0, 2), (2, 5)]
+    qwins = native.windows(ql.gateway, 30_000, query=True)
+    print(f"index save/load lmer={lmer} form={form}", flush=True)
+    ix = native.Index(p, tl, twins)
+    path = str(tmp_path / "index.bin")
+    ix.save(path)
+    monkeypatch.setenv("CLAST_INDEX_FORM", form)
+    ix2 = native.Index.load(p, path)
+    e1, e2 = ix.entries(), ix2.entries()
+    for k in e1:
+        np.testing.assert_array_equal(e1[k], e2[k])
+    q = native.Query(p, ql, qwins)
+    got1 = native.search_batch(p, ix, q).arrays()
+    got2 = native.search_batch(p, ix2, q).arrays()
+    for h in (q, ix, ix2):
+        h.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(got2, got1, what="loaded vs built")
+    assert_rows_equal(got2, o["final"], what="loaded vs oracle")
+
+
 def run_unit(p, t, r, tw=None, qw=None, keep=True):
     """keep=True retains the stages (every strand on the global path);
     keep=False runs the per-strand LDS path."""

@@ -1,12 +1,13 @@
 """Host-side code without a GPU: the C-ABI library loads and exports every
 declared entry point, the host list layout (and its gateway computed from
-read lengths alone), windows, FASTA chunking, the driver's flag handling
-and the preprocessDB splitter."""
+read lengths alone), windows, FASTA chunking, the driver's flag handling,
+the preprocessDB splitter and the checks of a persisted index file."""
 from __future__ import annotations
 
 import ctypes
 import os
 import re
+import struct
 import subprocess
 
 import numpy as np
@@ -139,6 +140,69 @@ def test_driver_prints_settings_before_device(tmp_path):
     assert "allowable seed-seed width: 50" in r.stdout
     assert "RAM usage for target (MB): 1.5" in r.stdout
     assert "No.2 b.fa" in r.stdout
+
+
+def index_file_head(lmer=15, stride=5, cut=20, nwin=1, nseq=1, nent=0, nslots=0, magic=b"CLASTIX1", version=1):
+    """Header of a persisted index file: magic, version, lMer, stride, repeat
+    cut, windows, sequences, entries, slots (host byte order, the u64 at
+    offset 40)."""
+    return struct.pack("<8sI5iI4xQ", magic, version, lmer, stride, cut, nwin, nseq, nent, nslots)
+
+
+def index_file(head_kw, win_slot, id_begin, seq_slot, seq_len, seq_win, seq_ref, classes, epos, codes):
+    return (index_file_head(**head_kw) + np.asarray(win_slot, "<u8").tobytes() + np.asarray(id_begin, "<i4").tobytes()
+            + np.asarray(seq_slot, "<u8").tobytes() + np.asarray(seq_len, "<i4").tobytes()
+            + np.asarray(seq_win, "<i4").tobytes() + np.asarray(seq_ref, "<i4").tobytes()
+            + np.asarray(classes, np.uint8).tobytes() + np.asarray(epos, "<i4").tobytes()
+            + np.asarray(codes, "<u8").tobytes())
+
+
+def test_index_file_rejected_before_device(tmp_path):
+    # clast_index_load checks a file on the host before any device work: a
+    # missing file, another format, a size that does not match the header,
+    # and tables the kernels could not index are CLAST_ERR_IO (5)
+    lib = ctypes.CDLL(os.path.join(ROOT, "clast_amd", "lib", "libclast_hip.so"))
+    lib.clast_index_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.clast_last_error.restype = ctypes.c_char_p
+
+    def load(data, name="ix.bin"):
+        path = tmp_path / name
+        if data is not None:
+            path.write_bytes(data)
+        h = ctypes.c_void_p()
+        rc = lib.clast_index_load(os.fsencode(str(path)), ctypes.byref(h))
+        return rc, lib.clast_last_error().decode()
+
+    # one window, one 20-base sequence (+ 14 joint slots), two entries
+    good = dict(head_kw=dict(nent=2, nslots=34), win_slot=[0, 34], id_begin=[0], seq_slot=[0], seq_len=[20],
+                seq_win=[0], seq_ref=[0], classes=[0, 1, 2, 3] * 8 + [0, 1], epos=[[0, 0], [0, 5]], codes=[7, 9])
+    assert len(index_file(**good)) == 48 + 16 + 4 + 8 + 12 + 34 + 32
+
+    def bad(**kw):
+        d = dict(good)
+        d.update(kw)
+        return index_file(**d)
+
+    cases = [
+        (None, "cannot open"),
+        (b"not an index", "truncated"),
+        (index_file_head(magic=b"CLASTIX0") + bytes(100), "not an index file"),
+        (index_file_head(version=2) + bytes(100), "unsupported version"),
+        (index_file_head(lmer=32) + bytes(100), "header out of range"),
+        (index_file(**good)[:-8], "size does not match"),
+        (index_file(**good) + b"x", "size does not match"),
+        (bad(win_slot=[0, 33]), "window table"),
+        (bad(seq_len=[40]), "sequence table"),
+        (bad(seq_win=[1]), "sequence table"),
+        (bad(classes=[0] * 33 + [7]), "class bytes"),
+        (bad(codes=[9, 7]), "entry codes"),
+        (bad(codes=[7, 1 << 30]), "entry codes"),
+        (bad(epos=[[0, 0], [1, 0]]), "entries"),
+        (bad(epos=[[0, 0], [0, 6]]), "entries"),
+    ]
+    for i, (data, msg) in enumerate(cases):
+        rc, err = load(data, f"ix{i}.bin")
+        assert rc == 5 and msg in err, (i, rc, err)
 
 
 PRE = os.path.join(ROOT, "clast_amd", "bin", "preprocessDB")
