@@ -96,9 +96,16 @@ class ResultHolder {
 };
 
 // CHostSchedular::search (CHostSchedular.cu:43-130): every target window x
-// query window, one clast_search each (plain) or one clast_search_batch per
-// call (batched; same output).
+// query window, one clast_search each.
 void search(const Settings& s, const SeqList& targets, const SeqList& queries, ResultHolder& holder);
+
+// Batched path (-batch 1): the query set of a query chunk is built once, on
+// the device from the reads as given (clast_query_create_reads: both strands
+// and joints, windows moveQueryWindow computes from the read lengths), and
+// searched against every target chunk with one launch chain each; the same
+// rows as search() over the host-built query list.
+clast_query query_chunk_reads(const Settings& s, const std::vector<Fasta>& reads);  // null: no windows
+void search_batched(const Settings& s, const SeqList& targets, clast_query queries, ResultHolder& holder);
 
 // Throws std::runtime_error with clast_last_error() on a failed call.
 void check(int rc, const char* what);

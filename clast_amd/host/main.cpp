@@ -3,6 +3,7 @@
 #include <ctime>
 #include <fstream>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
@@ -23,15 +24,24 @@ int main(int argc, const char** argv) {
       std::vector<Fasta> qf;
       qloader.load(qf);
       ResultHolder holder(qf);
+      // batched: the query set once per query chunk, built on the device
+      std::unique_ptr<clast_query_s, int (*)(clast_query)> qdev(s.batch ? query_chunk_reads(s, qf) : nullptr,
+                                                                 clast_query_destroy);
       FastaLoader tloader(s.targets, static_cast<int>(s.t_ram), true);
       while (tloader.file_index() != -1) {
         std::vector<Fasta> tf;
         tloader.load(tf);
-        SeqList t, q;
-        t.lmer = q.lmer = s.lmer;
+        SeqList t;
+        t.lmer = s.lmer;
         t.add_targets(tf);
-        q.add_queries(qf);
-        search(s, t, q, holder);
+        if (s.batch) {
+          search_batched(s, t, qdev.get(), holder);
+        } else {
+          SeqList q;
+          q.lmer = s.lmer;
+          q.add_queries(qf);
+          search(s, t, q, holder);
+        }
       }
       std::cout << " Fixing and printing the results." << std::endl;
       holder.fix();
