@@ -232,15 +232,22 @@ static bool dp16_disabled() {
   const char* f = std::getenv("CLAST_DP16");
   return f && !std::strcmp(f, "0");
 }
-// CLAST_DP_BAND=1 sends the longest extensions (at least kBandRows rows and
-// over twice one SIMD's average share) to the lane-per-band kernel
-// (k_dp_band); same results.  Off by default: it spends about five times
-// the instructions per cell to finish single long tasks sooner, which lost
-// when searches share the GPU (serving latency p50 and p99 both 16% worse)
-// and was even on long-read batches.
-static bool band_enabled() {
+// Which long extensions (at least kBandRows rows) run on the lane-per-band
+// kernel (k_dp_band: about five times the instructions per cell, a long task
+// finishing sooner), from CLAST_DP_BAND: off (default, 0) none; rest (1)
+// those k_dp16 cannot take (bands that leave the target sequence, star / X
+// letters, local mode), which otherwise run alone in one int32 lane; all (2)
+// every extension over twice one SIMD's average share of rows.  Every
+// setting gives the same rows.  Measured: serving p99 averaged about 12%
+// higher with either of the others (2-3 runs each; the kernel's raised wave
+// priority takes issue slots from the 16-bit DP of searches beside it), and
+// off had the best long-read batch throughput.
+static int band_mode() {
   const char* f = std::getenv("CLAST_DP_BAND");
-  return f && !std::strcmp(f, "1");
+  if (!f || !std::strcmp(f, "off")) return 0;
+  if (!std::strcmp(f, "rest")) return 1;
+  if (!std::strcmp(f, "all")) return 2;
+  throw ClastError(CLAST_ERR_ARG, "CLAST_DP_BAND must be off, rest or all");
 }
 
 // Cell lookup over the top B key bits of the sorted entries (keys in ek,
@@ -630,6 +637,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
     throw ClastError(CLAST_ERR_ARG, "params differ from those the index/query were built with");
   if (ix.device != q.device) throw ClastError(CLAST_ERR_ARG, "index and query on different devices");
+  const int band_env = band_mode();  // read (and checked) before anything is queued
   set_device(ix.device);
   Runtime& rt = runtime();
   std::lock_guard<std::mutex> lk(rt.mutex());
@@ -893,21 +901,24 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     // class bits on top of the rows (two with long tasks possible, else one:
     // classes 1 and 2 only), within the radix passes the rows need
     const bool packed = !p.local && !dp16_disabled();
-    const bool band = maxrows >= kBandRows && band_enabled();
+    const int bmode = maxrows >= kBandRows ? band_env : 0;
+    const bool band = bmode != 0;
     const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
     const int classbit = sortbits - (band ? 2 : 1);
     const int rowshift = std::max(0, rowbits - classbit);
     const uint64_t cbase = band ? 0 : 1;  // lowest class in the keys
     uint32_t* split = ws.get<uint32_t>("d.split", 2);
     int* band_rows = ws.get<int>("d.bandrows", 1);
-    if (band) {
+    if (bmode == 2) {
       auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
       k_dp_rows_sum<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, rsum);
       k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * rt.compute_units() * 128ull, band_rows);
+    } else if (bmode == 1) {
+      k_fill<int><<<1, 64, 0, st>>>(band_rows, 1, kBandRows);
     }
     const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
                          ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
-                         band ? 1 : 0, band_rows, maxrows, rowshift, classbit, dk, dporder,
+                         bmode, band_rows, maxrows, rowshift, classbit, dk, dporder,
                          reinterpret_cast<unsigned long long*>(tot64 + 10),
                          reinterpret_cast<unsigned long long*>(tot64 + 20)};
     k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);

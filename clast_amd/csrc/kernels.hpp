@@ -2109,7 +2109,8 @@ void launch_dp16(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
 // DP task order by class (key bits from classbit up): 0 the long tasks
-// (k_dp_band), 1 those k_dp16 can take, 2 the rest (k_dp); each class by
+// (k_dp_band: with band = 1 the long tasks k_dp16 cannot take, with band = 2
+// every long task), 1 those k_dp16 can take, 2 the rest (k_dp); each class by
 // rows descending (key low bits = (maxrows - rows) >> rowshift, the shift
 // keeping the class bits inside the same radix passes), so the longest
 // extensions start first and a wavefront's lanes run near-equal row counts.
@@ -2124,7 +2125,7 @@ struct TaskKeyArgs {
   const uint64_t* wslot;
   const uint8_t* tclean;   // per target sequence: no star / X class
   int lmer, gap, packed;   // packed = 0: no task goes to k_dp16 (local mode)
-  int band;                // band = 0: no task goes to k_dp_band
+  int band;                // 0: no task goes to k_dp_band; 1: long tasks k_dp16 cannot take; 2: every long task
   const int* band_rows;    // tasks with at least this many rows go to k_dp_band
   int maxrows, rowshift, classbit;
   uint64_t* key;
@@ -2136,7 +2137,7 @@ struct TaskKeyArgs {
 static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
   __shared__ uint32_t wsum[2 * (kBlock / 64)];
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  bool p16 = false;
+  bool p16 = false, long_task = false;
   int rows = 0;
   if (t < 2 * a.nhit) {
     const uint32_t i = t >> 1;
@@ -2144,8 +2145,7 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
     const int s = a.hs[i], q = a.hq[i];
     const int qLen = a.qlen[s];
     rows = bwd ? qLen - (q + a.lmer) : q;
-    const bool long_task = a.band && rows >= *a.band_rows;
-    if (!long_task && a.packed && rows > 0 && qLen <= kPacked16MaxLen && a.qclean[s]) {
+    if (a.packed && rows > 0 && qLen <= kPacked16MaxLen && a.qclean[s]) {
       const int gs = a.hgs[i], tIdx = a.ht[i];
       if (a.tclean[gs]) {
         // the band covers target indices tb + ts * [0, rows + W - 2] (k_dp's ring)
@@ -2159,13 +2159,14 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
         p16 = first >= lo && last < hi;
       }
     }
+    long_task = a.band && rows >= *a.band_rows && (a.band == 2 || !p16);
+    if (long_task) p16 = false;
     const uint64_t cls = long_task ? 0u : p16 ? 1u : 2u;
     a.key[t] = ((cls - (a.band ? 0u : 1u)) << a.classbit) | static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
     a.task[t] = t;
   }
-  const bool lt = a.band && rows >= *a.band_rows;
   const uint32_t r16 = wave_sum_incl(p16 ? static_cast<uint32_t>(rows) : 0u);
-  const uint32_t rb = wave_sum_incl(lt ? static_cast<uint32_t>(rows) : 0u);
+  const uint32_t rb = wave_sum_incl(long_task ? static_cast<uint32_t>(rows) : 0u);
   if (lane_id() == 63) {
     wsum[wave_id()] = r16;
     wsum[kBlock / 64 + wave_id()] = rb;

@@ -189,8 +189,8 @@ int clast_hits_stage_times(clast_hits h, float* out8);
  * entering the DP, [2] DP cells (band width x rows, both directions), [3]
  * result rows, [4] DP tasks (hit extensions, one per direction) run in packed
  * 16-bit lanes, [5] their cells (included in [2]), [6] DP tasks of at least
- * 1024 rows run one lane per band (only with CLAST_DP_BAND=1 in the
- * environment), [7] their cells (included in [2]). */
+ * 1024 rows run one lane per band (none unless CLAST_DP_BAND=rest or all in
+ * the environment), [7] their cells (included in [2]). */
 int clast_hits_counts(clast_hits h, int64_t* out8);
 int clast_hits_destroy(clast_hits h);
 

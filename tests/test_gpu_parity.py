@@ -319,8 +319,10 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     # lane group, interleaved a step apart); shorter ones take the other DP
     # kernels.  Reads of 1-6 kbp, some with star or X letters, some from
     # sequence ends (bands that leave the target), in one set.  Rows equal
-    # the oracle's with the lane-per-band path on (CLAST_DP_BAND=1) and off
-    # (the default).
+    # the oracle's with every long extension on the lane-per-band kernel
+    # (CLAST_DP_BAND=all), with only those the 16-bit kernel cannot take
+    # (rest: here the reads with X letters and those running off a sequence
+    # end; in local mode all), and with none (off, the default).
     t, _ = workload(n_t=3, t_len=50_000, n_r=1, seed=13)
     long_r = gen.make_reads(gen.ReadSpec(31, 16, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=6000,
                                          sub_rate=0.02, indel_rate=0.01), t)
@@ -342,19 +344,29 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     ix = native.Index(p, tl, twins)
     q = native.Query(p, ql, qwins)
     print(f"band local={local} gap={gap}: {len(qwins)} query windows", flush=True)
+    runs = {}
+    for mode in ("all", "rest", "off", None):
+        if mode is None:
+            monkeypatch.delenv("CLAST_DP_BAND", raising=False)
+        else:
+            monkeypatch.setenv("CLAST_DP_BAND", mode)
+        h = native.search_batch(p, ix, q)
+        runs[mode] = (h.arrays(), h.counts())
     monkeypatch.setenv("CLAST_DP_BAND", "1")
-    h = native.search_batch(p, ix, q)
-    got, cnt = h.arrays(), h.counts()
+    with pytest.raises(native.ClastError, match="CLAST_DP_BAND"):
+        native.search_batch(p, ix, q)
     monkeypatch.delenv("CLAST_DP_BAND")
-    h0 = native.search_batch(p, ix, q)
-    got0, cnt0 = h0.arrays(), h0.counts()
     ix.close()
     q.close()
     o = oracle.search(p, tl, twins, ql, qwins)
     assert len(o["final"]["tID"]) > 0
-    assert cnt[6] > 0 and cnt[7] > 0 and cnt0[6] == 0 and cnt0[2] == cnt[2], f"band tasks {cnt[6]}"
-    assert_rows_equal(got, o["final"], what=f"band local={local} gap={gap}")
-    assert_rows_equal(got0, o["final"], what=f"no band local={local} gap={gap}")
+    c_all, c_rest, c_off, c_def = (runs[m][1] for m in ("all", "rest", "off", None))
+    assert c_all[6] >= c_rest[6] > 0 and c_off[6] == 0 and c_def[6] == 0, (c_all[6], c_rest[6], c_off[6])
+    assert c_all[2] == c_rest[2] == c_off[2] == c_def[2]
+    if local:
+        assert c_rest[6] == c_all[6]  # no 16-bit kernel in local mode
+    for mode, (got, _) in runs.items():
+        assert_rows_equal(got, o["final"], what=f"band mode {mode} local={local} gap={gap}")
 
 
 @pytest.mark.parametrize("lmer", [12, 15])
