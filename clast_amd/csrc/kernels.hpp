@@ -1924,7 +1924,7 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   const uint32_t sb = X.st.next(j);
   const bool bot = b == 0, top = b == W - 1;
   const uint32_t qc = bot ? sb : lQ;
-  const uint32_t tm = 4u * ((X.ti >= X.lo) & (X.ti < X.hi) ? sb : kClsX);  // t_mask
+  const uint32_t tm = 4u * (((X.ti >= X.lo) & (X.ti < X.hi)) ? sb : kClsX);  // t_mask
   const uint32_t T = top ? tm : rT;
   X.ti += X.ts;
   const int v0 = max(rS + (kGapOpen + kGapExt), rV + kGapExt);
