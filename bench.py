@@ -94,6 +94,67 @@ def measured_traffic(kernel: str):
     return None
 
 
+def full_size_checks(p, args, tchunks, qchunks, indexes, queries) -> dict:
+    """Size-independent checks on every row of one full step (untimed): the
+    rows of each search with the bench's threads equal those one search at a
+    time; rows are in the reference's output order (query pair ascending,
+    score descending: CHostResultHolder.cu:100-109), no (target, tIdx, query,
+    qIdx) repeats, each E-value is K D qLen exp(-lambda score)
+    (krnlCalculateEvalue.cu:18-21) within 1e-9, survivors pass the bad-hit
+    filters (E <= cutOff, qHL >= lMer + stride; ...deleteBadHits.cu), and
+    hits lie inside their sequences with match <= both lengths."""
+    from clast_amd import native
+
+    keys = [(qi, ii) for qi in range(len(queries)) for ii in range(len(indexes))]
+
+    def collect(streams):
+        res = {}
+
+        def run(part):
+            for qi, ii in part:
+                h = native.search_batch(p, indexes[ii], queries[qi])
+                res[(qi, ii)] = h.arrays()
+                h.close()
+
+        parts = [keys[k::streams] for k in range(streams)]
+        with ThreadPoolExecutor(max_workers=streams) as ex:
+            list(ex.map(run, parts))
+        return res
+
+    many, one = collect(max(args.streams, 1)), collect(1)
+    fields = native.HIT_FIELDS + ("evalue", "twin", "qwin")
+    threads_equal = all(np.array_equal(many[k][f], one[k][f]) for k in keys for f in fields)
+    # window -> (target chunk, window in chunk) for a merged index
+    wchunk = [c for c, (_, w) in enumerate(tchunks) for _ in w]
+    rows = 0
+    order = unique = filters = bounds = True
+    emax = 0.0
+    for qi, ii in keys:
+        a = one[(qi, ii)]
+        n = len(a["tID"])
+        rows += n
+        if n == 0:
+            continue
+        pair, sc = a["qID"] // 2, a["score"].astype(np.int64)
+        order &= bool(np.all((pair[1:] > pair[:-1]) | ((pair[1:] == pair[:-1]) & (sc[1:] <= sc[:-1]))))
+        tup = np.stack([a["twin"], a["tID"], a["tIdx"], a["qID"], a["qIdx"]], axis=1)
+        unique &= len(np.unique(tup, axis=0)) == n
+        qlen = qchunks[qi][0].length[a["qID"]].astype(np.float64)
+        exp_e = p.K * p.total_db_size * qlen * np.exp(-p.lam * sc.astype(np.float64))
+        emax = max(emax, float(np.max(np.abs(a["evalue"] - exp_e) / np.maximum(exp_e, 1e-300))))
+        filters &= bool(np.all(a["qHL"] >= p.lmer + p.stride))
+        if p.cut_off != -1:
+            filters &= bool(np.all(a["evalue"] <= p.cut_off))
+        chunk = np.array([wchunk[w] for w in a["twin"]]) if len(indexes) == 1 else np.full(n, ii)
+        tlen = np.array([tchunks[c][0].length[t] for c, t in zip(chunk, a["tID"])], dtype=np.int64)
+        bounds &= bool(np.all((a["qIdx"] >= 0) & (a["qIdx"] + a["qHL"] <= qlen) & (a["tIdx"] >= 0) &
+                              (a["tIdx"].astype(np.int64) + a["tHL"] <= tlen) &
+                              (a["match"] <= np.minimum(a["qHL"], a["tHL"]))))
+    return {"rows": rows, "threads_equal_one_at_a_time": bool(threads_equal), "output_order": bool(order),
+            "no_repeated_rows": bool(unique), "evalue_max_rel": emax, "bad_hit_filters": bool(filters),
+            "within_sequences": bool(bounds)}
+
+
 def cpu_baseline(p, tchunks, qchunks, n_twin_total: int, threads: int):
     """The oracle (OpenMP, one query window per thread) on one target window
     x `threads` query windows.  Like the GPU figure it counts the searches
@@ -183,6 +244,8 @@ def main() -> int:
                     help="host threads issuing a step's searches (each with its own library streams)")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-checks", action="store_true",
+                    help="skip the untimed property checks over every row of one full step")
     ap.add_argument("--plain", action="store_true", help="also time the per-window (one launch chain per unit) path")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--index", choices=("merged", "per-chunk"), default="merged",
@@ -383,6 +446,10 @@ def main() -> int:
                     ix.close()
         plain_s = time.perf_counter() - tp
         out["plain_value"] = round(qchunks[0][2] / 1e6 / plain_s, 3)
+    if not args.no_full_checks:
+        t2 = time.perf_counter()
+        out["full_size_checks"] = full_size_checks(p, args, tchunks, qchunks, indexes, queries)
+        log(f"full-size checks in {time.perf_counter() - t2:.1f} s: {out['full_size_checks']}")
     if not args.no_cpu_baseline:
         cb, parity = cpu_baseline(p, tchunks, qchunks, n_twin, args.cpu_threads)
         out["cpu_baseline"] = cb
