@@ -779,16 +779,19 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* sval = nullptr;
     if (novf > 0) {
       if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
-      uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", np);
-      off2 = ws.get<uint32_t>("g.off", np);
-      k_ovf_counts<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, ovf_flag, cellp, cnt2);
-      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, np, off2, tot64 + 1, false);
+      // the listed strands' seeds in strand order (off2[s]: strand s's first
+      // seed, before and after the sort, which keys on the strand first)
+      uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", q.nseq);
+      off2 = ws.get<uint32_t>("g.off", q.nseq);
+      k_ovf_strand_counts<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, scnt, q.nseq, cnt2);
+      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, q.nseq, off2, tot64 + 1, false);
       uint64_t t2 = 0;
       rt.read_scalars(tot64 + 1, &t2, 1);
+      if (t2 >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
       ng = static_cast<uint32_t>(t2);
       skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
       sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
-      k_seed_write<<<grid_for(np, kBlock), kBlock, 0, st>>>(a, pk, ovf_flag, cellp, off2, skey, sval);
+      k_seed_write<<<grid_for(novf, kSegWaves), kBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2, skey, sval);
       rt.check_launch("k_seed_write");
     }
     if (!keep_stages) {
@@ -852,18 +855,15 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         k_seed_dup<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, p.width, p.gap, kidx, lastk, gstart, dflag);
         rt.check_launch("k_seed_dup");
         device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, ng, hoff, tot32 + 2, false);
-        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, q.pos_off.ptr, soff, off2, ss, sgs,
-                                             stt, sq);
+        k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, soff, off2, ss, sgs, stt, sq);
         rt.check_launch("k_hits_scatter");
         // per-strand hit counts of the listed strands (nh2 read on device)
         uint32_t nh2 = 0;
         rt.read_scalars(tot32 + 2, &nh2, 1);
-        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.pos_off.ptr, np, off2, ng, hoff, nh2,
-                                                               hcnt);
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, ng, hoff, nh2, hcnt);
         mark(EV_FILT1);
       } else {
-        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.pos_off.ptr, np, off2, 0, hcnt, 0,
-                                                               hcnt);
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, 0, hcnt, 0, hcnt);
       }
     }
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);

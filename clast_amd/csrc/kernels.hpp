@@ -1076,45 +1076,52 @@ struct SeedDecode {
   }
 };
 
-// Entry counts of the positions of listed strands only.
-static __global__ __launch_bounds__(kBlock) void k_ovf_counts(SeedArgs a, const uint8_t* __restrict__ ovf_flag,
-                                                              const uint4* __restrict__ cellp,
-                                                              uint32_t* __restrict__ cnt2) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= a.npos) return;
-  cnt2[p] = ovf_flag[a.pos_strand[p]] ? cellp[p].y : 0u;
+// Raw seed counts of the strands of the global path (ovf_flag), 0 for the
+// others: scanned, strand s's seeds start at off[s] in strand order.
+static __global__ __launch_bounds__(kBlock) void k_ovf_strand_counts(const uint8_t* __restrict__ ovf_flag,
+                                                                     const uint32_t* __restrict__ scnt, uint32_t nseq,
+                                                                     uint32_t* __restrict__ cnt) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < nseq) cnt[s] = ovf_flag[s] ? scnt[s] : 0u;
 }
 
-// Writes the seeds of each listed position at its scanned offset, in
-// query-position then cell order (krnlWriteSeedList.cu:3-37 order), and
-// records the strands as unit candidates like k_seed_segments.
-static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk, const uint8_t* __restrict__ ovf_flag,
+// One wavefront per listed strand (any order) writes its seeds from
+// off[s] on, in query-position then cell order (krnlWriteSeedList.cu:3-37
+// order), and records the strand as a unit candidate like k_seed_segments.
+static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk,
+                                                              const uint32_t* __restrict__ list, uint32_t nlist,
                                                               const uint4* __restrict__ cellp,
                                                               const uint32_t* __restrict__ off,
                                                               uint64_t* __restrict__ skey,
                                                               uint32_t* __restrict__ sval) {
-  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= a.npos) return;
-  const uint32_t s = a.pos_strand[p];
-  if (!ovf_flag[s]) return;
-  const uint4 cp = cellp[p];
-  if (cp.y == 0) return;
-  const int qIdx = static_cast<int>(p - a.pos_off[s]);
+  const uint32_t k = blockIdx.x * kSegWaves + __builtin_amdgcn_readfirstlane(wave_id());
+  if (k >= nlist) return;
+  const int lane = lane_id();
+  const uint32_t s = list[k];
+  const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
   const int qLen = a.qlen[s];
   const int qw = a.qwin[s];
   const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
-  uint32_t o = off[p];
-  for (uint32_t j = 0; j < cp.y; ++j, ++o) {
-    const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
-    const int d = e.y - qIdx;
-    const int2 ti = a.tinfo[e.x];
-    if (corner_ok(a, d, qLen, ti.x)) {
-      skey[o] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
-      note_unit_first(a, ti.y, qw, s);
-    } else {
-      skey[o] = sk | ((1ull << (pk.gs_bits + pk.d_bits)) - 1ull);
+  uint32_t base = off[s];
+  for (uint32_t c0 = p0; c0 < p1; c0 += 64) {
+    const uint32_t p = c0 + static_cast<uint32_t>(lane);
+    const uint4 cp = p < p1 ? cellp[p] : make_uint4(0, 0, 0, 0);
+    const uint32_t incl = wave_sum_incl(cp.y);
+    const int qIdx = static_cast<int>(p - p0);
+    uint32_t o = base + incl - cp.y;
+    for (uint32_t j = 0; j < cp.y; ++j, ++o) {
+      const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
+      const int d = e.y - qIdx;
+      const int2 ti = a.tinfo[e.x];
+      if (corner_ok(a, d, qLen, ti.x)) {
+        skey[o] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
+        note_unit_first(a, ti.y, qw, s);
+      } else {
+        skey[o] = sk | ((1ull << (pk.gs_bits + pk.d_bits)) - 1ull);
+      }
+      sval[o] = static_cast<uint32_t>(qIdx);
     }
-    sval[o] = static_cast<uint32_t>(qIdx);
+    base += static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
   }
 }
 
@@ -1192,7 +1199,6 @@ static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* 
                                                                 const uint32_t* __restrict__ sval, uint32_t ns,
                                                                 SeedDecode dc, const uint32_t* __restrict__ dflag,
                                                                 const uint32_t* __restrict__ hoff,
-                                                                const uint32_t* __restrict__ pos_off,
                                                                 const uint32_t* __restrict__ soff,
                                                                 const uint32_t* __restrict__ goff,
                                                                 int32_t* __restrict__ hs, int32_t* __restrict__ hgs,
@@ -1201,7 +1207,8 @@ static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* 
   if (i >= ns || !dflag[i]) return;
   const uint64_t k = skey[i];
   const uint32_t s = dc.s(k);
-  const uint32_t o = soff[s] + (hoff[i] - hoff[goff[pos_off[s]]]);
+  // goff[s]: strand s's first seed, in strand order before and after the sort
+  const uint32_t o = soff[s] + (hoff[i] - hoff[goff[s]]);
   const int q = static_cast<int>(sval[i]);
   hs[o] = static_cast<int32_t>(s);
   hgs[o] = static_cast<int32_t>(dc.gs(k));
@@ -1210,21 +1217,17 @@ static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* 
 }
 
 // Hit counts of the global-path strands: surviving seeds between the
-// strand's first global-path seed and the next strand's.
+// strand's first global-path seed (goff[s], strand order) and the next
+// strand's.
 static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __restrict__ ovf, uint32_t novf,
-                                                            const uint32_t* __restrict__ pos_off, uint32_t npos,
-                                                            const uint32_t* __restrict__ goff, uint32_t ng,
-                                                            const uint32_t* __restrict__ hoff, uint32_t nh,
+                                                            uint32_t nseq, const uint32_t* __restrict__ goff,
+                                                            uint32_t ng, const uint32_t* __restrict__ hoff, uint32_t nh,
                                                             uint32_t* __restrict__ hcnt) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= novf) return;
   const uint32_t s = ovf[k];
-  if (pos_off[s] == pos_off[s + 1]) {  // no k-mer position (the list may hold every strand)
-    hcnt[s] = 0;
-    return;
-  }
-  const uint32_t g0 = goff[pos_off[s]];
-  const uint32_t g1 = pos_off[s + 1] < npos ? goff[pos_off[s + 1]] : ng;
+  const uint32_t g0 = goff[s];
+  const uint32_t g1 = s + 1u < nseq ? goff[s + 1u] : ng;
   const uint32_t h0 = g0 < ng ? hoff[g0] : nh;
   const uint32_t h1 = g1 < ng ? hoff[g1] : nh;
   hcnt[s] = h1 - h0;
