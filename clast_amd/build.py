@@ -75,7 +75,7 @@ def build_gen() -> str:
 
 def build_host() -> str:
     """libclast_host.so (driver code as a library), the clast executable and
-    the preprocessDB tool."""
+    the preprocessDB and devideQuery tools."""
     os.makedirs(BIN, exist_ok=True)
     srcs = [os.path.join(HOST, u) for u in HOST_UNITS]
     lib = os.path.join(LIB, "libclast_host.so")
@@ -86,9 +86,10 @@ def build_host() -> str:
     _run([*common, os.path.join(HOST, "main.cpp"), f"-L{LIB}", "-lclast_host", "-lclast_hip",
           "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
     os.replace(exe + ".tmp", exe)
-    pre = os.path.join(BIN, "preprocessDB")
-    _run(["g++", "-O3", "-std=c++17", os.path.join(HOST, "preprocess_db.cpp"), "-o", pre + ".tmp"])
-    os.replace(pre + ".tmp", pre)
+    for src, name in (("preprocess_db.cpp", "preprocessDB"), ("divide_query.cpp", "devideQuery")):
+        tool = os.path.join(BIN, name)
+        _run(["g++", "-O3", "-std=c++17", os.path.join(HOST, src), "-o", tool + ".tmp"])
+        os.replace(tool + ".tmp", tool)
     return exe
 
 

@@ -1,7 +1,8 @@
 """Host-side code without a GPU: the C-ABI library loads and exports every
 declared entry point, the host list layout (and its gateway computed from
 read lengths alone), windows, FASTA chunking, the driver's flag handling,
-the preprocessDB splitter and the checks of a persisted index file."""
+the preprocessDB and devideQuery tools and the checks of a persisted index
+file."""
 from __future__ import annotations
 
 import ctypes
@@ -282,6 +283,45 @@ def test_preprocess_db_arguments(tmp_path):
     (tmp_path / "empty.fa").write_text("ACGT\n")
     assert run_pre(["empty.fa", "o.fa", "10", "2"], tmp_path).returncode == 0
     assert not (tmp_path / "o.fa").exists()
+
+
+DIV = os.path.join(ROOT, "clast_amd", "bin", "devideQuery")
+
+
+def test_devide_query_known_parts(tmp_path):
+    # devideQuery.cpp's greedy rule worked by hand: each record, in file
+    # order, goes to the part with the smallest total so far, the lowest
+    # index on ties; written as label, sequence on one line, empty line
+    (tmp_path / "q.fa").write_text(">a\nAC GTA\n>b\nCCC\n\n>c\nGG\nGG\n>d\nT\n")
+    r = subprocess.run([DIV, "2", "q.fa"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout
+    # totals: a(5) -> 0 [5, 0]; b(3) -> 1 [5, 3]; c(4) -> 1 [5, 7]; d(1) -> 0 [6, 7]
+    assert (tmp_path / "q.fa.0").read_text() == ">a\nACGTA\n\n>d\nT\n\n"
+    assert (tmp_path / "q.fa.1").read_text() == ">b\nCCC\n\n>c\nGGGG\n\n"
+    for args in ([], ["0", "q.fa"], ["2"], ["x", "q.fa"]):
+        r = subprocess.run([DIV, *args], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=60)
+        assert r.returncode != 0
+
+
+def test_devide_query_round_trip(tmp_path):
+    # every record lands in exactly one part, unchanged; parts differ in
+    # total length by at most the longest record
+    rng = np.random.default_rng(3)
+    recs = [(f">r{i}", "".join(rng.choice(list("ACGT"), size=int(n)))) for i, n in
+            enumerate(rng.integers(1, 400, size=300))]
+    with open(tmp_path / "q.fa", "w") as f:
+        for lab, s in recs:
+            f.write(lab + "\n" + "\n".join(s[k:k + 60] for k in range(0, len(s), 60)) + "\n")
+    assert subprocess.run([DIV, "7", "q.fa"], cwd=tmp_path, timeout=60).returncode == 0
+    got, totals = [], []
+    for k in range(7):
+        blocks = [b.split("\n") for b in (tmp_path / f"q.fa.{k}").read_text().split("\n\n") if b]
+        got += [(b[0], b[1]) for b in blocks]
+        totals.append(sum(len(b[1]) for b in blocks))
+    assert sorted(got) == sorted(recs)
+    assert max(totals) - min(totals) <= max(len(s) for _, s in recs)
 
 
 def test_poisson_requests_partition_reads():
