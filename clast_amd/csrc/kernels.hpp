@@ -15,6 +15,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "prims.hpp"
 
@@ -1692,15 +1693,15 @@ struct PairRows {
 // both gap terms then read it unchanged, and the diagonal term is
 // S + (4 mc - 3) = Z + 4 + 4 mc.  T[b-1] and Q hold the target and query
 // codes of both halves; mc = 1 - min(t ^ q, 1).
-template <int W>
+template <int W, int O>
 __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
-                                         const uint32_t (&T)[W], uint32_t Q) {
+                                         const uint32_t (&T)[W + 4], uint32_t Q) {
   s16x2 hp = pk_splat(-30000);
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
     const s16x2 v = __builtin_elementwise_max(Z[b + 1], V[b + 1] + pk_splat(kGapExt));
     const s16x2 h = __builtin_elementwise_max(Z[b - 1], hp + pk_splat(kGapExt));
-    const u16x2 x = __builtin_bit_cast(u16x2, T[b - 1] ^ Q);
+    const u16x2 x = __builtin_bit_cast(u16x2, T[O + b - 1] ^ Q);
     const s16x2 mc = __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(u16x2{1, 1}, x));
     const s16x2 pt = pk_mad4(mc, Z[b]) + pk_splat(kMatch - kMismatch);
     const s16x2 vh = __builtin_elementwise_max(v, h);
@@ -1791,7 +1792,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   }
   // target ring: stream positions kLead .. kLead + W - 1
   PairRows tp, qp;
-  uint32_t T[W];
+  uint32_t T[W + 4];
 #pragma unroll
   for (int g = 0; g < kInitGroups; ++g) {
     uint32_t a0, a1, b0, b1;
@@ -1815,22 +1816,40 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   // 32 rows per iteration: even group, 16 rows, odd group, 16 rows (each
   // stream register then has one definition in the loop, so the loads land
   // in place); every active lane is at the same row
+  // Rows go U at a time, unrolled: row i of U reads the target ring from
+  // T[i] and appends its new position at T[W + i], so the ring moves back by
+  // U once per U rows and, with U = 4, the code words shift at fixed rows.
+  // Wider bands keep U = 1 (four rows' ring would not fit the registers).
+  constexpr int U = G <= 8 ? 4 : 1;
+  auto row = [&](auto oi, int j, int r) {
+    constexpr int i = decltype(oi)::value;
+    const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
+    T[W + i] = tp.next(j);          // target position r + W + kLead = r (mod 16)
+    dp_row16<W, i>(Z, V, M, T, Q);
+    if (r + 1 == rsplit) {
+      const int h = rows[0] == rsplit ? 0 : 1;
+      dp16_end<G>(Z, M, a.dres, h ? task1 : task0, h, rsplit);
+    }
+  };
   auto rows16 = [&](int r0) {
 #pragma unroll 1
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < 16; j += U) {
       const int ju = __builtin_amdgcn_readfirstlane(j);
       const int r = r0 + ju;
       if (r >= rmax) break;
-      const uint32_t Q = qp.next(ju);
-      const uint32_t tn = tp.next(ju);  // target position r + W + kLead = r (mod 16)
-      dp_row16<W>(Z, V, M, T, Q);
-#pragma unroll
-      for (int k = 0; k < W - 1; ++k) T[k] = T[k + 1];
-      T[W - 1] = tn;
-      if (r + 1 == rsplit) {
-        const int h = rows[0] == rsplit ? 0 : 1;
-        dp16_end<G>(Z, M, a.dres, h ? task1 : task0, h, rsplit);
+      if constexpr (U == 1) {
+        row(std::integral_constant<int, 0>{}, ju, r);
+      } else {
+        row(std::integral_constant<int, 0>{}, 0, r);
+        if (r + 1 >= rmax) break;
+        row(std::integral_constant<int, 1>{}, 1, r + 1);
+        if (r + 2 >= rmax) break;
+        row(std::integral_constant<int, 2>{}, 2, r + 2);
+        if (r + 3 >= rmax) break;
+        row(std::integral_constant<int, 3>{}, 3, r + 3);
       }
+#pragma unroll
+      for (int k = 0; k < W; ++k) T[k] = T[k + U];
     }
   };
 #pragma unroll 1

@@ -236,16 +236,19 @@ def test_result_sort():
 
 
 def test_long_reads_mix_fast_and_global_paths():
-    # at stride 2: 6-10 kbp reads carry > 2048 seeds on their true strand and
-    # take the global path, 0.8-1.5 kbp reads (> 256 seeds) the big LDS pass,
+    # at stride 2 (about one seed per two bases on the true strand): 6-10 kbp
+    # reads carry > 2048 seeds and take the global path, 2.5-3.8 kbp reads
+    # (1024-2048 seeds) and 0.8-1.5 kbp reads (256-1024) the big LDS pass,
     # 150 bp reads the first LDS pass, all in one query set
     t, _ = workload(n_t=3, t_len=60_000, n_r=1)
     long_r = gen.make_reads(gen.ReadSpec(21, 12, len_mode=gen.LEN_LOGUNIFORM, len_a=6000, len_b=10_000,
                                          sub_rate=0.01, indel_rate=0.002), t)
+    big_r = gen.make_reads(gen.ReadSpec(24, 10, len_mode=gen.LEN_LOGUNIFORM, len_a=2500, len_b=3800,
+                                        sub_rate=0.01, indel_rate=0.002), t)
     mid_r = gen.make_reads(gen.ReadSpec(23, 20, len_mode=gen.LEN_LOGUNIFORM, len_a=800, len_b=1500,
                                         sub_rate=0.02, indel_rate=0.005), t)
     short_r = gen.make_reads(gen.ReadSpec(22, 80, sub_rate=0.03, indel_rate=0.01), t)
-    parts = [long_r, mid_r, short_r]
+    parts = [long_r, big_r, mid_r, short_r]
     bases = np.concatenate([x.bases for x in parts])
     off = [np.zeros(1, np.int64)]
     for x in parts:
@@ -255,11 +258,11 @@ def test_long_reads_mix_fast_and_global_paths():
         p = native.Params(total_db_size=db_size(t), local=local, stride=2)
         tl, ql = lists(t, r, p.lmer)
         ix = native.Index(p, tl, [(0, 3)])
-        q = native.Query(p, ql, [(0, 56), (56, 112)])
+        q = native.Query(p, ql, [(0, 61), (61, 122)])
         print(f"long reads local={local}", flush=True)
         got = native.search_batch(p, ix, q)
         assert got.times()[7] > 3000  # seeds
-        o = oracle.search(p, tl, [(0, 3)], ql, [(0, 56), (56, 112)])
+        o = oracle.search(p, tl, [(0, 3)], ql, [(0, 61), (61, 122)])
         assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
 
 
