@@ -783,6 +783,16 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t k) {
   if constexpr (KK < N) return wave_sort<N, KK * 2>(k);
   else return k;
 }
+// Stages J .. 1 of a bitonic merge over the wavefront's 64 lanes (a bitonic
+// sequence in, sorted out), ascending when asc.
+template <int J>
+__device__ __forceinline__ uint64_t wave_merge(uint64_t k, int lane, bool asc) {
+  const uint64_t o = xor_lane64<J>(k);
+  const bool keep_min = ((lane & J) == 0) == asc;
+  k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+  if constexpr (J > 1) return wave_merge<J / 2>(k, lane, asc);
+  else return k;
+}
 
 // Isolate-filter flags of one sorted seed v with predecessor pv (has_pred:
 // v is not the list's first): group head (new target sequence), window head
@@ -953,32 +963,53 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
     while (np < m) np <<= 1;
     for (uint32_t i = m + lane; i < np; i += 64) seg_none(L, w, i);
     wave_lds_sync();
-    // bitonic sort ascending
-    for (uint32_t kk = 2; kk <= np; kk <<= 1) {
-      for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = lane; i < np; i += 64) {
-          const uint32_t x = i ^ j;
-          if (x > i) {
-            const uint64_t ka = L.k[w][i], kb = L.k[w][x];
-            bool gt;
-            if constexpr (PACKED) {
-              gt = ka > kb;
-            } else {
-              gt = ka > kb || (ka == kb && L.q[w][i] > L.q[w][x]);
-            }
-            if (gt == ((i & kk) == 0)) {
-              L.k[w][i] = kb;
-              L.k[w][x] = ka;
-              if constexpr (!PACKED) {
-                const int qa = L.q[w][i];
-                L.q[w][i] = L.q[w][x];
-                L.q[w][x] = qa;
-              }
-            }
+    // Bitonic sort ascending.  One compare-exchange of stage (kk, j) per
+    // lane and step: pair p is elements i = p with a zero inserted at bit j
+    // and i | j, ascending when bit kk of i is clear.
+    auto lds_stage = [&](uint32_t kk, uint32_t j) {
+      for (uint32_t p = lane; p < np / 2; p += 64) {
+        const uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u)), x = i | j;
+        const uint64_t ka = L.k[w][i], kb = L.k[w][x];
+        bool gt;
+        if constexpr (PACKED) {
+          gt = ka > kb;
+        } else {
+          gt = ka > kb || (ka == kb && L.q[w][i] > L.q[w][x]);
+        }
+        if (gt == ((i & kk) == 0)) {
+          L.k[w][i] = kb;
+          L.k[w][x] = ka;
+          if constexpr (!PACKED) {
+            const int qa = L.q[w][i];
+            L.q[w][i] = L.q[w][x];
+            L.q[w][x] = qa;
           }
+        }
+      }
+      wave_lds_sync();
+    };
+    if constexpr (PACKED) {
+      // Stages with j < 64 pair lanes of one 64-seed chunk: they run in
+      // registers.  Levels up to 64 sort each chunk, ascending for even
+      // chunks and descending for odd ones (stored lane-reversed) as bit 64
+      // of the index asks; each later level kk runs its stages j >= 64 in
+      // LDS, then j = 32 .. 1 per chunk in registers.
+      for (uint32_t c = 0; c < np; c += 64) {
+        const uint64_t k = wave_sort<64>(L.k[w][c + lane]);
+        L.k[w][c + ((c & 64u) ? 63u - lane : lane)] = k;
+      }
+      wave_lds_sync();
+      for (uint32_t kk = 128; kk <= np; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j >= 64; j >>= 1) lds_stage(kk, j);
+        for (uint32_t c = 0; c < np; c += 64) {
+          const uint64_t k = wave_merge<32>(L.k[w][c + lane], lane, (c & kk) == 0);
+          L.k[w][c + lane] = k;
         }
         wave_lds_sync();
       }
+    } else {
+      for (uint32_t kk = 2; kk <= np; kk <<= 1)
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) lds_stage(kk, j);
     }
     // filters over the sorted seeds, 64 at a time
     int carry_k = -1, carry_g = -1;
