@@ -791,7 +791,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       ng = static_cast<uint32_t>(t2);
       skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
       sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
-      k_seed_write<<<grid_for(novf, kSegWaves), kBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2, skey, sval);
+      k_seed_write<<<novf, kWriteBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2, skey, sval);
       rt.check_launch("k_seed_write");
     }
     if (!keep_stages) {

@@ -1117,30 +1117,48 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_strand_counts(const uint8
   if (s < nseq) cnt[s] = ovf_flag[s] ? scnt[s] : 0u;
 }
 
-// One wavefront per listed strand (any order) writes its seeds from
-// off[s] on, in query-position then cell order (krnlWriteSeedList.cu:3-37
-// order), and records the strand as a unit candidate like k_seed_segments.
-static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPack pk,
-                                                              const uint32_t* __restrict__ list, uint32_t nlist,
-                                                              const uint4* __restrict__ cellp,
-                                                              const uint32_t* __restrict__ off,
-                                                              uint64_t* __restrict__ skey,
-                                                              uint32_t* __restrict__ sval) {
-  const uint32_t k = blockIdx.x * kSegWaves + __builtin_amdgcn_readfirstlane(wave_id());
-  if (k >= nlist) return;
+// One workgroup per listed strand (any order) writes its seeds from off[s]
+// on, in query-position then cell order (krnlWriteSeedList.cu:3-37 order),
+// and records the strand as a unit candidate like k_seed_segments.  The
+// strands here hold thousands of positions: kWriteBlock of them per round,
+// each seed's slot from a workgroup-wide prefix of the cell counts (wave
+// scans, then the wave totals through LDS, double-buffered so that one
+// barrier per round suffices).
+constexpr int kWriteBlock = 1024;
+static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, SeedPack pk,
+                                                                   const uint32_t* __restrict__ list, uint32_t nlist,
+                                                                   const uint4* __restrict__ cellp,
+                                                                   const uint32_t* __restrict__ off,
+                                                                   uint64_t* __restrict__ skey,
+                                                                   uint32_t* __restrict__ sval) {
+  constexpr int kW = kWriteBlock / 64;
+  __shared__ uint32_t wsum[2][kW];
+  const uint32_t k = blockIdx.x;
+  if (k >= nlist) return;  // the whole workgroup
   const int lane = lane_id();
+  const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const uint32_t s = list[k];
   const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
   const int qLen = a.qlen[s];
   const int qw = a.qwin[s];
   const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
   uint32_t base = off[s];
-  for (uint32_t c0 = p0; c0 < p1; c0 += 64) {
-    const uint32_t p = c0 + static_cast<uint32_t>(lane);
+  int buf = 0;
+  for (uint32_t c0 = p0; c0 < p1; c0 += kWriteBlock) {
+    const uint32_t p = c0 + threadIdx.x;
     const uint4 cp = p < p1 ? cellp[p] : make_uint4(0, 0, 0, 0);
     const uint32_t incl = wave_sum_incl(cp.y);
+    if (lane == 63) wsum[buf][w] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < kW; ++v) {
+      const uint32_t t = wsum[buf][v];
+      before += v < w ? t : 0u;
+      total += t;
+    }
     const int qIdx = static_cast<int>(p - p0);
-    uint32_t o = base + incl - cp.y;
+    uint32_t o = base + before + incl - cp.y;
     for (uint32_t j = 0; j < cp.y; ++j, ++o) {
       const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
       const int d = e.y - qIdx;
@@ -1153,7 +1171,8 @@ static __global__ __launch_bounds__(kBlock) void k_seed_write(SeedArgs a, SeedPa
       }
       sval[o] = static_cast<uint32_t>(qIdx);
     }
-    base += static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+    base += total;
+    buf ^= 1;
   }
 }
 
