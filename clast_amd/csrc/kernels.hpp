@@ -43,9 +43,15 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v) {
       __builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(v), CTRL, ROWS, 0xF, false));
 }
 
-// Value of lane i ^ J (J a power of two): DPP for J <= 8, ds_bpermute above.
+// Value of lane i ^ J (J a power of two; every lane active): DPP for
+// J <= 8; for 16 and 32 the gfx950 row / half swaps, which with both
+// operands v return v of lane i ^ J in the second register of the pair
+// where bit J of i is clear and in the first where it is set (16: odd rows
+// of the first operand trade with even rows of the second; 32: its upper 32
+// lanes with the lower 32 of the second).
 template <int J>
 __device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+  static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane distance");
   if constexpr (J == 1) {
     return dpp_u32<kDppQuadXor1>(v, v);
   } else if constexpr (J == 2) {
@@ -54,8 +60,12 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
     const uint32_t up = dpp_u32<kDppRowShl + J>(v, v);  // lanes with bit J clear read i + J
     const uint32_t dn = dpp_u32<kDppRowShr + J>(v, v);  // lanes with bit J set read i - J
     return (lane_id() & J) ? dn : up;
+  } else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane_id() & J) ? r[0] : r[1];
   } else {
-    return static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), J, 64));
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane_id() & J) ? r[0] : r[1];
   }
 }
 
