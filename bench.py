@@ -98,11 +98,12 @@ def full_size_checks(p, args, tchunks, qchunks, indexes, queries) -> dict:
     """Size-independent checks on every row of one full step (untimed): the
     rows of each search with the bench's threads equal those one search at a
     time; rows are in the reference's output order (query pair ascending,
-    score descending: CHostResultHolder.cu:100-109), no (target, tIdx, query,
-    qIdx) repeats, each E-value is K D qLen exp(-lambda score)
+    score descending: CHostResultHolder.cu:100-109), each E-value is
+    K D qLen exp(-lambda score)
     (krnlCalculateEvalue.cu:18-21) within 1e-9, survivors pass the bad-hit
     filters (E <= cutOff, qHL >= lMer + stride; ...deleteBadHits.cu), and
-    hits lie inside their sequences with match <= both lengths."""
+    hits lie inside their sequences with match <= both lengths; rows that
+    share a (target, tIdx, query, qIdx) start are counted."""
     from clast_amd import native
 
     keys = [(qi, ii) for qi in range(len(queries)) for ii in range(len(indexes))]
