@@ -259,3 +259,188 @@ def test_total_db_size(tmp_path):
     f.write_bytes(b">a\nACGT\r\nAC\n@12\n>b>c\nGG\n")
     # every byte of every line without '>', '\r' and '@' lines included
     assert oracle.total_db_size([str(f)]) == 5 + 2 + 3 + 2
+
+
+# ---------------------------------------------------------------- local-mode DP
+#
+# localAlign{Backward,Forward} (krnlAlignment.cu:312-533): the end node is the
+# best cell over every row and band, scanned row by row and band by band with
+# a strict '<' against a running best that starts at score 0
+# (calcIdx = -1, endPoint = gap); the extension is calcIdx + 1 rows and
+# endPoint - gap target columns.  Each expected tuple below is worked by hand
+# from that rule and the scores M = +1, X = -3, gap = -5 - 2k (common.hpp).
+
+PL = native.Params(local=1)
+
+
+def _other(c: int) -> bytes:
+    return bytes([next(b for b in b"ACGT" if b != c)])
+
+
+def test_local_dp_stops_at_last_match():
+    # 85 exact rows after the seed, then 'N' rows (each -3 on the diagonal,
+    # gaps cost at least 7): the best cell is row 84 on the centre band
+    # (score 85), so the extension is 84 + 1 rows and no target shift.
+    # A lost '+1' gives qHL 99; a best cell taken at the last row gives 150.
+    q = TGT[1000:1100] + b"N" * 50
+    assert align(PL, TGT, q, 1000, 0) == (1000, 0, 100, 100, 100, 100)
+
+
+def test_local_dp_all_negative_rows_extend_nothing():
+    # every cell after the seed is below zero: no cell beats the initial
+    # best of 0, calcIdx stays -1, so the extension is 0 rows and
+    # endPoint - gap = 0 columns.  Starting the best from -infinity instead
+    # would extend by one row (qHL 16, score 12).
+    q = TGT[1000:1015] + b"N" * 30
+    assert align(PL, TGT, q, 1000, 0) == (1000, 0, 15, 15, 15, 15)
+
+
+def test_local_dp_equal_score_tie_keeps_first_cell():
+    # rows 0..34 exact (centre score 35 at row 34), row 35 a mismatch (32),
+    # rows 36..38 exact (35 again at row 38), then 'N'.  Strict '<' keeps the
+    # first 35 (row 34): 35 rows, score 15 + 35.  With '<=' the later tie
+    # wins: qHL 54, match 53.
+    q = TGT[1000:1050] + _other(TGT[1050]) + TGT[1051:1054] + b"N" * 20
+    assert align(PL, TGT, q, 1000, 0) == (1000, 0, 50, 50, 50, 50)
+
+
+def test_local_dp_forward_across_deletion():
+    # query = 20 N | target 1000..1017 | target 1020..1099 | 20 N (the query
+    # lacks target bases 1018-1019), seed at query 60 <-> target 1042.
+    # Backward: query 75..117 exact (43 rows) then N: +43, no shift.
+    # Forward: 22 exact rows (query 59..38), one 2-base gap (-5 - 2*2 = -9),
+    # 18 exact rows (query 37..20) -> best 22 - 9 + 18 = 31 at row 39 on band
+    # offset +2: 40 rows, 42 target columns.  Totals: tIdx 1042 - 42 = 1000,
+    # qIdx 60 - 40 = 20, tHL 15 + 43 + 42, qHL 15 + 43 + 40, match 15 + 43
+    # + 40, score 15 + 43 + 31 (= 98 matches - 9 for the gap).
+    q = b"N" * 20 + TGT[1000:1018] + TGT[1020:1100] + b"N" * 20
+    assert align(PL, TGT, q, 1042, 60) == (1000, 20, 100, 98, 98, 89)
+
+
+# ---------------------------------------------------------------- result dedup
+#
+# deleteHits_duplicateResult (CDeviceHitList.cu:32-128, is_duplicate
+# CDeviceHitList.cuh:32-49): after resultSorting (stable, query pair
+# ascending, score descending) a row is dropped when its (tID, tIdx, qID,
+# qIdx) equals the row just before it in that order.
+
+def _two_chain_read(rng):
+    """A 3,000-base random target region R = 400 bases whose read carries
+    exact ends (0..139, 260..399) and a middle with a substitution every 10
+    bases (no exact 15-mer there): two seed chains on one diagonal more than
+    `width` apart, each leaving one hit after the isolate and duplicate
+    filters.  Global extension of either hit covers the whole read on the
+    diagonal: tHL = qHL = 400, 388 matches, score 388 - 3 * 12 = 352."""
+    region = bytes(rng.choice(list(b"ACGT"), 400).astype(np.uint8))
+    read = bytearray(region)
+    for i in range(145, 260, 10):
+        read[i] = _other(read[i])[0]
+    return region, bytes(read)
+
+
+def _search_final(p, seqs, read):
+    t = gen.Seqs(np.frombuffer(b"".join(seqs), np.uint8).copy(),
+                 np.concatenate([[0], np.cumsum([len(s) for s in seqs])]).astype(np.int64))
+    r = gen.Seqs(np.frombuffer(read, np.uint8).copy(), np.array([0, len(read)], np.int64))
+    tl, ql = lists(t, r, p.lmer)
+    return oracle.search(p, tl, [(0, len(seqs))], ql, [(0, 1)])["final"]
+
+
+def test_result_dedup_adjacent_duplicate_collapses():
+    rng = np.random.default_rng(21)
+    region, read = _two_chain_read(rng)
+    bg = bytes(rng.choice(list(b"ACGT"), 3000).astype(np.uint8))
+    seq0 = bg[:1000] + region + bg[1400:]
+    p = native.Params(total_db_size=3000.0)
+    f = _search_final(p, [seq0], read)
+    # the two chains' hits extend to one start with one score and sit next to
+    # each other after the result sort: one row survives
+    plus = f["qID"] == 0
+    rows = list(zip(*[f[k][plus].tolist() for k in ("tID", "tIdx", "qIdx", "tHL", "qHL", "match", "score")]))
+    assert rows == [(0, 1000, 0, 400, 400, 388, 352)]
+
+
+def test_result_dedup_keeps_duplicates_that_are_not_adjacent():
+    # The same region copied into a second target sequence at the same stride
+    # phase (slot 3000 + 14 joint + 501 = 3515, a multiple of 5 like 1000), so
+    # both copies give the same seeds.  Chain representatives after the
+    # filters: in tID 0 chain A keeps its first seed (qIdx 0, the window's
+    # element 0) and chain B its second (qIdx 265); in tID 1 both chains
+    # keep their second seed (qIdx 5 and 265).  All four extend to one start
+    # per copy with score 352, so the result order is the DP order: seed qIdx
+    # descending, ties in seed order (tID): B0 B1 A1 A0.  B1 and A1 are
+    # neighbours with equal (tID, tIdx, qID, qIdx): A1 goes.  B0 and A0 are
+    # equal but not neighbours: both stay.  A dedup over all rows would leave
+    # 2 rows, a missed neighbour 4, an order by tID first (B0 A0 B1 A1) 2.
+    rng = np.random.default_rng(21)
+    region, read = _two_chain_read(rng)
+    bg = bytes(rng.choice(list(b"ACGT"), 3000).astype(np.uint8))
+    bg1 = bytes(rng.choice(list(b"ACGT"), 3000).astype(np.uint8))
+    seq0 = bg[:1000] + region + bg[1400:]
+    seq1 = bg1[:501] + region + bg1[901:]
+    assert (3000 + 14 + 501) % 5 == 0 and 1000 % 5 == 0
+    p = native.Params(total_db_size=6000.0)
+    f = _search_final(p, [seq0, seq1], read)
+    plus = f["qID"] == 0
+    rows = list(zip(*[f[k][plus].tolist() for k in ("tID", "tIdx", "qIdx", "tHL", "qHL", "match", "score")]))
+    a0, a1 = (0, 1000, 0, 400, 400, 388, 352), (1, 501, 0, 400, 400, 388, 352)
+    assert rows == [a0, a1, a0]
+
+
+# ---------------------------------------------------------------- printResult
+#
+# CHostResultHolder::printResult (CHostResultHolder.cu:131-243): rows in
+# fixResult order; a row equal in all ten printed fields to the last printed
+# row is skipped and not counted; per query label the first row always
+# prints and at most numOut rows print (numOut -1: all).
+
+def _run_oracle_cli(tmp_path, targets: bytes, queries: bytes, *flags):
+    (tmp_path / "t.fa").write_bytes(targets)
+    (tmp_path / "q.fa").write_bytes(queries)
+    out = tmp_path / "o.tsv"
+    subprocess.run([oracle.EXE, "-t", str(tmp_path / "t.fa"), "-q", str(tmp_path / "q.fa"), "-o", str(out),
+                    *[str(f) for f in flags]], check=True, stdout=subprocess.DEVNULL)
+    return out.read_text().splitlines()
+
+
+def _tsv_case():
+    """Targets: chrA split into two overlapping records (@0 and @600, the
+    preprocessDB layout) so that a read inside the overlap maps to both with
+    identical printed rows; chrB holds the read with one substitution, chrC
+    with two.  The read: 150 bases of chrA 700..849."""
+    rng = np.random.default_rng(33)
+    a = bytes(rng.choice(list(b"ACGT"), 1500).astype(np.uint8))
+    read = a[700:850]
+    b = bytes(rng.choice(list(b"ACGT"), 1000).astype(np.uint8))
+    c = bytes(rng.choice(list(b"ACGT"), 1000).astype(np.uint8))
+    rb = bytearray(read)
+    rb[75] = _other(rb[75])[0]
+    rc = bytearray(read)
+    rc[50] = _other(rc[50])[0]
+    rc[100] = _other(rc[100])[0]
+    b = b[:300] + bytes(rb) + b[450:]
+    c = c[:200] + bytes(rc) + c[350:]
+    fa = (b">chrA\n@0\n" + a[:1000] + b"\n>chrA\n@600\n" + a[600:] + b"\n>chrB\n" + b + b"\n>chrC\n" + c + b"\n")
+    return fa, b">r1\n" + read + b"\n", read
+
+
+def _expected_rows(db):
+    def row(tl, ti, m, s):
+        e = 0.71 * db * 150 * math.exp(-1.37 * s)
+        return f"r1\t0\t150\t+\t{tl}\t{ti}\t150\t{m}({m * 100 / 150:g}%)\t{s}\t{e:g}"
+    # chrA at 700 (printed once: the overlap's second record gives the same
+    # ten fields), chrB at 300 (149 matches, 149 - 3), chrC at 200 (148, 148 - 6)
+    return [row("chrA", 700, 150, 150), row("chrB", 300, 149, 146), row("chrC", 200, 148, 142)]
+
+
+@pytest.mark.parametrize("num_out,n", [(-1, 3), (1, 1), (2, 2), (0, 1), (5, 3)])
+def test_print_num_out_and_equal_row_skip(tmp_path, num_out, n):
+    fa, qa, _ = _tsv_case()
+    (tmp_path / "db.fa").write_bytes(fa)
+    db = oracle.total_db_size([str(tmp_path / "db.fa")])
+    # one target window per record (tVRAM 0.001 MBase), as split targets run
+    rows = _run_oracle_cli(tmp_path, fa, qa, "-numOut", num_out, "-tVRAM", 0.001)
+    plus = [r for r in rows if r.split("\t")[3] == "+"]
+    # the skipped duplicate does not count toward numOut: numOut 2 prints
+    # chrA and chrB, not chrA alone
+    assert plus == _expected_rows(db)[:n]
