@@ -1010,6 +1010,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       uint32_t sc3[3];
       rt.read_scalars(tot32 + 5, sc3, 3);
       const uint32_t nk = sc3[0];
+      // every row of the large pairs may have failed the bad-hit filters:
+      // then nothing is left to order (and the score range is empty)
+      if (nk > 0) {
       const int32_t smin_v = static_cast<int32_t>(sc3[1]), smax_v = static_cast<int32_t>(sc3[2]);
       const int scb = bits_for(static_cast<uint64_t>(static_cast<int64_t>(smax_v) - smin_v + 1));
       const int pbits = bits_for((q.nseq + 1) / 2);
@@ -1053,6 +1056,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         gfirst = ws.get<uint32_t>("r.gfirst", nhit);
         k_big_counts<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hbase, dflag, gfirst);
         rt.check_launch("k_big_counts");
+      }
       }
     }
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nhit, doff, tot32 + 3, false);

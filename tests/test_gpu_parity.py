@@ -465,3 +465,29 @@ def test_index_forms(form, lmer, monkeypatch):
     o = oracle.search(p, tl, twins, ql, qwins)
     assert len(o["final"]["tID"]) > 0
     assert_rows_equal(got, o["final"], what=f"form {form} lmer {lmer}")
+
+
+@pytest.mark.parametrize("cut_off", [10.0, 1e-200])
+def test_large_pair_all_rows_filtered(cut_off):
+    # one read whose region sits in 300 target sequences (no repeat cut):
+    # its '+' strand has 300 hits, a pair over 256 hits, which takes the
+    # sort path of the result stage.  With a tiny cutOff every row fails the
+    # E-value cut and the pair leaves no row (the sort path has nothing to
+    # order); with the default one every copy gives a row.  Both = oracle.
+    rng = np.random.default_rng(44)
+    region = rng.choice(list(b"ACGT"), 150).astype(np.uint8)
+    seqs = [np.concatenate([rng.choice(list(b"ACGT"), 60).astype(np.uint8), region,
+                            rng.choice(list(b"ACGT"), 90).astype(np.uint8)]) for _ in range(300)]
+    t = gen.Seqs(np.concatenate(seqs), np.arange(301, dtype=np.int64) * 300)
+    r = gen.Seqs(region.copy(), np.array([0, 150], np.int64))
+    p = native.Params(cut_repeat=-1, cut_off=cut_off, total_db_size=float(len(t.bases)))
+    tl, ql = lists(t, r, p.lmer)
+    print(f"large pair, cutOff {cut_off}", flush=True)
+    ix = native.Index(p, tl, [(0, 300)])
+    q = native.Query(p, ql, [(0, 1)])
+    got = native.search_batch(p, ix, q).arrays()
+    exp = oracle.search(p, tl, [(0, 300)], ql, [(0, 1)])["final"]
+    for h in (q, ix):
+        h.close()
+    assert len(exp["tID"]) == (300 if cut_off == 10.0 else 0)
+    assert_rows_equal(got, exp, what=f"large pair cutOff {cut_off}")
