@@ -51,6 +51,40 @@ Runtime& runtime() {
   return *rt;
 }
 
+Runtime* runtime_if_exists() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_rt_mu);
+  auto it = g_rt.find(std::make_pair(dev, std::this_thread::get_id()));
+  return it != g_rt.end() ? it->second : nullptr;
+}
+
+// Scratch of clast_search_batch_noalloc.  Each device holds its own copy of
+// the array until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (set up once per device
+// by clast_init), never through the array's name, which in host code is not
+// a device address.  Calls on one device share it, so the entry point holds
+// that device's lock for the whole call.
+static __device__ __align__(256) uint8_t scratch[CLAST_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+  void* ptr = nullptr;
+  std::mutex call;  // one allocation-free call at a time on the device
+};
+static ScratchSlot g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static void scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw ClastError(CLAST_ERR_LIMIT, "device index over 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev].ptr) return;
+  void* ptr = nullptr;
+  hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!ptr) throw ClastError(CLAST_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+  g_scratch[dev].ptr = ptr;
+}
+
 // Byte arrays the DP streams read as 16-byte blocks: whole blocks, at least one.
 static uint64_t padded_blocks(uint64_t bytes) {
   const uint64_t b = (bytes + 15) / 16;
@@ -632,8 +666,18 @@ struct StageEvents {
 // sequence, diagonal) keys and scan-based filters.  CLAST_KEEP_STAGES sends
 // every strand through the global path so its intermediate arrays can be
 // read back.
+// Caller-owned host arrays that an allocation-free search writes its rows
+// into (clast_search_batch_noalloc): no stage events, no pinned block, no
+// copy stream.
+struct CallerRows {
+  int64_t capacity;
+  int32_t* col[10];  // tID tIdx qID qIdx tHL qHL match score twin qwin (null: skipped)
+  double* evalue;
+  int64_t needed = 0;
+};
+
 void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int flags,
-                clast_hits_s& out) {
+                clast_hits_s& out, CallerRows* rows_out = nullptr) {
   if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
     throw ClastError(CLAST_ERR_ARG, "params differ from those the index/query were built with");
   if (ix.device != q.device) throw ClastError(CLAST_ERR_ARG, "index and query on different devices");
@@ -646,10 +690,11 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const bool keep_stages = (flags & CLAST_KEEP_STAGES) != 0;
   const bool use_cut = p.cut_off != -1;
   if (use_cut) ensure_smin(rt, p, q);
-  StageEvents ev;
+  std::unique_ptr<StageEvents> ev(rows_out ? nullptr : new StageEvents());
   bool timed[kEvents] = {};
   auto mark = [&](int k) {
-    hip_check(hipEventRecord(ev.e[k], st), "event");
+    if (!ev) return;
+    hip_check(hipEventRecord(ev->e[k], st), "event");
     timed[k] = true;
   };
   mark(EV_START);
@@ -1075,7 +1120,28 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   clast::pinned_pool().release(out.rows);
   out.rows = {};
   out.n = 0;
-  if (nfinal) {
+  if (rows_out) {
+    rows_out->needed = nfinal;
+    if (nfinal > rows_out->capacity)
+      throw ClastError(CLAST_ERR_LIMIT, "result rows exceed the caller's capacity (" + std::to_string(nfinal) +
+                                            " needed)");
+    if (nfinal) {
+      uint8_t* ob = ws.get<uint8_t>("o0", static_cast<uint64_t>(nfinal) * 48);
+      int32_t* o32 = reinterpret_cast<int32_t*>(ob);
+      double* oev = reinterpret_cast<double*>(ob + static_cast<uint64_t>(nfinal) * 40);
+      OutArgs oa{nfinal, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
+                 ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
+      k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
+      rt.check_launch("k_gather_out");
+      for (int c = 0; c < 10; ++c)
+        if (rows_out->col[c])
+          hip_check(hipMemcpyAsync(rows_out->col[c], o32 + static_cast<uint64_t>(c) * nfinal, 4ull * nfinal,
+                                   hipMemcpyDeviceToHost, st),
+                    "D2H rows");
+      if (rows_out->evalue)
+        hip_check(hipMemcpyAsync(rows_out->evalue, oev, 8ull * nfinal, hipMemcpyDeviceToHost, st), "D2H rows");
+    }
+  } else if (nfinal) {
     const int slot = rt.next_out_slot();
     const std::string tag = slot ? "o1" : "o0";
     // staging buffer `slot` is free once the copy that last read it is done
@@ -1109,7 +1175,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   for (int k = 0; k < 7; ++k) {
     float t = 0;
     if (timed[pairs[k][0]] && timed[pairs[k][1]])
-      hip_check(hipEventElapsedTime(&t, ev.e[pairs[k][0]], ev.e[pairs[k][1]]), "elapsed");
+      hip_check(hipEventElapsedTime(&t, ev->e[pairs[k][0]], ev->e[pairs[k][1]]), "elapsed");
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
@@ -1180,6 +1246,7 @@ int clast_init(int32_t device) {
     if (device < 0 || device >= n) throw ClastError(CLAST_ERR_ARG, "no such device");
     set_device(device);
     (void)runtime();
+    scratch_setup(device);
   });
 }
 
@@ -1446,6 +1513,51 @@ int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, in
     auto h = std::make_unique<clast_hits_s>();
     run_search(*p, *idx, *q, flags, *h);
     *out = h.release();
+  });
+}
+
+int clast_search_batch_noalloc(const clast_params* p, clast_index idx, clast_query q, int64_t capacity,
+                               int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                               int32_t* qHL, int32_t* match, int32_t* score, double* evalue, int32_t* twin,
+                               int32_t* qwin, int64_t* n_rows) {
+  return guard([&] {
+    if (n_rows) *n_rows = 0;
+    check_params(p);
+    if (!idx || !q) throw ClastError(CLAST_ERR_ARG, "null handle");
+    if (capacity < 0) throw ClastError(CLAST_ERR_ARG, "negative row capacity");
+    if (idx->device < 0 || idx->device >= kMaxDevices) throw ClastError(CLAST_ERR_ARG, "bad device");
+    set_device(idx->device);
+    Runtime* rt = runtime_if_exists();
+    ScratchSlot& sc = g_scratch[idx->device];
+    if (!rt || !sc.ptr)
+      throw ClastError(CLAST_ERR_ARG, "clast_search_batch_noalloc: call clast_init(device) on this thread first");
+    // what the scratch cannot hold is refused before any device work: the
+    // search keeps a 16-byte cell record per query k-mer position and about
+    // 64 bytes per strand before the first seed
+    const uint64_t floor_bytes = 16ull * q->npos + 64ull * q->nseq;
+    if (floor_bytes > CLAST_NOALLOC_SCRATCH_BYTES)
+      throw ClastError(CLAST_ERR_LIMIT, "query set too large for the allocation-free scratch (" +
+                                            std::to_string(q->npos) + " k-mer positions): split it");
+    std::lock_guard<std::mutex> lk(sc.call);
+    // on every exit the streams drain before the scratch is released
+    struct ArenaScope {
+      Runtime* rt;
+      ~ArenaScope() {
+        (void)hipStreamSynchronize(rt->side_stream());
+        (void)hipStreamSynchronize(rt->stream());
+        rt->ws().end_arena();
+      }
+    } scope{rt};
+    rt->ws().begin_arena(sc.ptr, CLAST_NOALLOC_SCRATCH_BYTES);
+    CallerRows rows{capacity, {tID, tIdx, qID, qIdx, tHL, qHL, match, score, twin, qwin}, evalue};
+    clast_hits_s h;
+    try {
+      run_search(*p, *idx, *q, 0, h, &rows);
+    } catch (...) {
+      if (n_rows) *n_rows = rows.needed;
+      throw;
+    }
+    if (n_rows) *n_rows = rows.needed;
   });
 }
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -64,6 +65,7 @@ class Workspace {
   template <class T>
   T* get(const std::string& name, uint64_t count) {
     const uint64_t bytes = (count == 0 ? 1 : count) * sizeof(T);
+    if (arena_) return static_cast<T*>(arena_get(name, bytes));
     Slot& s = slots_[name];
     if (s.bytes < bytes) {
       if (s.ptr) {
@@ -86,13 +88,60 @@ class Workspace {
     return b;
   }
 
+  // Arena mode: until end_arena, every slot is carved from the device block
+  // [base, base + cap) by bump allocation (256-byte aligned), nothing is
+  // allocated, freed or synchronised, and a request the block cannot hold
+  // throws CLAST_ERR_LIMIT.  A name asked for again gets its region back if
+  // it fits, else a new one (content is not kept, as on growth).  Regions
+  // are laid out front to back for the call; begin_arena starts over.  The slot
+  // table is fixed-size and its names are short (no heap use).
+  void begin_arena(void* base, uint64_t cap) {
+    arena_ = static_cast<uint8_t*>(base);
+    arena_cap_ = cap;
+    arena_used_ = 0;
+    arena_n_ = 0;
+  }
+  void end_arena() { arena_ = nullptr; }
+  uint64_t arena_used() const { return arena_used_; }
+
  private:
   struct Slot {
     void* ptr = nullptr;
     uint64_t bytes = 0;
   };
+  struct ArenaSlot {
+    char name[24];
+    uint64_t off, bytes;
+  };
+  static constexpr int kArenaSlots = 160;
+  void* arena_get(const std::string& name, uint64_t bytes) {
+    if (name.size() >= sizeof(ArenaSlot::name)) throw ClastError(4, "workspace slot name too long: " + name);
+    int k = 0;
+    for (; k < arena_n_; ++k)
+      if (name == arena_slots_[k].name) break;
+    if (k < arena_n_ && arena_slots_[k].bytes >= bytes) return arena_ + arena_slots_[k].off;
+    // the same 25% slack as a grown slot, so both modes lay buffers out alike
+    bytes += bytes / 4;
+    const uint64_t off = (arena_used_ + 255) & ~255ull;
+    if (off > arena_cap_ || bytes > arena_cap_ - off)
+      throw ClastError(4, "the search needs more than the " + std::to_string(arena_cap_ >> 20) +
+                              " MiB scratch of the allocation-free entry point (slot " + name + ")");
+    if (k == arena_n_) {
+      if (arena_n_ == kArenaSlots) throw ClastError(4, "workspace slot table full");
+      std::memcpy(arena_slots_[k].name, name.c_str(), name.size() + 1);
+      ++arena_n_;
+    }
+    arena_slots_[k].off = off;
+    arena_slots_[k].bytes = bytes;
+    arena_used_ = off + bytes;
+    return arena_ + off;
+  }
   hipStream_t stream_, copy_, side_;
   std::map<std::string, Slot> slots_;
+  uint8_t* arena_ = nullptr;
+  uint64_t arena_cap_ = 0, arena_used_ = 0;
+  int arena_n_ = 0;
+  ArenaSlot arena_slots_[kArenaSlots];
 };
 
 class Runtime {
@@ -221,5 +270,8 @@ PinnedPool& pinned_pool();
 
 // The runtime of the calling thread's current device (created on first use).
 Runtime& runtime();
+// The same, or null when the calling thread has none on its current device
+// yet (nothing is created).
+Runtime* runtime_if_exists();
 
 }  // namespace clast

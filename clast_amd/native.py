@@ -91,6 +91,7 @@ def hip() -> C.CDLL:
             "clast_query_destroy": (C.c_int, [vp]),
             "clast_search": (C.c_int, [pp, vp, vp, i32, i32, C.POINTER(vp)]),
             "clast_search_batch": (C.c_int, [pp, vp, vp, i32, C.POINTER(vp)]),
+            "clast_search_batch_noalloc": (C.c_int, [pp, vp, vp, i64] + [vp] * 12),
             "clast_hits_count": (i64, [vp]),
             "clast_hits_wait": (C.c_int, [vp]),
             "clast_hits_counts": (C.c_int, [vp, vp]),
@@ -442,6 +443,29 @@ def search_batch(params: Params, index: Index, query: Query, keep_stages: bool =
     _check(hip().clast_search_batch(C.byref(params.c()), index.h, query.h, CLAST_KEEP_STAGES if keep_stages else 0,
                                     C.byref(h)), "clast_search_batch")
     return Hits(h)
+
+
+NOALLOC_SCRATCH_BYTES = 256 << 20  # CLAST_NOALLOC_SCRATCH_BYTES
+
+
+def search_batch_noalloc(params: Params, index: Index, query: Query, capacity: int) -> tuple[int, dict]:
+    """clast_search_batch_noalloc: the batched search with its device buffers
+    in the library's fixed per-device scratch and the rows written into
+    host arrays of `capacity` rows.  Returns (status, rows); on
+    CLAST_ERR_LIMIT rows["needed"] holds the rows the search produced."""
+    out = {k: np.zeros(max(capacity, 1), np.int32) for k in HIT_FIELDS + ("twin", "qwin")}
+    out["evalue"] = np.zeros(max(capacity, 1), np.float64)
+    n = C.c_int64(0)
+    cols = [_ptr(out[k]) for k in HIT_FIELDS]
+    rc = hip().clast_search_batch_noalloc(C.byref(params.c()), index.h, query.h, capacity, *cols,
+                                         _ptr(out["evalue"]), _ptr(out["twin"]), _ptr(out["qwin"]), C.byref(n))
+    rows = {k: v[:n.value] for k, v in out.items()} if rc == 0 else {}
+    rows["needed"] = int(n.value)
+    return rc, rows
+
+
+def last_error() -> str:
+    return hip().clast_last_error().decode()
 
 
 def result_sort(rows: dict[str, np.ndarray]) -> None:

@@ -157,6 +157,28 @@ int clast_search(const clast_params* p, clast_index idx, clast_query q, int32_t 
 int clast_search_batch(const clast_params* p, clast_index idx, clast_query q, int32_t flags,
                        clast_hits* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): clast_search_batch's launch chain and rows, with every
+ * device buffer of the search carved from a fixed scratch array that the
+ * library holds on each device it runs on (CLAST_NOALLOC_SCRATCH_BYTES, kept
+ * until the process ends), and the rows written into the caller's host
+ * arrays (`capacity` rows each; null arrays are skipped; twin / qwin as
+ * clast_hits_copy_windows gives them) before the call returns.  It allocates
+ * no device or pinned memory, so clast_init(device) must have run on the
+ * calling thread.  Limits: a query set whose k-mer positions the scratch
+ * cannot hold (16 B each) is refused before any device work, and a search
+ * that needs more scratch fails, both with CLAST_ERR_LIMIT; more rows than
+ * `capacity` fail with CLAST_ERR_LIMIT and *n_rows set to the rows needed.
+ * Calls on one device share its scratch, so the library runs them one at a
+ * time (a second call waits for the first); the scratch keeps the last
+ * call's contents and every call clears what it needs cleared. */
+#define CLAST_NOALLOC_SCRATCH_BYTES (256u << 20)
+int clast_search_batch_noalloc(const clast_params* p, clast_index idx, clast_query q, int64_t capacity,
+                               int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
+                               int32_t* qHL, int32_t* match, int32_t* score, double* evalue, int32_t* twin,
+                               int32_t* qwin, int64_t* n_rows);
+
 /* CDeviceHitList::getResult (CDeviceHitList.cu:425-451): the 9 result
  * arrays (tID and qID are list IDs).  Null pointers are skipped.  The rows
  * reach host memory asynchronously after a search returns (so the copy
