@@ -1,22 +1,41 @@
 #!/usr/bin/env python3
 """Benchmark of the CLAST search path on one MI355X.
 
-Workload (default ``large``): 1 Gbp target (1,000 x 1 Mbp, 10% repeat
-content) and 1,000,000 reads of 150 bp, reference defaults (lMer 15, stride
-5, repeat 20, width 100, gap 8, global, cutOff 10; tRAM = qRAM = tVRAM = 64
-MBase, qVRAM = 2 MBase).  The target's 16 RAM chunks (one window each) are
-indexed once and kept resident in HBM; each of the 5 query RAM chunks (~35
-query windows each) is uploaded before the timed region.  One step = every
-query chunk searched against every target window (clast_search_batch per
-query chunk), hits copied back to host memory.
+Headline (the serving configuration): a 3 Gbp target (3,000 x 1 Mbp, 40%
+repeats from 1,000 families at 5-25% divergence, 1% (AT)n / (CA)n tracts)
+indexed once and kept resident in HBM; requests of 1-10,000 reads
+(log-uniform count; lengths half 150 bp, half log-uniform 1-15 kbp; 4%
+substitutions, 8% indels) arrive as a Poisson process at --rate requests per
+second and are served by continuous batching, admitted by a device-memory
+budget (clast_amd/serve.py).  Reference defaults: lMer 15, stride 5, repeat
+20, width 100, gap 8, global, cutOff 10, query windows of 2 MBase.
 
-value = query Mbp (one strand, as read) aligned per second over the K timed
-steps.  latency = per query-chunk search time (p50/p99 over all timed
-searches).  roofline = the seed lookup kernel (k_seed_count) against HBM
-peak; its traffic (HBM bytes per launch) comes from the counter runs recorded
-in profiles/large_hbm_traffic.txt.  cpu_baseline = the CPU oracle on a bounded sample (one target window
-x 16 query windows, OpenMP, searches only), scaled to the whole target; the
-same sample is searched on the GPU and must match it bit-exactly.
+A step is one second of arrivals (--rate requests); W warmup steps are
+served first, then K timed steps as one continuous stream.  value = query
+Mbp (one strand, as read) served per second of wall time from the first
+arrival to the last request's rows in host memory; latency = arrival to rows
+in host memory per request (p50 / p99 over all K x rate requests).  Every
+request's time includes uploading its reads and building its strands on the
+device (clast_query_create_reads) and copying its rows back.
+
+Also reported (untimed checks and extra legs):
+* full_size_checks: every served row through the size-independent checks
+  (clast_amd/checks.py); mix_equals_alone: a sample of requests searched
+  again alone, rows equal;
+* capacity: the same requests queued at once (served Mbp/s with no arrival
+  gaps);
+* roofline: the kernel with the most device time in the timed run, its
+  algorithmic work per launch over its hipEvent-timed launch durations;
+* cpu_baseline: the oracle (built -march=native on this host, OpenMP) on a
+  bounded sample of the same requests against one target window, scaled by
+  the number of target windows; the GPU searches the same unit and must match
+  it bit-exactly (parity_sample);
+* pressure_sweep: the seed-pressure stream (50% random, 25% repeat-family,
+  25% reverse-complement reads of 100 bp-20 kbp) at 10-200 requests/s, each
+  request's rows equal at every rate;
+* large: the 1 Gbp target with 1,000,000 reads of 150 bp in 5 query RAM
+  chunks, each step uploading and encoding every chunk's reads on the device
+  and copying the rows back, from 3 host threads.
 """
 from __future__ import annotations
 
@@ -24,6 +43,7 @@ import argparse
 from concurrent.futures import ThreadPoolExecutor
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -33,56 +53,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VALU_INT32_PEAK_TOPS = 39.3  # 256 CU x 64 lanes x 2.4 GHz, one int32 op per lane per clock
+CLOCK_GHZ = 2.4
+N_CU = 256
+# VALU peaks at the 2.4 GHz engine clock: 64 int32 lane-ops per CU per clock,
+# 128 16-bit lane-ops when packed (two per 32-bit lane)
+VALU_INT32_PEAK_TOPS = N_CU * 64 * CLOCK_GHZ / 1e3
+VALU_INT16_PEAK_TOPS = N_CU * 128 * CLOCK_GHZ / 1e3
+OPS_PER_CELL = 14  # int32-equivalent operations per DP cell (v, h, diagonal, max, match-count choice)
+METRIC = "query Mbp/s aligned (bit-exact hits) + p50/p99 request latency, 1 MI355X"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_target(name: str, scale: float, p, t_ram: float, t_vram: float):
-    """Target chunks as the FASTA loader would cut them, their host lists and
-    windows, flattened in (chunk, window) order."""
-    from clast_amd import gen, native
-
-    tspec, _ = gen.workload(name, scale)
-    t = gen.make_target(tspec)
-    chunks, keep = native.chunks(t.lengths, int(t_ram))
-    out = []
-    for a, b in chunks:
-        idx = [i for i in range(a, b) if keep[i]]
-        if not idx:
-            continue
-        sub = t.subset(idx[0], idx[-1] + 1)
-        lst = native.build_list(sub.bases, sub.offsets, p.lmer, False)
-        wins = native.windows(lst.gateway, t_vram, False)
-        out.append((lst, wins))
-    return t, out
-
-
-def build_queries(name: str, scale: float, p, t, q_ram: float, q_vram: float):
-    from clast_amd import gen, native
-
-    _, rspec = gen.workload(name, scale)
-    r = gen.make_reads(rspec, t)
-    chunks, keep = native.chunks(r.lengths, int(q_ram / 2))
-    out = []
-    for a, b in chunks:
-        idx = [i for i in range(a, b) if keep[i]]
-        if not idx:
-            continue
-        sub = r.subset(idx[0], idx[-1] + 1)
-        lst = native.build_list(sub.bases, sub.offsets, p.lmer, True)
-        wins = native.windows(lst.gateway, q_vram, True)
-        out.append((lst, wins, int(sub.lengths.sum())))
-    return r, out
-
-
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on this workload (FETCH_SIZE +
-    WRITE_SIZE, calibrated against a plain device copy), as recorded by the
-    last counter runs in profiles/large_hbm_traffic.txt; None if absent."""
-    path = os.path.join(ROOT, "profiles", "large_hbm_traffic.txt")
+def measured_traffic(name: str, kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE scaled by a
+    plain device copy's counters) from profiles/<name>_hbm_traffic.txt."""
+    path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -94,176 +82,301 @@ def measured_traffic(kernel: str):
     return None
 
 
-def full_size_checks(p, args, tchunks, qchunks, indexes, queries) -> dict:
-    """Size-independent checks on every row of one full step (untimed): the
-    rows of each search with the bench's threads equal those one search at a
-    time; rows are in the reference's output order (query pair ascending,
-    score descending: CHostResultHolder.cu:100-109), each E-value is
-    K D qLen exp(-lambda score)
-    (krnlCalculateEvalue.cu:18-21) within 1e-9, survivors pass the bad-hit
-    filters (E <= cutOff, qHL >= lMer + stride; ...deleteBadHits.cu), and
-    hits lie inside their sequences with match <= both lengths; rows that
-    share a (target, tIdx, query, qIdx) start are counted."""
+# ---------------------------------------------------------------- serving
+
+def serving_target():
+    from clast_amd import configs, gen, native
+
+    tspec, _ = gen.workload("serving")
+    t = gen.make_target(tspec)
+    p = native.Params(total_db_size=float(t.bases.shape[0]))
+    tch = configs.target_chunks(t, p)
+    return t, p, tch
+
+
+def request_checks(p, reqs, tch) -> dict:
+    from clast_amd import checks
+
+    wchunk = np.array([c for c, (_, w) in enumerate(tch) for _ in w], dtype=np.int64)
+    tlens = [lst.length for lst, _ in tch]
+    res = []
+    for r in reqs:
+        rows = r.rows
+        n = len(rows["tID"])
+        qlens = np.repeat(np.diff(r.offsets), 2)
+        if n == 0:
+            res.append(checks.check_rows(p, rows, np.zeros(0), np.zeros(0)))
+            continue
+        ch = wchunk[rows["twin"]]
+        tl = np.empty(n, np.int64)
+        for c in np.unique(ch):
+            sel = ch == c
+            tl[sel] = tlens[c][rows["tID"][sel]]
+        res.append(checks.check_rows(p, rows, qlens[rows["qID"]], tl))
+    m = checks.merge(res)
+    m["passed"] = checks.passed(m)
+    return m
+
+
+def rows_equal(a: dict, b: dict) -> bool:
     from clast_amd import native
 
-    keys = [(qi, ii) for qi in range(len(queries)) for ii in range(len(indexes))]
-
-    def collect(streams):
-        res = {}
-
-        def run(part):
-            for qi, ii in part:
-                h = native.search_batch(p, indexes[ii], queries[qi])
-                res[(qi, ii)] = h.arrays()
-                h.close()
-
-        parts = [keys[k::streams] for k in range(streams)]
-        with ThreadPoolExecutor(max_workers=streams) as ex:
-            list(ex.map(run, parts))
-        return res
-
-    many, one = collect(max(args.streams, 1)), collect(1)
-    fields = native.HIT_FIELDS + ("evalue", "twin", "qwin")
-    threads_equal = all(np.array_equal(many[k][f], one[k][f]) for k in keys for f in fields)
-    # window -> (target chunk, window in chunk) for a merged index
-    wchunk = [c for c, (_, w) in enumerate(tchunks) for _ in w]
-    rows = 0
-    order = filters = bounds = True
-    repeated = 0
-    emax = 0.0
-    for qi, ii in keys:
-        a = one[(qi, ii)]
-        n = len(a["tID"])
-        rows += n
-        if n == 0:
-            continue
-        pair, sc = a["qID"] // 2, a["score"].astype(np.int64)
-        order &= bool(np.all((pair[1:] > pair[:-1]) | ((pair[1:] == pair[:-1]) & (sc[1:] <= sc[:-1]))))
-        # rows sharing a start: seeds on diagonals too far apart for the
-        # duplicate filter can extend to one start; the reference keeps
-        # both (no row dedup after the DP), so this is a count, not a check
-        tup = np.stack([a["twin"], a["tID"], a["tIdx"], a["qID"], a["qIdx"]], axis=1)
-        repeated += n - len(np.unique(tup, axis=0))
-        qlen = qchunks[qi][0].length[a["qID"]].astype(np.float64)
-        exp_e = p.K * p.total_db_size * qlen * np.exp(-p.lam * sc.astype(np.float64))
-        emax = max(emax, float(np.max(np.abs(a["evalue"] - exp_e) / np.maximum(exp_e, 1e-300))))
-        filters &= bool(np.all(a["qHL"] >= p.lmer + p.stride))
-        if p.cut_off != -1:
-            filters &= bool(np.all(a["evalue"] <= p.cut_off))
-        chunk = np.array([wchunk[w] for w in a["twin"]]) if len(indexes) == 1 else np.full(n, ii)
-        tlen = np.array([tchunks[c][0].length[t] for c, t in zip(chunk, a["tID"])], dtype=np.int64)
-        bounds &= bool(np.all((a["qIdx"] >= 0) & (a["qIdx"] + a["qHL"] <= qlen) & (a["tIdx"] >= 0) &
-                              (a["tIdx"].astype(np.int64) + a["tHL"] <= tlen) &
-                              (a["match"] <= np.minimum(a["qHL"], a["tHL"]))))
-    return {"rows": rows, "threads_equal_one_at_a_time": bool(threads_equal), "output_order": bool(order),
-            "rows_sharing_a_start": int(repeated), "evalue_max_rel": emax, "bad_hit_filters": bool(filters),
-            "within_sequences": bool(bounds)}
+    keys = native.HIT_FIELDS + ("twin",)
+    if len(a["tID"]) != len(b["tID"]):
+        return False
+    if not all(np.array_equal(a[k], b[k]) for k in keys):
+        return False
+    if len(a["tID"]) == 0:
+        return True
+    rel = np.abs(a["evalue"] - b["evalue"]) / np.maximum(np.abs(b["evalue"]), 1e-300)
+    return bool(np.all(rel <= 1e-9))
 
 
-def cpu_baseline(p, tchunks, qchunks, n_twin_total: int, threads: int):
-    """The oracle (OpenMP, one query window per thread) on one target window
-    x `threads` query windows.  Like the GPU figure it counts the searches
-    only: the oracle builds that window's table first, timed apart.  Query
-    Mbp/s is scaled by the number of target windows every read meets.  The
-    GPU searches the same units and must match bit-exactly."""
+def kernel_roofline(records, name: str) -> tuple[dict, dict]:
+    """Per-kernel device time over the timed batches and the roofline of the
+    kernel with the most time."""
+    kt = np.array([r[2] for r in records], dtype=np.float64).reshape(-1, 8)
+    cnt = np.array([r[3] for r in records], dtype=np.float64).reshape(-1, 8)
+    pos = np.array([r[1] for r in records], dtype=np.float64)
+    cells, cells16, cellsband = cnt[:, 2], cnt[:, 5], cnt[:, 7]
+    cells32 = cells - cells16 - cellsband
+    kernels = {
+        # name: (ms, launches, work, unit, peak, bound)
+        "k_seed_count": (kt[:, 0], 16.0 * pos, "GB/s", HBM_PEAK_GBS, "hbm"),
+        "k_dp16<8>": (kt[:, 1], OPS_PER_CELL * cells16, "Tops/s", VALU_INT16_PEAK_TOPS, "valu_int16"),
+        "k_dp<8,false>": (kt[:, 2], OPS_PER_CELL * cells32, "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
+        "k_dp_band<8,false>": (kt[:, 3], OPS_PER_CELL * cellsband, "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
+    }
+    per = {}
+    for k, (ms, work, unit, peak, bound) in kernels.items():
+        launched = ms > 0
+        tot_ms = float(ms.sum())
+        per[k] = {"ms": round(tot_ms, 3), "launches": int(launched.sum()),
+                  "avg_us": round(1e3 * tot_ms / max(int(launched.sum()), 1), 2)}
+        if tot_ms > 0:
+            scale = 1e9 if unit == "GB/s" else 1e12
+            per[k][unit] = round(float(work[launched].sum()) / (tot_ms / 1e3) / scale, 3)
+    per["dp_stage"] = {"ms": round(float(kt[:, 4].sum()), 3)}
+    per["seed_generation"] = {"ms": round(float(kt[:, 5].sum()), 3)}
+    per["results"] = {"ms": round(float(kt[:, 6].sum()), 3)}
+    per["search_total"] = {"ms": round(float(kt[:, 7].sum()), 3)}
+    dom = max(kernels, key=lambda k: per[k]["ms"])
+    ms, work, unit, peak, bound = kernels[dom]
+    launched = ms > 0
+    scale = 1e9 if unit == "GB/s" else 1e12
+    achieved = float(work[launched].sum()) / (float(ms.sum()) / 1e3) / scale if ms.sum() > 0 else 0.0
+    roof = {"kernel": dom, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": unit,
+            "frac": round(achieved / peak, 4), "launches": int(launched.sum()),
+            "avg_launch_us": per[dom]["avg_us"],
+            "traffic": measured_traffic(name, dom.split("<")[0]),
+            "work_per_unit": ("16 B per query k-mer position" if unit == "GB/s" else
+                              f"{OPS_PER_CELL} int32-equivalent ops per DP cell (band width x rows)")}
+    if bound == "valu_int16":
+        roof["peak_note"] = (f"packed 16-bit VALU: {N_CU} CU x 128 lane-ops per clock at {CLOCK_GHZ} GHz; "
+                             f"the int32 peak is {VALU_INT32_PEAK_TOPS:.1f}")
+    return roof, per
+
+
+def cpu_baseline_serving(p, tch, reqs, sample_mbp: float, threads: int):
+    """The oracle on the first requests up to sample_mbp of reads against
+    target window 0 (its table built first, timed apart), scaled by the
+    number of target windows; the GPU searches the same unit."""
+    from clast_amd import configs, native
+
+    build = "build_native"
+    try:
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=300)
+    except (subprocess.SubprocessError, OSError) as e:
+        log(f"native oracle build failed ({e}); timing the portable build")
+        build = "build"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    from clast_amd import native
 
-    tl, twins = tchunks[0]
-    ql, qwins, _ = qchunks[0]
-    tw, qws = twins[0], qwins[:threads]
-    q_bases = sum(int(ql.length[2 * a:2 * b:2].sum()) for a, b in qws)
+    oracle.use_build(build)
+    sel, mbp = [], 0.0
+    for r in reqs:
+        sel.append(r)
+        mbp += r.n_bases / 1e6
+        if mbp >= sample_mbp:
+            break
+    bases = np.concatenate([r.bases[r.offsets[0]:r.offsets[-1]] for r in sel])
+    offs = [np.zeros(1, np.int64)]
+    for r in sel:
+        offs.append((r.offsets[1:] - r.offsets[0]).astype(np.int64) + offs[-1][-1])
+    offs = np.concatenate(offs)
+    ql = native.build_list(bases, offs, p.lmer, True)
+    qwins = native.windows(ql.gateway, configs.Q_VRAM, True)
+    tl, twins = tch[0]
+    tw = twins[0]
+    n_twin = sum(len(w) for _, w in tch)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    o = oracle.search(p, tl, [tw], ql, qws)
+    o = oracle.search(p, tl, [tw], ql, qwins)
     exp = o["final"]
     ix = native.Index(p, tl, [tw])
-    q = native.Query(p, ql, qws)
+    q = native.Query.from_reads(p, bases, offs, qwins)
     got = native.search_batch(p, ix, q).arrays()
     ix.close()
     q.close()
-    exact = len(got["tID"]) == len(exp["tID"]) and all(
-        np.array_equal(got[k], exp[k]) for k in ("tID", "tIdx", "qID", "qIdx", "tHL", "qHL", "match", "score"))
-    rel = float(np.max(np.abs(got["evalue"] - exp["evalue"]) / np.maximum(exp["evalue"], 1e-300))) \
-        if exact and len(exp["tID"]) else 0.0
-    mbps = q_bases / 1e6 / (o["search_s"] * n_twin_total)
-    sample = (f"oracle on {threads} threads: 1 of {n_twin_total} target windows ({tw[1] - tw[0]} seqs, table "
-              f"built in {o['index_s']:.1f} s, not counted) x {len(qws)} query windows "
-              f"({qws[-1][1] - qws[0][0]} reads, {q_bases / 1e6:.2f} Mbp) searched in {o['search_s']:.2f} s; "
-              f"Mbp/s scaled by {n_twin_total} target windows")
-    return {"value": round(mbps, 4), "unit": "query Mbp/s aligned", "cores": threads, "kind": "port",
-            "sample": sample}, {"rows": int(len(exp["tID"])), "bit_exact": bool(exact), "evalue_max_rel": rel}
+    exact = rows_equal(got, exp)
+    value = mbp / (o["search_s"] * n_twin)
+    sample = (f"oracle ({build}, {threads} OpenMP threads) on {len(sel)} requests ({len(offs) - 1} reads, "
+              f"{mbp:.1f} Mbp) x target window 0 of {n_twin} ({tw[1] - tw[0]} seqs; table built in "
+              f"{o['index_s']:.1f} s, not counted), searched in {o['search_s']:.1f} s; Mbp/s scaled by "
+              f"{n_twin} target windows")
+    rel = 0.0
+    if exact and len(exp["tID"]):
+        rel = float(np.max(np.abs(got["evalue"] - exp["evalue"]) / np.maximum(exp["evalue"], 1e-300)))
+    return ({"value": round(value, 5), "unit": "query Mbp/s aligned", "cores": threads,
+             "nproc": os.cpu_count(), "kind": "port", "sample": sample},
+            {"rows": int(len(exp["tID"])), "bit_exact": bool(exact), "evalue_max_rel": rel})
 
 
-def serve_bench(args) -> int:
-    """The serving loop on the 3 Gbp workload (40% repeats, 1% low
-    complexity): the index of every target window resident in HBM, requests of
-    1..10,000 reads (log-uniform count; lengths 50/50 150 bp and 1-15 kbp)
-    arriving as a Poisson process, admitted by a query k-mer budget and
-    coalesced into shared launch chains; latency = arrival to rows in host
-    memory."""
-    import dataclasses
+def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int) -> dict:
+    """The seed-pressure request stream at each arrival rate: the same
+    requests, arrival gaps scaled; each request's rows must be equal at
+    every rate (bit-exact under any admission pattern)."""
+    from clast_amd import configs, gen, serve
 
-    from clast_amd import gen, native, serve
+    _, rspec = gen.workload("pressure")
+    base = serve.make_requests(rspec, t, n_requests, 1.0, seed=5)
+    arrivals = np.array([r.arrival for r in base])
+    mbp = sum(r.n_bases for r in base) / 1e6
+    ref = None
+    points = []
+    exact = True
+    for rate in rates:
+        reqs = [serve.Request(r.rid, r.bases, r.offsets, float(a / rate)) for r, a in zip(base, arrivals)]
+        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=workers)
+        st = srv.run(reqs, realtime=True)
+        if ref is None:
+            ref = [r.rows for r in reqs]
+        else:
+            exact &= all(rows_equal(r.rows, e) for r, e in zip(reqs, ref))
+        points.append({"rate_req_s": rate, "offered_mbp_s": round(mbp * rate / max(arrivals[-1], 1e-9), 2),
+                       "served_mbp_s": st["mbp_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                       "batches": st["batches"], "mean_batch": st["mean_batch"], "wall_s": st["wall_s"]})
+        log(f"pressure {rate} req/s: {points[-1]}")
+    return {"requests": n_requests, "query_mbp": round(mbp, 2),
+            "rows": int(sum(len(r["tID"]) for r in ref)), "rows_equal_at_every_rate": bool(exact),
+            "points": points}
 
-    native.init(0)
-    p = native.Params()
+
+# ---------------------------------------------------------------- large (1 Gbp x 1 M reads)
+
+def large_leg(steps: int, warmup: int, streams: int) -> dict:
+    """Each step: every query RAM chunk's reads uploaded and encoded on the
+    device (clast_query_create_reads), searched against the resident index
+    of all 16 target windows, rows copied to host memory; chunks dealt to
+    `streams` host threads.  Also timed with the query sets resident before
+    the step (resident_value), and per kernel with one search at a time."""
+    from clast_amd import configs, native
+
     t0 = time.perf_counter()
-    t, tchunks = build_target("serving", args.scale, p, 64e6, 64e6)
-    p.total_db_size = float(t.bases.shape[0])
-    _, rspec = gen.workload("serving", args.scale)
-    rspec = dataclasses.replace(rspec, n_reads=args.serve_reads)
-    r = gen.make_reads(rspec, t)
-    log(f"generated {t.bases.shape[0] / 1e9:.2f} Gbp target, {len(r)} reads ({r.bases.shape[0] / 1e6:.0f} Mbp) "
-        f"in {time.perf_counter() - t0:.1f} s")
+    t, r, p = configs.workload("large")
+    tch = configs.target_chunks(t, p)
+    rch = configs.read_chunks(r, p)
+    log(f"large: generated in {time.perf_counter() - t0:.1f} s: {len(tch)} target chunks, {len(rch)} query chunks")
     t1 = time.perf_counter()
-    ix = native.Index.merged(p, tchunks)
+    ix = native.Index.merged(p, tch)
     index_s = time.perf_counter() - t1
-    n_twin = sum(len(w) for _, w in tchunks)
-    log(f"index over {n_twin} target windows built in {index_s:.2f} s")
-    srv = serve.Server(p, ix, q_vram=2e6, max_positions=args.serve_budget, workers=args.serve_workers)
-    reqs = serve.poisson_requests(r.bases, r.offsets, rate=args.rate, seed=7, min_reads=1, max_reads=10_000)
-    srv.search_alone(serve.Request(-1, reqs[0].bases, reqs[0].offsets))  # warm up
-    log(f"{len(reqs)} requests at {args.rate} req/s")
-    stats = srv.run(reqs, realtime=True)
-    out = {
-        "metric": "served query Mbp/s + p50/p99 request latency (arrival to rows in host memory), 1 MI355X",
-        "value": stats["mbp_per_s"], "unit": "query Mbp/s", "higher_is_better": True,
-        "latency_ms": {"p50": stats["p50_ms"], "p99": stats["p99_ms"]},
-        "config": {"workload": f"serving: {t.bases.shape[0] / 1e9:.2f} Gbp target ({len(t)} seqs, 40% repeats), "
-                               f"{len(r)} reads, Poisson {args.rate} req/s",
-                   "target_windows": n_twin, "admission_budget_kmer_positions": args.serve_budget},
-        "serving": stats, "index_build_s": round(index_s, 3), "dtype": "int32", "data": "synthetic",
-    }
-    print(json.dumps(out), flush=True)
-    return 0
+    total = sum(c.n_bases for c in rch)
 
+    def job(ch, record=None):
+        q = native.Query.from_reads(p, ch.bases, ch.offsets, ch.wins)
+        h = native.search_batch(p, ix, q)
+        h.wait()
+        n = len(h)
+        if record is not None:
+            record.append((h.kernel_times(), h.counts(), ch))
+        h.close()
+        q.close()
+        return n
+
+    def step(pool):
+        return sum(pool.map(job, rch))
+
+    with ThreadPoolExecutor(max_workers=streams) as pool:
+        for _ in range(warmup):
+            step(pool)
+        ms = []
+        for _ in range(steps):
+            ts = time.perf_counter()
+            rows = step(pool)
+            ms.append((time.perf_counter() - ts) * 1e3)
+        # the query sets resident before the step (the inputs-in-HBM figure)
+        qs = [native.Query.from_reads(p, c.bases, c.offsets, c.wins) for c in rch]
+
+        def job_res(q):
+            h = native.search_batch(p, ix, q)
+            h.wait()
+            h.close()
+
+        ms_res = []
+        for _ in range(steps):
+            ts = time.perf_counter()
+            list(pool.map(job_res, qs))
+            ms_res.append((time.perf_counter() - ts) * 1e3)
+        for q in qs:
+            q.close()
+    rec = []
+    for ch in rch:
+        job(ch, rec)
+    ix.close()
+    kt = np.array([x[0] for x in rec], dtype=np.float64)
+    cnt = np.array([x[1] for x in rec], dtype=np.float64)
+    npos = np.array([float(2 * np.clip(np.diff(x[2].offsets) - p.lmer + 1, 0, None).sum()) for x in rec])
+    seed_ms = float(kt[:, 0].sum())
+    achieved = 16.0 * npos.sum() / (seed_ms / 1e3) / 1e9
+    dp16_ms = float(kt[:, 1].sum())
+    step_ms = float(np.mean(ms))
+    return {
+        "workload": f"large: {t.bases.shape[0] / 1e9:.3g} Gbp target ({len(t)} seqs, 10% repeats) x {len(r)} reads "
+                    f"of 150 bp ({total / 1e6:.0f} Mbp), {len(rch)} query RAM chunks, "
+                    f"{sum(len(c.wins) for c in rch)} query windows, {sum(len(w) for _, w in tch)} target windows",
+        "value": round(total / 1e6 / (step_ms / 1e3), 2), "unit": "query Mbp/s", "ms_per_step": round(step_ms, 3),
+        "steps": steps, "streams": streams, "rows": int(rows),
+        "timed_span": "reads in host memory -> upload + device encoding -> search -> rows in host memory",
+        "resident_value": round(total / 1e6 / (float(np.mean(ms_res)) / 1e3), 2),
+        "resident_note": "query sets already in HBM when the step starts",
+        "index_build_s": round(index_s, 3),
+        "kernels_one_search_at_a_time_ms": {
+            "k_seed_count": round(seed_ms, 3), "k_dp16": round(dp16_ms, 3),
+            "k_dp_int32": round(float(kt[:, 2].sum()), 3), "dp_stage": round(float(kt[:, 4].sum()), 3),
+            "seed_generation": round(float(kt[:, 5].sum()), 3), "results": round(float(kt[:, 6].sum()), 3),
+            "search_total": round(float(kt[:, 7].sum()), 3)},
+        "seed_lookup_roofline": {"kernel": "k_seed_count", "bound": "hbm", "achieved": round(achieved, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                 "traffic": measured_traffic("large", "k_seed_count")},
+        "dp16": {"GCUPS": round(float(cnt[:, 5].sum()) / (dp16_ms / 1e3) / 1e9, 1) if dp16_ms else 0.0,
+                 "cells_16bit_frac": round(float(cnt[:, 5].sum()) / max(float(cnt[:, 2].sum()), 1.0), 4)},
+    }
+
+
+# ---------------------------------------------------------------- main
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="large")
-    ap.add_argument("--streams", type=int, default=3,
-                    help="host threads issuing a step's searches (each with its own library streams)")
-    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--steps", type=int, default=20, help="timed seconds of arrivals (rate requests each)")
+    ap.add_argument("--warmup", type=int, default=3, help="warmup seconds of arrivals")
+    ap.add_argument("--rate", type=float, default=50.0, help="requests per second")
+    ap.add_argument("--workers", type=int, default=2, help="device threads draining the batch queue")
+    ap.add_argument("--capacity-requests", type=int, default=300)
+    ap.add_argument("--alone-sample", type=int, default=12)
+    ap.add_argument("--cpu-sample-mbp", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    ap.add_argument("--sweep-rates", default="10,25,50,100,200")
+    ap.add_argument("--sweep-requests", type=int, default=80)
+    ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-full-checks", action="store_true",
-                    help="skip the untimed property checks over every row of one full step")
-    ap.add_argument("--plain", action="store_true", help="also time the per-window (one launch chain per unit) path")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--index", choices=("merged", "per-chunk"), default="merged",
-                    help="one resident index over every target window, or one per target chunk")
+    ap.add_argument("--no-checks", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-capacity", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only copy MiB of device memory once (the known-size kernel that scales the HBM "
                          "traffic counters) and exit")
-    ap.add_argument("--serve", action="store_true", help="run the serving loop instead (Poisson request stream)")
-    ap.add_argument("--serve-reads", type=int, default=100_000)
-    ap.add_argument("--serve-workers", type=int, default=2, help="device threads draining the batch queue (--serve)")
-    ap.add_argument("--rate", type=float, default=50.0, help="requests per second (--serve)")
-    ap.add_argument("--serve-budget", type=int, default=64_000_000,
-                    help="query k-mer positions admitted per launch chain (--serve)")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -276,189 +389,101 @@ def main() -> int:
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: there is no multi-GPU path (the search would shard by query "
-              "chunk with no exchange during compute).", file=sys.stderr)
+              "request or by target window, with no exchange during compute).", file=sys.stderr)
         return 2
 
-    if args.serve:
-        return serve_bench(args)
-
-    from clast_amd import native
+    from clast_amd import configs, gen, native, serve
 
     native.init(0)
-    p = native.Params()
-    t_ram = t_vram = 64e6
-    q_ram, q_vram = 64e6, 2e6
     t0 = time.perf_counter()
-    t, tchunks = build_target(args.workload, args.scale, p, t_ram, t_vram)
-    p.total_db_size = float(t.bases.shape[0])  # all sequence-line bytes of the target FASTA
-    r, qchunks = build_queries(args.workload, args.scale, p, t, q_ram, q_vram)
-    log(f"generated {len(t)} targets ({t.bases.shape[0] / 1e6:.0f} Mbp), {len(r)} reads in "
-        f"{time.perf_counter() - t0:.1f} s; {len(tchunks)} target chunks, {len(qchunks)} query chunks")
-
-    # resident index: every window of every target chunk, (chunk, window) order
+    t, p, tch = serving_target()
+    n_twin = sum(len(w) for _, w in tch)
+    log(f"serving target: {t.bases.shape[0] / 1e9:.2f} Gbp, {len(t)} seqs, {n_twin} target windows "
+        f"({time.perf_counter() - t0:.1f} s)")
     t1 = time.perf_counter()
-    indexes = []
-    n_twin = sum(len(w) for _, w in tchunks)
-    if args.index == "merged":
-        indexes.append(native.Index.merged(p, tchunks))
-    else:
-        for lst, wins in tchunks:
-            indexes.append(native.Index(p, lst, wins))
+    ix = native.Index.merged(p, tch)
     index_s = time.perf_counter() - t1
-    log(f"index built in {index_s:.2f} s over {n_twin} target windows")
-    queries = [native.Query(p, lst, wins) for lst, wins, _ in qchunks]
-    total_bases = sum(b for _, _, b in qchunks)
-    log(f"{len(queries)} query chunks resident, {total_bases / 1e6:.1f} Mbp")
-
-    jobs = [(q, ix) for q in queries for ix in indexes]
-
-    def run_jobs(mine, record):
-        # each search's rows reach host memory while the next search runs;
-        # a search's handle is released (after its copy) once the next one
-        # has been issued
-        rows = 0
-        prev = None
-        for q, ix in mine:
-            h = native.search_batch(p, ix, q)
-            rows += len(h)
-            if record is not None:
-                record.append((h.times(), h.counts()))
-            if prev is not None:
-                prev.close()
-            prev = h
-        if prev is not None:
-            prev.close()
-        return rows
-
-    def step(record, streams):
-        # the searches of a step dealt round-robin to `streams` host threads,
-        # each with its own library streams, so searches overlap on the GPU;
-        # the step ends with every copy done
-        if streams <= 1:
-            return run_jobs(jobs, record)
-        parts = [jobs[k::streams] for k in range(streams)]
-        with ThreadPoolExecutor(max_workers=streams) as ex:
-            return sum(ex.map(lambda part: run_jobs(part, None), parts))
-
-    for w in range(args.warmup):
-        step(None, args.streams)
-        log(f"warmup {w + 1}/{args.warmup} done")
-    step_ms = []
-    rows = 0
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        rows = step(None, args.streams)
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {rows} hits ({args.streams} streams)")
-    ms = float(np.mean(step_ms))
-    value = total_bases / 1e6 / (ms / 1e3)
-    # per-stage kernel times and work counters: one more step with one
-    # search at a time, so each kernel's duration is its own
-    rec = []
-    step(rec, 1)
-    # latency of one query chunk alone: call to rows in host memory
-    lat = []
-    for q in queries:
-        for ix in indexes:
-            ts = time.perf_counter()
-            h = native.search_batch(p, ix, q)
-            h.wait()
-            lat.append((time.perf_counter() - ts) * 1e3)
-            h.close()
-    lat = np.array(lat)
-    times = np.array([x[0] for x in rec])  # per search: lookup, segments, sort, filt, dp, res, total, raw seeds
-    # raw seeds, DP hits, DP cells, rows, 16-bit DP tasks, their cells, long
-    # DP tasks (one lane per band), their cells
-    counts = np.array([x[1] for x in rec], dtype=np.float64)
-    # k_seed_count algorithmic bytes per launch: 16 B per query k-mer position
-    # (8 B index key lookup + 8 B cell (offset, count)); positions per launch
-    # are taken from the query chunks
-    npos = []
-    for lst, wins, _ in qchunks:
-        ln = lst.length.astype(np.int64)
-        npos.append(int(np.clip(ln - p.lmer + 1, 0, None).sum()))
-    n_idx = len(indexes)
-    pos_per_launch = np.repeat(np.array(npos, dtype=np.float64), n_idx)
-    count_ms = times[:, 0].astype(np.float64)
-    bytes_count = 16.0 * pos_per_launch
-    achieved = float(bytes_count.sum() / (count_ms.sum() / 1e3) / 1e9)
-    traffic = measured_traffic("k_seed_count") if args.workload == "large" and args.scale == 1.0 else None
-    # per-stage work against MI355X peaks (algorithmic bytes / ops): lookup
-    # 16 B per query k-mer; segments 24 B per raw seed
-    # (expansion) + 32 B (sort) + 16 B (isolate/duplicate pass) + 16 B per hit
-    # out; DP 14 int32 ops per cell; results 40 B per hit in + 48 B per row out
-    # (a stage that did not run, e.g. no seeds, reports a rate of 0)
-    sec = lambda col: max(float(times[:, col].sum()) / 1e3, 1e-12)  # noqa: E731
-    seeds_raw, dp_hits, cells, out_rows = (float(counts[:, c].sum()) for c in range(4))
-    seg_bytes = 72.0 * seeds_raw + 16.0 * dp_hits
-    res_bytes = 40.0 * dp_hits + 48.0 * out_rows
-    stages = {
-        "seed_lookup": {"GB/s": round(bytes_count.sum() / sec(0) / 1e9, 1), "peak": HBM_PEAK_GBS},
-        "seed_segments": {"GB/s": round(seg_bytes / sec(1) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                          "raw_seeds_per_search": int(seeds_raw / len(counts))},
-        "dp": {"GCUPS": round(cells / sec(4) / 1e9, 1), "Tops/s": round(14.0 * cells / sec(4) / 1e12, 2),
-               "peak_Tops/s": VALU_INT32_PEAK_TOPS, "hits_per_search": int(dp_hits / len(counts)),
-               "cells_per_search": int(cells / len(counts)),
-               "cells_16bit_frac": round(float(counts[:, 5].sum()) / max(cells, 1.0), 4),
-               "cells_band_frac": round(float(counts[:, 7].sum()) / max(cells, 1.0), 4)},
-        "results": {"GB/s": round(res_bytes / sec(5) / 1e9, 1), "peak": HBM_PEAK_GBS},
-    }
-    stage_names = ["seed_lookup", "seed_segments", "global_seed_sort", "global_seed_filters", "dp", "results",
-                   "search_total"]
-    stage_ms = {n: round(float(times[:, i].sum()), 3) for i, n in enumerate(stage_names)}
-    log(f"stage ms per step (one search at a time): {stage_ms}")
-
+    log(f"index over {n_twin} target windows built in {index_s:.2f} s")
+    _, rspec = gen.workload("serving")
+    n_warm = int(round(args.warmup * args.rate))
+    n_req = int(round(args.steps * args.rate))
+    t2 = time.perf_counter()
+    warm = serve.make_requests(rspec, t, max(n_warm, 1), args.rate, seed=100)
+    reqs = serve.make_requests(rspec, t, n_req, args.rate, seed=1)
+    mbp = sum(r.n_bases for r in reqs) / 1e6
+    log(f"{n_warm} warmup + {n_req} timed requests ({mbp:.0f} Mbp timed) generated in "
+        f"{time.perf_counter() - t2:.1f} s")
+    if n_warm:
+        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers)
+        ws = srv.run(warm, realtime=True)
+        log(f"warmup: {ws['requests']} requests, {ws['mbp_per_s']} Mbp/s, p99 {ws['p99_ms']} ms")
+    srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers)
+    stats = srv.run(reqs, realtime=True, record=True)
+    log(f"timed: {stats}")
+    wall = stats["wall_s"]
+    roof, per_kernel = kernel_roofline(srv.records, "serving")
+    cnt = np.array([r[3] for r in srv.records], dtype=np.float64).reshape(-1, 8)
+    cells16_frac = float(cnt[:, 5].sum()) / max(float(cnt[:, 2].sum()), 1.0)
     out = {
-        "metric": "query Mbp/s aligned (bit-exact hits) + p50/p99 request latency, 1 MI355X",
-        "value": round(value, 3),
+        "metric": METRIC,
+        "value": stats["mbp_per_s"],
         "unit": "query Mbp/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
+        "dp_cells_int16_frac": round(cells16_frac, 4),
+        "dtype_note": "integer scores; DP cells whose range proof allows it run in packed 16-bit lanes with "
+                      "bit-identical results (dp_cells_int16_frac of all cells)",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {t.bases.shape[0] / 1e9:.3g} Gbp target ({len(t)} seqs) x {len(r)} reads "
-                        f"({total_bases / 1e6:.0f} Mbp), reference defaults, batched resident index",
-            "target_windows": n_twin, "query_chunks": len(queries),
-            "query_windows": sum(len(w) for _, w, _ in qchunks), "hits": rows,
-            "streams": args.streams,
+            "workload": f"serving: {t.bases.shape[0] / 1e9:.2f} Gbp target ({len(t)} seqs, 40% repeats, 1% low "
+                        f"complexity) resident in HBM; {n_req} requests of 1-10,000 reads (150 bp / 1-15 kbp), "
+                        f"Poisson {args.rate:g} req/s; reference defaults",
+            "target_windows": n_twin, "requests": n_req, "query_mbp": round(mbp, 2), "workers": args.workers,
+            "step": f"one second of arrivals ({args.rate:g} requests)",
         },
-        "latency_ms": {"unit": "one query chunk (~200k reads) alone, call to rows in host memory, searched "
-                               "against the resident index of every target window",
-                       "p50": round(float(np.percentile(lat, 50)), 3),
-                       "p99": round(float(np.percentile(lat, 99)), 3)},
-        "stage_ms_per_step": stage_ms,
-        "stages": stages,
+        "latency_ms": {"p50": stats["p50_ms"], "p99": stats["p99_ms"],
+                       "unit": f"per request, arrival to rows in host memory ({n_req} requests)"},
+        "serving": stats,
         "index_build_s": round(index_s, 3),
-        "roofline": {"kernel": "k_seed_count", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roof,
+        "kernels": per_kernel,
     }
-    if args.plain:
-        tp = time.perf_counter()
-        for (lst, wins, _) in qchunks[:1]:
-            for (tl, twins) in tchunks:
-                for a, b in twins:
-                    ix = native.Index(p, tl, [(a, b)])
-                    for qa, qb in wins:
-                        q = native.Query(p, lst, [(qa, qb)])
-                        native.search(p, ix, q, a, 2 * qa).close()
-                        q.close()
-                    ix.close()
-        plain_s = time.perf_counter() - tp
-        out["plain_value"] = round(qchunks[0][2] / 1e6 / plain_s, 3)
-    if not args.no_full_checks:
-        t2 = time.perf_counter()
-        out["full_size_checks"] = full_size_checks(p, args, tchunks, qchunks, indexes, queries)
-        log(f"full-size checks in {time.perf_counter() - t2:.1f} s: {out['full_size_checks']}")
+    if not args.no_checks:
+        t3 = time.perf_counter()
+        out["full_size_checks"] = request_checks(p, reqs, tch)
+        order = sorted(range(len(reqs)), key=lambda k: len(reqs[k].offsets))
+        pick = [order[int(i)] for i in np.linspace(0, len(order) - 1, min(args.alone_sample, len(order)))]
+        same = 0
+        for k in pick:
+            alone = srv.search_alone(serve.Request(reqs[k].rid, reqs[k].bases, reqs[k].offsets))
+            same += rows_equal(reqs[k].rows, alone)
+        out["mix_equals_alone"] = {"requests": len(pick), "equal": same}
+        log(f"checks in {time.perf_counter() - t3:.1f} s: {out['full_size_checks']}, alone {same}/{len(pick)}")
+    if not args.no_capacity:
+        cap = [serve.Request(r.rid, r.bases, r.offsets, 0.0) for r in reqs[:args.capacity_requests]]
+        cs = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers).run(cap, realtime=False)
+        out["capacity"] = {"requests": cs["requests"], "query_mbp": cs["query_mbp"], "mbp_per_s": cs["mbp_per_s"],
+                           "wall_s": cs["wall_s"], "batches": cs["batches"],
+                           "note": "the first requests of the timed stream, all queued at once"}
+        log(f"capacity: {out['capacity']}")
     if not args.no_cpu_baseline:
-        cb, parity = cpu_baseline(p, tchunks, qchunks, n_twin, args.cpu_threads)
+        cb, parity = cpu_baseline_serving(p, tch, reqs, args.cpu_sample_mbp, args.cpu_threads)
         out["cpu_baseline"] = cb
         out["parity_sample"] = parity
+        log(f"cpu baseline: {cb}; parity {parity}")
+    if not args.no_sweep:
+        rates = [float(x) for x in args.sweep_rates.split(",") if x]
+        out["pressure_sweep"] = pressure_sweep(p, ix, t, rates, args.sweep_requests, args.workers)
+    ix.close()
+    if not args.no_large:
+        out["large"] = large_leg(args.large_steps, 1, 3)
+        log(f"large: {out['large']['value']} Mbp/s ({out['large']['resident_value']} resident)")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -28,9 +28,27 @@ namespace clast {
 
 // One runtime per (device, host thread): searches issued from different
 // threads run on their own streams and workspaces and overlap on the GPU.
-// Runtimes live until the process ends.
+// When a thread ends, its runtimes go to a per-device pool that the next new
+// thread draws from, so a host that keeps starting threads holds at most as
+// many runtimes as it ever ran threads at once.  Returning one makes no HIP
+// call (queued work, e.g. a result copy not yet waited for, stays ordered on
+// the runtime's streams for its next owner).  Runtimes live until the
+// process ends.
 static std::mutex g_rt_mu;
 static std::map<std::pair<int, std::thread::id>, Runtime*> g_rt;
+static std::map<int, std::vector<Runtime*>> g_rt_free;
+
+struct ThreadRuntimes {
+  std::map<int, Runtime*> mine;  // this thread's, without the lock
+  ~ThreadRuntimes() {
+    std::lock_guard<std::mutex> lk(g_rt_mu);
+    for (auto& kv : mine) {
+      g_rt.erase(std::make_pair(kv.first, std::this_thread::get_id()));
+      g_rt_free[kv.first].push_back(kv.second);
+    }
+  }
+};
+static thread_local ThreadRuntimes t_rt;
 
 PinnedPool& pinned_pool() {
   static PinnedPool* pool = new PinnedPool();
@@ -40,23 +58,27 @@ PinnedPool& pinned_pool() {
 Runtime& runtime() {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  thread_local std::map<int, Runtime*> mine;  // this thread's, without the lock
-  auto hit = mine.find(dev);
-  if (hit != mine.end()) return *hit->second;
+  auto hit = t_rt.mine.find(dev);
+  if (hit != t_rt.mine.end()) return *hit->second;
   std::lock_guard<std::mutex> lk(g_rt_mu);
-  const auto key = std::make_pair(dev, std::this_thread::get_id());
-  auto it = g_rt.find(key);
-  Runtime* rt = it != g_rt.end() ? it->second : (g_rt[key] = new Runtime(dev));
-  mine[dev] = rt;
+  Runtime* rt = nullptr;
+  auto& pool = g_rt_free[dev];
+  if (!pool.empty()) {
+    rt = pool.back();
+    pool.pop_back();
+  } else {
+    rt = new Runtime(dev);
+  }
+  g_rt[std::make_pair(dev, std::this_thread::get_id())] = rt;
+  t_rt.mine[dev] = rt;
   return *rt;
 }
 
 Runtime* runtime_if_exists() {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  std::lock_guard<std::mutex> lk(g_rt_mu);
-  auto it = g_rt.find(std::make_pair(dev, std::this_thread::get_id()));
-  return it != g_rt.end() ? it->second : nullptr;
+  auto hit = t_rt.mine.find(dev);
+  return hit != t_rt.mine.end() ? hit->second : nullptr;
 }
 
 // Scratch of clast_search_batch_noalloc.  Each device holds its own copy of
@@ -188,6 +210,7 @@ struct clast_query_s {
   std::vector<int32_t> h_qlen, h_qref, h_qwin, h_id_begin;
   bool smin_valid = false;
   double smin_key[4] = {0, 0, 0, 0};
+  std::mutex smin_mu;  // searches from several threads share the thresholds
 };
 
 struct clast_hits_s {
@@ -217,6 +240,7 @@ struct clast_hits_s {
   std::vector<int32_t> stage[5][8];
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // clast_hits_kernel_times
   // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells, long tasks, their cells
   int64_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -614,6 +638,7 @@ void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_
 
 void ensure_smin(Runtime& rt, const clast_params& p, clast_query_s& q) {
   const double key[4] = {p.cut_off, p.total_db_size, p.K, p.lambda};
+  std::lock_guard<std::mutex> lk(q.smin_mu);
   if (q.smin_valid && std::memcmp(key, q.smin_key, sizeof key) == 0) return;
   std::unordered_map<int, int32_t> memo;
   std::vector<int32_t> sm(q.nseq);
@@ -641,7 +666,9 @@ void d2h(std::vector<T>& dst, const T* src, uint64_t n, hipStream_t st) {
 // their elapsed time is the kernel's duration on the library stream.
 enum {
   EV_START, EV_COUNT0, EV_COUNT1, EV_WRITE0, EV_WRITE1, EV_SORT0, EV_SORT1, EV_FILT0, EV_FILT1, EV_DP0, EV_DP1,
-  EV_RES0, EV_RES1, kEvents
+  EV_RES0, EV_RES1,
+  // the three DP kernels, each bracketed on the stream it runs on
+  EV_D16_0, EV_D16_1, EV_D32_0, EV_D32_1, EV_DBAND0, EV_DBAND1, kEvents
 };
 
 struct StageEvents {
@@ -692,11 +719,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   if (use_cut) ensure_smin(rt, p, q);
   std::unique_ptr<StageEvents> ev(rows_out ? nullptr : new StageEvents());
   bool timed[kEvents] = {};
-  auto mark = [&](int k) {
+  auto mark_on = [&](int k, hipStream_t on) {
     if (!ev) return;
-    hip_check(hipEventRecord(ev->e[k], st), "event");
+    hip_check(hipEventRecord(ev->e[k], on), "event");
     timed[k] = true;
   };
+  auto mark = [&](int k) { mark_on(k, st); };
   mark(EV_START);
 
   const uint32_t np = q.npos;
@@ -978,20 +1006,32 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     mark(EV_DP0);
     // the long tasks (raised wave priority) and the int32 tasks run on the
     // side stream beside the 16-bit ones, so the three kernels' tails overlap
+    hipStream_t ss = rt.side_stream();
     if (packed) {
       rt.fork(0);
-      hipStream_t ss = rt.side_stream();
-      if (band) launch_dp_band(p.gap, false, da, ss);
+      if (band) {
+        mark_on(EV_DBAND0, ss);
+        launch_dp_band(p.gap, false, da, ss);
+        mark_on(EV_DBAND1, ss);
+      }
+      mark_on(EV_D32_0, ss);
       launch_dp_global(p.gap, da, ss);
+      mark_on(EV_D32_1, ss);
+      mark(EV_D16_0);
       launch_dp16(p.gap, da, st);
+      mark(EV_D16_1);
       rt.join(0);
     } else {
       if (band) {
         rt.fork(0);
-        launch_dp_band(p.gap, p.local != 0, da, rt.side_stream());
+        mark_on(EV_DBAND0, ss);
+        launch_dp_band(p.gap, p.local != 0, da, ss);
+        mark_on(EV_DBAND1, ss);
       }
+      mark(EV_D32_0);
       if (p.local) launch_dp_local(p.gap, da, st);
       else launch_dp_global(p.gap, da, st);
+      mark(EV_D32_1);
       if (band) rt.join(0);
     }
     mark(EV_DP1);
@@ -1179,6 +1219,15 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     out.times[k] = t;
   }
   out.times[7] = static_cast<float>(nseed);
+  const int kpairs[8][2] = {{EV_COUNT0, EV_COUNT1}, {EV_D16_0, EV_D16_1}, {EV_D32_0, EV_D32_1},
+                            {EV_DBAND0, EV_DBAND1}, {EV_DP0, EV_DP1},    {EV_WRITE0, EV_WRITE1},
+                            {EV_RES0, EV_RES1},     {EV_START, EV_RES1}};
+  for (int k = 0; k < 8; ++k) {
+    float t = 0;
+    if (timed[kpairs[k][0]] && timed[kpairs[k][1]])
+      hip_check(hipEventElapsedTime(&t, ev->e[kpairs[k][0]], ev->e[kpairs[k][1]]), "elapsed");
+    out.ktimes[k] = t;
+  }
   uint64_t cnt[26];
   rt.read_scalars(tot64 + 2, cnt, 26);
   uint64_t cells = 0, rows16 = 0, rowsband = 0;
@@ -1613,6 +1662,30 @@ int clast_hits_stage_times(clast_hits h, float* out8) {
   return guard([&] {
     if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
     std::memcpy(out8, h->times, sizeof h->times);
+  });
+}
+
+int clast_hits_kernel_times(clast_hits h, float* out8) {
+  return guard([&] {
+    if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out8, h->ktimes, sizeof h->ktimes);
+  });
+}
+
+int clast_mem_info(int64_t* free_bytes, int64_t* total_bytes) {
+  return guard([&] {
+    size_t f = 0, t = 0;
+    hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    if (free_bytes) *free_bytes = static_cast<int64_t>(f);
+    if (total_bytes) *total_bytes = static_cast<int64_t>(t);
+  });
+}
+
+int clast_workspace_bytes(int64_t* bytes) {
+  return guard([&] {
+    if (!bytes) throw ClastError(CLAST_ERR_ARG, "null argument");
+    Runtime* rt = runtime_if_exists();
+    *bytes = rt ? static_cast<int64_t>(rt->ws().bytes()) : 0;
   });
 }
 

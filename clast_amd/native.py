@@ -100,6 +100,9 @@ def hip() -> C.CDLL:
             "clast_hits_stage_count": (i64, [vp, i32]),
             "clast_hits_stage_copy": (C.c_int, [vp, i32] + [vp] * 8),
             "clast_hits_stage_times": (C.c_int, [vp, vp]),
+            "clast_hits_kernel_times": (C.c_int, [vp, vp]),
+            "clast_mem_info": (C.c_int, [vp, vp]),
+            "clast_workspace_bytes": (C.c_int, [vp]),
             "clast_hits_destroy": (C.c_int, [vp]),
             "clast_result_sort": (C.c_int, [i64] + [vp] * 9),
         }
@@ -143,6 +146,20 @@ def _ptr(a: np.ndarray) -> int:
 
 def init(device: int = 0) -> None:
     _check(hip().clast_init(device), "clast_init")
+
+
+def mem_info() -> tuple[int, int]:
+    """(free, total) device bytes of the calling thread's device."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    _check(hip().clast_mem_info(C.byref(f), C.byref(t)), "clast_mem_info")
+    return int(f.value), int(t.value)
+
+
+def workspace_bytes() -> int:
+    """Device bytes of the calling thread's search workspace."""
+    b = C.c_int64(0)
+    _check(hip().clast_workspace_bytes(C.byref(b)), "clast_workspace_bytes")
+    return int(b.value)
 
 
 # ---------------------------------------------------------------- host lists
@@ -416,6 +433,13 @@ class Hits:
         """seed count, seed write, seed sort, filters, DP, results, total (ms), seeds."""
         t = np.zeros(8, np.float32)
         _check(hip().clast_hits_stage_times(self.h, _ptr(t)), "clast_hits_stage_times")
+        return t
+
+    def kernel_times(self) -> np.ndarray:
+        """k_seed_count, k_dp16, k_dp, k_dp_band, DP stage, seed generation,
+        results, whole search (ms)."""
+        t = np.zeros(8, np.float32)
+        _check(hip().clast_hits_kernel_times(self.h, _ptr(t)), "clast_hits_kernel_times")
         return t
 
     def close(self) -> None:

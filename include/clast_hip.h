@@ -25,9 +25,17 @@
  * throws; the message of the last failure on the calling thread is
  * available from clast_last_error().  Host buffers belong to the caller and
  * are not retained after a call returns; device memory belongs to the handle.
- * Handles are single-owner; calls on different handles may come from
- * different threads, but the library serialises its device work on one
- * stream per device.
+ *
+ * Threads: each host thread has its own runtime on each device (a compute,
+ * a side and a copy stream, and a search workspace that grows to the
+ * largest search), so searches issued from different threads overlap on the
+ * GPU.  Index and query handles may be searched from several threads at
+ * once; searches of one query set that overlap in time must use the same
+ * cut_off, total_db_size, K and lambda (a search stores the query's
+ * per-strand score thresholds for those, under a lock of that query).
+ * Create and destroy a handle on one thread, and not while it is searched.  A thread's runtimes go to a pool when
+ * the thread ends and are reused by later threads; the pool lives until the
+ * process ends.
  */
 #ifndef CLAST_HIP_H_
 #define CLAST_HIP_H_
@@ -94,11 +102,11 @@ typedef struct clast_index_s* clast_index;
 typedef struct clast_query_s* clast_query;
 typedef struct clast_hits_s* clast_hits;
 
-/* cudaSetDevice(setting.getDeviceID()) (main.cu:30). Optional: the first
- * call on a thread without it uses device 0.  Each host thread has its own
- * streams and scratch on each device (kept until the process ends), so
- * searches issued from different threads run concurrently; indexes and
- * query sets may be shared between threads once created. */
+/* cudaSetDevice(setting.getDeviceID()) (main.cu:30), and the calling
+ * thread's runtime on that device.  Optional for every entry point but
+ * clast_search_batch_noalloc: the first call on a thread without it uses
+ * device 0.  Indexes and query sets may be shared between threads once
+ * created (see Threads above). */
 int clast_init(int32_t device);
 const char* clast_last_error(void);
 /* Library version string. */
@@ -214,6 +222,20 @@ int clast_hits_stage_times(clast_hits h, float* out8);
  * 1024 rows run one lane per band (none unless CLAST_DP_BAND=rest or all in
  * the environment), [7] their cells (included in [2]). */
 int clast_hits_counts(clast_hits h, int64_t* out8);
+/* No counterpart in the reference (rooflines): device times in milliseconds
+ * of single kernels of the search, each from a hipEvent pair on the stream
+ * it ran on: out[0] k_seed_count, [1] k_dp16 (DP in packed 16-bit lanes),
+ * [2] k_dp (int32 DP; beside k_dp16 on a second stream in global mode),
+ * [3] k_dp_band (one lane per band), [4] the DP stage, [5] seed generation
+ * (as clast_hits_stage_times out[1]), [6] the result stage, [7] the whole
+ * search.  0 for a kernel that did not run. */
+int clast_hits_kernel_times(clast_hits h, float* out8);
+/* No counterpart in the reference (memory-budgeted admission): free and
+ * total device memory of the calling thread's current device
+ * (hipMemGetInfo), and the device bytes the calling thread's search
+ * workspace holds there (it grows to the largest search and is kept). */
+int clast_mem_info(int64_t* free_bytes, int64_t* total_bytes);
+int clast_workspace_bytes(int64_t* bytes);
 int clast_hits_destroy(clast_hits h);
 
 /* resultSorting (utilResultSorting.cu:6-47): stable sort of n result rows by

@@ -33,6 +33,16 @@ class OList(C.Structure):
 _lib = None
 
 
+def use_build(dirname: str) -> str:
+    """Load the oracle from oracle/<dirname>/ (e.g. build_native, made by
+    `make -C oracle native`) instead of oracle/build/; returns the path."""
+    global LIB, EXE, _lib
+    LIB = os.path.join(HERE, dirname, "liboracle.so")
+    EXE = os.path.join(HERE, dirname, "clast_oracle")
+    _lib = None
+    return LIB
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
