@@ -977,10 +977,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     const int bmode = maxrows >= kBandRows ? band_env : 0;
     const bool band = bmode != 0;
     const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
-    const int classbit = sortbits - (band ? 2 : 1);
+    const int classbit = sortbits - 2;  // four classes (k_dp_task_keys)
     const int rowshift = std::max(0, rowbits - classbit);
-    const uint64_t cbase = band ? 0 : 1;  // lowest class in the keys
-    uint32_t* split = ws.get<uint32_t>("d.split", 2);
+    uint32_t* split = ws.get<uint32_t>("d.split", 3);
     int* band_rows = ws.get<int>("d.bandrows", 1);
     if (bmode == 2) {
       auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
@@ -996,8 +995,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
                          reinterpret_cast<unsigned long long*>(tot64 + 20)};
     k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
     radix_sort_pairs(rt, dk, dporder, ntask, sortbits);
-    k_count_below<<<1, 64, 0, st>>>(dk, ntask, (1 - cbase) << classbit, split, nullptr);
-    k_count_below<<<1, 64, 0, st>>>(dk, ntask, (2 - cbase) << classbit, split + 1, nullptr);
+    for (int c = 0; c < 3; ++c)
+      k_count_below<<<1, 64, 0, st>>>(dk, ntask, static_cast<uint64_t>(c + 1) << classbit, split + c, nullptr);
     k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
     DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
               ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres, q.bmax,
@@ -1014,11 +1013,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         launch_dp_band(p.gap, false, da, ss);
         mark_on(EV_DBAND1, ss);
       }
+      launch_dp16(p.gap, true, da, ss);
       mark_on(EV_D32_0, ss);
       launch_dp_global(p.gap, da, ss);
       mark_on(EV_D32_1, ss);
       mark(EV_D16_0);
-      launch_dp16(p.gap, da, st);
+      launch_dp16(p.gap, false, da, st);
       mark(EV_D16_1);
       rt.join(0);
     } else {

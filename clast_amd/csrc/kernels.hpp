@@ -1335,7 +1335,10 @@ struct DPArgs {
   const int32_t* twin;     // window of each target sequence
   const uint64_t* wslot;   // window slot bounds (nwin + 1)
   const uint32_t* order;   // task of thread i: (hit << 1) | direction, longest first
-  const uint32_t* split;   // order[0, split[0]) go to k_dp_band, [split[0], split[1]) to k_dp16, the rest to k_dp
+  // order[0, split[0]) go to k_dp_band, [split[0], split[1]) to k_dp16,
+  // [split[1], split[2]) to k_dp16 with 'X' past the sequence end, the rest
+  // to k_dp
+  const uint32_t* split;
   const uint32_t* qnib;    // 4-bit copies of qbase / tbase (k_pack_nibbles), read by k_dp16
   const uint32_t* tnib;
   int4* dres;              // per task 2 * hit + direction: (move + end, move, match, score)
@@ -1523,7 +1526,7 @@ __device__ __forceinline__ uint32_t snake_chunk(uint32_t w, uint32_t nchunk, uin
 template <int G, bool LOCAL>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp(DPArgs a) {
   constexpr int W = 2 * G + 1;
-  const uint32_t first = a.split[1];
+  const uint32_t first = a.split[2];
   const uint32_t n = a.n - first;
   const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const uint32_t nchunk = (n + 63) / 64;
@@ -1625,6 +1628,19 @@ static __global__ __launch_bounds__(kBlock) void k_dp_combine(DPArgs a, uint32_t
 // >= -30002 - 2 (b - 1)), so scores lie in [-30075, 30001]; a match count
 // starts at 0 or the margin -30000 and grows by at most one per row.  The
 // results equal k_dp's bit for bit.
+//
+// With EDGE, the tasks whose band crosses the end of the target sequence
+// that k_dp guards (backward: index >= tLen, forward: index < 0) and that
+// stay inside the sequence and the window on the other side.  k_dp scores
+// such an 'X' cell's diagonal -1000, and that diagonal never wins: in any
+// cell the vertical or horizontal term is at least the diagonal
+// predecessor's score less 14 (band b's horizontal term reads band b-1 of
+// the same row, whose vertical term reads band b of the row before; band 1's
+// vertical term reads band 2, whose horizontal term reads band 1), so the
+// cell takes a gap term and keeps that term's match count.  Scoring the
+// diagonal S - 15 instead leads to the same choices and values and keeps
+// 16 bits: the ring marks such positions with bit 3 of their class code
+// (no query code has it), and the diagonal subtracts 12 more there.
 
 constexpr int kPacked16MaxLen = 30000;
 
@@ -1753,7 +1769,7 @@ struct PairRows {
 // both gap terms then read it unchanged, and the diagonal term is
 // S + (4 mc - 3) = Z + 4 + 4 mc.  T[b-1] and Q hold the target and query
 // codes of both halves; mc = 1 - min(t ^ q, 1).
-template <int W, int O>
+template <int W, int O, bool EDGE>
 __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
                                          const uint32_t (&T)[W + 4], uint32_t Q) {
   s16x2 hp = pk_splat(-30000);
@@ -1763,7 +1779,12 @@ __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s
     const s16x2 h = __builtin_elementwise_max(Z[b - 1], hp + pk_splat(kGapExt));
     const u16x2 x = __builtin_bit_cast(u16x2, T[O + b - 1] ^ Q);
     const s16x2 mc = __builtin_bit_cast(s16x2, __builtin_elementwise_sub_sat(u16x2{1, 1}, x));
-    const s16x2 pt = pk_mad4(mc, Z[b]) + pk_splat(kMatch - kMismatch);
+    s16x2 pt = pk_mad4(mc, Z[b]) + pk_splat(kMatch - kMismatch);
+    if constexpr (EDGE) {
+      // an 'X' target position (bit 3 of x): the diagonal S - 15
+      const s16x2 xb = __builtin_bit_cast(s16x2, x >> u16x2{3, 3});
+      pt = pt - xb * pk_splat(12);
+    }
     const s16x2 vh = __builtin_elementwise_max(v, h);
     const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
     const uint32_t nm = pk_pick(pk_less(vh, pt), pk_bits(M[b] + mc), ngap);
@@ -1799,15 +1820,15 @@ __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x
 // so near-equal rows) in the low and high halves; a lone last task is run
 // in both.  Each half's end node is taken at its own last row.  Target and
 // query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
-template <int G>
+template <int G, bool EDGE>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
   constexpr int W = 2 * G + 1;
   // the target streams start kLead positions early, so that both streams
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
   constexpr int kInitGroups = (W + kLead) / 16;  // even
-  const uint32_t first = a.split[0];
-  const uint32_t n = a.split[1] - first;
+  const uint32_t first = a.split[EDGE ? 1 : 0];
+  const uint32_t n = a.split[EDGE ? 2 : 1] - first;
   const uint32_t npair = (n + 1) / 2;
   const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const uint32_t nchunk = (npair + 63) / 64;
@@ -1822,6 +1843,8 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   const int32_t tumax = a.t_bmax, qumax = a.q_bmax;  // one 16-slot unit per 16-byte block
   NibWin tw[2], qw[2];
   int rows[2];
+  // EDGE: ring position j (target index tb + ts j) is 'X' from j = xk[h] on
+  int xk[2] = {INT32_MAX, INT32_MAX};
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const uint32_t task = h ? task1 : task0;
@@ -1831,6 +1854,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
     rows[h] = bwd ? a.qlen[s] - (qIdx + L) : qIdx;
     const int ts = bwd ? 1 : -1;
     const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+    if constexpr (EDGE) xk[h] = bwd ? a.tlen[a.hgs[i]] - tb : tb + 1;
     tw[h].init(tn2, tumax, static_cast<int64_t>(a.tslot[a.hgs[i]]) + tb - ts * kLead, ts);
     qw[h].init(qn2, qumax, static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), ts);
   }
@@ -1868,7 +1892,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
     for (int j = 0; j < 16; ++j) {
       const uint32_t v = tp.next(j);
       const int k = 16 * g + j - kLead;
-      if (k >= 0 && k < W) T[k] = v;
+      if (k >= 0 && k < W) T[k] = EDGE ? v | (k >= xk[0] ? 0x8u : 0u) | (k >= xk[1] ? 0x80000u : 0u) : v;
     }
   }
   const int rmax = max(rows[0], rows[1]);
@@ -1885,7 +1909,8 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
     constexpr int i = decltype(oi)::value;
     const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
     T[W + i] = tp.next(j);          // target position r + W + kLead = r (mod 16)
-    dp_row16<W, i>(Z, V, M, T, Q);
+    if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
+    dp_row16<W, i, EDGE>(Z, V, M, T, Q);
     if (r + 1 == rsplit) {
       const int h = rows[0] == rsplit ? 0 : 1;
       dp16_end<G>(Z, M, a.dres, h ? task1 : task0, h, rsplit);
@@ -2185,14 +2210,16 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
 // translation unit (dp_global.hip, dp_local.hip).
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
-// k_dp16 over order[split[0], split[1]) (dp16.hip), k_dp_band over
-// order[0, split[0]) (dp_band.hip); a.n is the task count.
-void launch_dp16(int gap, const DPArgs& a, hipStream_t st);
+// k_dp16 over order[split[0], split[1]) and, with edge, over
+// order[split[1], split[2]) (dp16.hip), k_dp_band over order[0, split[0])
+// (dp_band.hip); a.n is the task count.
+void launch_dp16(int gap, bool edge, const DPArgs& a, hipStream_t st);
 void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
 // DP task order by class (key bits from classbit up): 0 the long tasks
 // (k_dp_band: with band = 1 the long tasks k_dp16 cannot take, with band = 2
-// every long task), 1 those k_dp16 can take, 2 the rest (k_dp); each class by
+// every long task), 1 those k_dp16 can take, 2 those k_dp16 can take with
+// 'X' past the sequence end (EDGE), 3 the rest (k_dp); each class by
 // rows descending (key low bits = (maxrows - rows) >> rowshift, the shift
 // keeping the class bits inside the same radix passes), so the longest
 // extensions start first and a wavefront's lanes run near-equal row counts.
@@ -2219,7 +2246,7 @@ struct TaskKeyArgs {
 static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
   __shared__ uint32_t wsum[2 * (kBlock / 64)];
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  bool p16 = false, long_task = false;
+  bool p16 = false, edge = false, long_task = false;
   int rows = 0;
   if (t < 2 * a.nhit) {
     const uint32_t i = t >> 1;
@@ -2236,18 +2263,23 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
         const int64_t last = first + span;
         const int64_t ts = static_cast<int64_t>(a.tslot[gs]);
         const int w = a.twin[gs];
-        const int64_t lo = max(static_cast<int64_t>(a.wslot[w]) - ts, int64_t{0});
-        const int64_t hi = min(static_cast<int64_t>(a.wslot[w + 1]) - ts, static_cast<int64_t>(a.tlen[gs]));
+        const int64_t wlo = static_cast<int64_t>(a.wslot[w]) - ts, whi = static_cast<int64_t>(a.wslot[w + 1]) - ts;
+        const int64_t tl = a.tlen[gs];
+        const int64_t lo = max(wlo, int64_t{0});
+        const int64_t hi = min(whi, tl);
         p16 = first >= lo && last < hi;
+        // past the guarded end only: backward reads past tLen ('X'), forward
+        // reads before 0 ('X'), the other end inside sequence and window
+        edge = !p16 && (bwd ? first >= lo && last >= tl : first < 0 && last < whi);
       }
     }
-    long_task = a.band && rows >= *a.band_rows && (a.band == 2 || !p16);
-    if (long_task) p16 = false;
-    const uint64_t cls = long_task ? 0u : p16 ? 1u : 2u;
-    a.key[t] = ((cls - (a.band ? 0u : 1u)) << a.classbit) | static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
+    long_task = a.band && rows >= *a.band_rows && (a.band == 2 || !(p16 || edge));
+    if (long_task) p16 = edge = false;
+    const uint64_t cls = long_task ? 0u : p16 ? 1u : edge ? 2u : 3u;
+    a.key[t] = (cls << a.classbit) | static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
     a.task[t] = t;
   }
-  const uint32_t r16 = wave_sum_incl(p16 ? static_cast<uint32_t>(rows) : 0u);
+  const uint32_t r16 = wave_sum_incl(p16 || edge ? static_cast<uint32_t>(rows) : 0u);
   const uint32_t rb = wave_sum_incl(long_task ? static_cast<uint32_t>(rows) : 0u);
   if (lane_id() == 63) {
     wsum[wave_id()] = r16;
@@ -2298,11 +2330,11 @@ static __global__ void k_band_threshold(const unsigned long long* __restrict__ s
   *band_rows = static_cast<int>(max(static_cast<unsigned long long>(kBandRows), min(lim, 0x7FFFFFFFull)));
 }
 
-// The 16-bit and long task counts (split[1] - split[0], split[0]) into the
+// The 16-bit and long task counts (split[2] - split[0], split[0]) into the
 // work counters.
 static __global__ void k_split_counts(const uint32_t* __restrict__ split, unsigned long long* __restrict__ out) {
   if (threadIdx.x == 0) {
-    out[0] = split[1] - split[0];
+    out[0] = split[2] - split[0];
     out[1] = split[0];
   }
 }

@@ -491,3 +491,54 @@ def test_large_pair_all_rows_filtered(cut_off):
         h.close()
     assert len(exp["tID"]) == (300 if cut_off == 10.0 else 0)
     assert_rows_equal(got, exp, what=f"large pair cutOff {cut_off}")
+
+
+@pytest.mark.parametrize("gap", [2, 8, 16])
+def test_dp_16bit_past_sequence_end(gap, monkeypatch):
+    # Reads that run off a target sequence: the last part of a sequence
+    # followed by 1 to `gap` random bases (the backward extension's band
+    # reaches past the end, whose positions k_dp scores as 'X'), and 1 to
+    # `gap` random bases followed by the first part of a sequence (the
+    # forward band reaches before 0), 150 bp and 1.5-4 kbp long (a longer
+    # overhang fails the corner filter: no hit).  Those tasks run in the
+    # 16-bit kernel with 'X' marked past the end: every task with rows runs
+    # in 16-bit lanes, and the rows equal the oracle's and the int32 kernel's.
+    rng = np.random.default_rng(100 + gap)
+    t, _ = workload(n_t=4, t_len=30_000, n_r=1, n_rate=0.0, seed=17)
+    seqs = []
+    for k in range(4):
+        s = t.seq(k)
+        for length in (150, 1500, 4000):
+            for over in sorted({1, 2, max(1, gap // 2), gap}):
+                junk = bytes(rng.choice(list(b"ACGT"), over).astype(np.uint8))
+                seqs.append(s[-(length - over):] + junk)
+                seqs.append(junk + s[:length - over])
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t), gap=gap)
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 70_000, query=False)
+    qwins = native.windows(ql.gateway, 60_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    print(f"dp16 past the end gap={gap}: {len(twins)}x{len(qwins)} windows", flush=True)
+    h = native.search_batch(p, ix, q)
+    got, cnt = h.arrays(), h.counts()
+    monkeypatch.setenv("CLAST_DP16", "0")
+    got0 = native.search_batch(p, ix, q).arrays()
+    ix.close()
+    q.close()
+    o = oracle.search(p, tl, twins, ql, qwins)["final"]
+    assert len(o["tID"]) >= len(seqs)
+    assert_rows_equal(got, o, what=f"dp16 past the end, gap {gap}")
+    assert_rows_equal(got0, o, what=f"int32 dp past the end, gap {gap}")
+    # every task with rows took the 16-bit kernels (each hit has one or
+    # two), but for gap > lMer - 1: then a backward band from a seed at the
+    # sequence start also reads before it (the side k_dp does not guard,
+    # real bases of the window), and such a task stays on the int32 kernel
+    if gap < p.lmer:
+        assert cnt[4] >= cnt[1], (cnt[4], cnt[1])
+        assert cnt[5] == cnt[2]
+    else:
+        assert 0 < cnt[5] < cnt[2]
