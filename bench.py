@@ -453,6 +453,7 @@ def main() -> int:
         "index_build_s": round(index_s, 3),
         "roofline": roof,
         "kernels": per_kernel,
+        "dp_tuner": native.tuner_report().splitlines(),
     }
     if not args.no_checks:
         t3 = time.perf_counter()

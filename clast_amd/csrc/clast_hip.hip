@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "prims.hpp"
 #include "runtime.hpp"
+#include "tuner.hpp"
 
 namespace clast {
 
@@ -241,6 +242,7 @@ struct clast_hits_s {
   int64_t stage_n[5] = {0, 0, 0, 0, 0};
   float times[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float ktimes[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // clast_hits_kernel_times
+  clast::DPConfig dpcfg;  // DP configuration the search ran with
   // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells, long tasks, their cells
   int64_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -290,24 +292,6 @@ static bool dp16_disabled() {
   const char* f = std::getenv("CLAST_DP16");
   return f && !std::strcmp(f, "0");
 }
-// Which long extensions (at least kBandRows rows) run on the lane-per-band
-// kernel (k_dp_band: about five times the instructions per cell, a long task
-// finishing sooner), from CLAST_DP_BAND: off (default, 0) none; rest (1)
-// those k_dp16 cannot take (bands that leave the target sequence, star / X
-// letters, local mode), which otherwise run alone in one int32 lane; all (2)
-// every extension over twice one SIMD's average share of rows.  Every
-// setting gives the same rows.  Measured: serving p99 averaged about 12%
-// higher with either of the others (2-3 runs each; the kernel's raised wave
-// priority takes issue slots from the 16-bit DP of searches beside it), and
-// off had the best long-read batch throughput.
-static int band_mode() {
-  const char* f = std::getenv("CLAST_DP_BAND");
-  if (!f || !std::strcmp(f, "off")) return 0;
-  if (!std::strcmp(f, "rest")) return 1;
-  if (!std::strcmp(f, "all")) return 2;
-  throw ClastError(CLAST_ERR_ARG, "CLAST_DP_BAND must be off, rest or all");
-}
-
 // Cell lookup over the top B key bits of the sorted entries (keys in ek,
 // u64): presence bitmap + rank directory + a record per present prefix, or a
 // direct record table; keeps the keys as u32 or u64.
@@ -708,7 +692,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   if (ix.lmer != p.lmer || q.lmer != p.lmer || ix.stride != p.stride || ix.cut_repeat != p.cut_repeat)
     throw ClastError(CLAST_ERR_ARG, "params differ from those the index/query were built with");
   if (ix.device != q.device) throw ClastError(CLAST_ERR_ARG, "index and query on different devices");
-  const int band_env = band_mode();  // read (and checked) before anything is queued
+  const DPForced forced = DPForced::from_env();  // read (and checked) before anything is queued
   set_device(ix.device);
   Runtime& rt = runtime();
   std::lock_guard<std::mutex> lk(rt.mutex());
@@ -971,71 +955,134 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     int4* dres = ws.get<int4>("d.res", ntask);
     const int maxrows = std::max(q.max_qlen - p.lmer, 0);
     const int rowbits = bits_for(static_cast<uint64_t>(maxrows) + 1);
-    // class bits on top of the rows (two with long tasks possible, else one:
-    // classes 1 and 2 only), within the radix passes the rows need
     const bool packed = !p.local && !dp16_disabled();
-    const int bmode = maxrows >= kBandRows ? band_env : 0;
-    const bool band = bmode != 0;
-    const int sortbits = std::max(kRadixBits, (rowbits + kRadixBits - 1) / kRadixBits * kRadixBits);
-    const int classbit = sortbits - 2;  // four classes (k_dp_task_keys)
-    const int rowshift = std::max(0, rowbits - classbit);
     uint32_t* split = ws.get<uint32_t>("d.split", 3);
     int* band_rows = ws.get<int>("d.bandrows", 1);
-    if (bmode == 2) {
-      auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
-      k_dp_rows_sum<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, rsum);
-      k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * rt.compute_units() * 128ull, band_rows);
-    } else if (bmode == 1) {
-      k_fill<int><<<1, 64, 0, st>>>(band_rows, 1, kBandRows);
-    }
-    const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
-                         ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
-                         bmode, band_rows, maxrows, rowshift, classbit, dk, dporder,
-                         reinterpret_cast<unsigned long long*>(tot64 + 10),
-                         reinterpret_cast<unsigned long long*>(tot64 + 20)};
-    k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
-    radix_sort_pairs(rt, dk, dporder, ntask, sortbits);
-    for (int c = 0; c < 3; ++c)
-      k_count_below<<<1, 64, 0, st>>>(dk, ntask, static_cast<uint64_t>(c + 1) << classbit, split + c, nullptr);
-    k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
-    DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
-              ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres, q.bmax,
-              ix.bmax, p.lmer, static_cast<uint32_t>(4 * std::max(rt.compute_units(), 1)), ot, oq, thl, qhl, om,
-              osc};
-    mark(EV_DP0);
-    // the long tasks (raised wave priority) and the int32 tasks run on the
-    // side stream beside the 16-bit ones, so the three kernels' tails overlap
+    uint32_t* chunk_next = ws.get<uint32_t>("d.chunk", 1);
+    const uint32_t cu = static_cast<uint32_t>(std::max(rt.compute_units(), 1));
     hipStream_t ss = rt.side_stream();
-    if (packed) {
-      rt.fork(0);
-      if (band) {
-        mark_on(EV_DBAND0, ss);
-        launch_dp_band(p.gap, false, da, ss);
-        mark_on(EV_DBAND1, ss);
+    // The DP stage under configuration c (tuner.hpp); `final` marks the run
+    // whose kernels are timed and whose counters are kept.
+    auto dp_chain = [&](const DPConfig& c, bool final) -> DPArgs {
+      auto fmark = [&](int k) {
+        if (final) mark(k);
+      };
+      auto fmark_on = [&](int k, hipStream_t on) {
+        if (final) mark_on(k, on);
+      };
+      const int bmode = maxrows >= kBandRows ? c.band : 0;
+      const bool band = bmode != 0;
+      // the class bits on top of the rows, inside the radix passes the rows
+      // need (the row key drops its low bits when they do not fit)
+      const int d = c.radix;
+      const int sortbits = std::max(d, (rowbits + d - 1) / d * d);
+      const int classbit = sortbits - 2;  // four classes (k_dp_task_keys)
+      const int rowshift = std::max(0, rowbits - classbit);
+      k_fill<uint64_t><<<1, 64, 0, st>>>(tot64 + 10, 26, 0ull);  // this stage's counters
+      if (bmode == 2) {
+        auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
+        k_dp_rows_sum<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, rsum);
+        k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * cu * 128ull, band_rows);
+      } else if (bmode == 1) {
+        k_fill<int><<<1, 64, 0, st>>>(band_rows, 1, kBandRows);
       }
-      launch_dp16(p.gap, true, da, ss);
-      mark_on(EV_D32_0, ss);
-      launch_dp_global(p.gap, da, ss);
-      mark_on(EV_D32_1, ss);
-      mark(EV_D16_0);
-      launch_dp16(p.gap, false, da, st);
-      mark(EV_D16_1);
-      rt.join(0);
-    } else {
-      if (band) {
+      const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
+                           ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
+                           bmode, band_rows, maxrows, rowshift, classbit, dk, dporder,
+                           reinterpret_cast<unsigned long long*>(tot64 + 10),
+                           reinterpret_cast<unsigned long long*>(tot64 + 20)};
+      k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
+      radix_sort_pairs(rt, dk, dporder, ntask, sortbits, d);
+      for (int k = 0; k < 3; ++k)
+        k_count_below<<<1, 64, 0, st>>>(dk, ntask, static_cast<uint64_t>(k + 1) << classbit, split + k, nullptr);
+      k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
+      if (c.sched == 1) k_fill<uint32_t><<<1, 64, 0, st>>>(chunk_next, 1, 0u);
+      DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
+                ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres,
+                q.bmax, ix.bmax, p.lmer, 4 * cu, chunk_next, ot, oq, thl, qhl, om, osc};
+      fmark(EV_DP0);
+      // the long tasks (raised wave priority), the 16-bit tasks past a
+      // sequence end and the int32 tasks run on the side stream beside the
+      // 16-bit ones, so the kernels' tails overlap
+      if (packed) {
         rt.fork(0);
-        mark_on(EV_DBAND0, ss);
-        launch_dp_band(p.gap, p.local != 0, da, ss);
-        mark_on(EV_DBAND1, ss);
+        if (band) {
+          fmark_on(EV_DBAND0, ss);
+          launch_dp_band(p.gap, false, da, ss);
+          fmark_on(EV_DBAND1, ss);
+        }
+        launch_dp16(p.gap, true, false, 0, da, ss);
+        fmark_on(EV_D32_0, ss);
+        launch_dp_global(p.gap, da, ss);
+        fmark_on(EV_D32_1, ss);
+        fmark(EV_D16_0);
+        launch_dp16(p.gap, false, c.sched == 1, 4 * cu, da, st);
+        fmark(EV_D16_1);
+        rt.join(0);
+      } else {
+        if (band) {
+          rt.fork(0);
+          fmark_on(EV_DBAND0, ss);
+          launch_dp_band(p.gap, p.local != 0, da, ss);
+          fmark_on(EV_DBAND1, ss);
+        }
+        fmark(EV_D32_0);
+        if (p.local) launch_dp_local(p.gap, da, st);
+        else launch_dp_global(p.gap, da, st);
+        fmark(EV_D32_1);
+        if (band) rt.join(0);
       }
-      mark(EV_D32_0);
-      if (p.local) launch_dp_local(p.gap, da, st);
-      else launch_dp_global(p.gap, da, st);
-      mark(EV_D32_1);
-      if (band) rt.join(0);
+      fmark(EV_DP1);
+      rt.check_launch("k_dp");
+      return da;
+    };
+    // configuration: the tuner's for this kind of search, found now by timed
+    // trials (coordinate descent over sched, band, radix; every trial writes
+    // every task's result, so the trials change no row) if it has none yet
+    const DPTuner::Key tkey{p.gap, p.local ? 1 : 0, dp_bucket(maxrows)};
+    DPConfig cfg;
+    const bool tuned = DPTuner::get().lookup(tkey, &cfg);
+    cfg = forced.apply(tuned ? cfg : DPConfig{});
+    // (the allocation-free entry point runs no trials: they create events)
+    if (!tuned && forced.tune && !rows_out) {
+      hipEvent_t e0, e1;
+      hip_check(hipEventCreate(&e0), "hipEventCreate");
+      hip_check(hipEventCreate(&e1), "hipEventCreate");
+      std::vector<std::pair<DPConfig, float>> trials;
+      auto trial = [&](const DPConfig& c) {
+        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        (void)dp_chain(c, false);
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        trials.emplace_back(c, ms);
+        return ms;
+      };
+      (void)trial(cfg);  // first launches of the kernels, not compared
+      trials.clear();
+      float best = trial(cfg);
+      auto consider = [&](DPConfig c) {
+        c = forced.apply(c);
+        for (auto& t : trials)
+          if (t.first == c) return;
+        const float ms = trial(c);
+        if (ms < best) {
+          best = ms;
+          cfg = c;
+        }
+      };
+      if (packed && forced.sched < 0) consider(DPConfig{1 - cfg.sched, cfg.band, cfg.radix});
+      if (maxrows >= kBandRows && forced.band < 0)
+        for (int b = 0; b < 3; ++b) consider(DPConfig{cfg.sched, b, cfg.radix});
+      if (forced.radix < 0)
+        for (int r : {8, 6, 5}) consider(DPConfig{cfg.sched, cfg.band, r});
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      DPTuner::get().store(tkey, cfg, trials);
     }
-    mark(EV_DP1);
-    rt.check_launch("k_dp");
+    const DPArgs da = dp_chain(cfg, true);
+    out.dpcfg = cfg;
     k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);
     rt.check_launch("k_dp_combine");
   }
@@ -1670,6 +1717,25 @@ int clast_hits_kernel_times(clast_hits h, float* out8) {
     if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
     std::memcpy(out8, h->ktimes, sizeof h->ktimes);
   });
+}
+
+int clast_hits_dp_config(clast_hits h, int32_t* out3) {
+  return guard([&] {
+    if (!h || !out3) throw ClastError(CLAST_ERR_ARG, "null argument");
+    out3[0] = h->dpcfg.sched;
+    out3[1] = h->dpcfg.band;
+    out3[2] = h->dpcfg.radix;
+  });
+}
+
+int64_t clast_tuner_report(char* buf, int64_t cap) {
+  const std::string r = DPTuner::get().report();
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(r.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  return static_cast<int64_t>(r.size());
 }
 
 int clast_mem_info(int64_t* free_bytes, int64_t* total_bytes) {

@@ -1345,6 +1345,7 @@ struct DPArgs {
   int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
   int lmer;
   uint32_t round;          // wavefronts per dispatch round (one per SIMD): k_dp / k_dp16 snake order
+  uint32_t* chunk_next;    // persistent k_dp16: next chunk to take (zero before the launch)
   int32_t* o_tidx;
   int32_t* o_qidx;
   int32_t* o_thl;
@@ -1820,20 +1821,16 @@ __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x
 // so near-equal rows) in the low and high halves; a lone last task is run
 // in both.  Each half's end node is taken at its own last row.  Target and
 // query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
+// One chunk of 64 task pairs: lane l runs pair 64 chunk + l of the
+// kernel's task range [first, first + n).
 template <int G, bool EDGE>
-static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
+__device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint32_t n, uint32_t chunk) {
   constexpr int W = 2 * G + 1;
   // the target streams start kLead positions early, so that both streams
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
   constexpr int kInitGroups = (W + kLead) / 16;  // even
-  const uint32_t first = a.split[EDGE ? 1 : 0];
-  const uint32_t n = a.split[EDGE ? 2 : 1] - first;
-  const uint32_t npair = (n + 1) / 2;
-  const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const uint32_t nchunk = (npair + 63) / 64;
-  if (wv >= nchunk) return;
-  const uint32_t t = snake_chunk(wv, nchunk, a.round) * 64 + (threadIdx.x & 63u);
+  const uint32_t t = chunk * 64 + (threadIdx.x & 63u);
   if (2 * t >= n) return;
   const uint32_t task0 = a.order[first + 2 * t];
   const uint32_t task1 = 2 * t + 1 < n ? a.order[first + 2 * t + 1] : task0;
@@ -1964,6 +1961,34 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   // the longer half (both when equal; a lone task's copy writes the same)
   if (rows[0] >= rows[1] && rows[0] > 0) dp16_end<G>(Z, M, a.dres, task0, 0, rows[0]);
   if (rows[1] >= rows[0] && rows[1] > 0) dp16_end<G>(Z, M, a.dres, task1, 1, rows[1]);
+}
+
+// The chunks of the range go out longest first.  Static (PERSIST false):
+// one wavefront per chunk in snake order (snake_chunk), the dispatcher
+// starting workgroups as others finish.  Persistent (PERSIST true): a grid
+// of about one occupancy round, each wavefront taking the next chunk from a
+// device counter (a.chunk_next, zero before the launch) until none is left,
+// so a wavefront that finishes early takes over the remaining work instead
+// of waiting for its workgroup; every wavefront exits once the counter
+// passes the last chunk.
+template <int G, bool EDGE, bool PERSIST>
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
+  const uint32_t first = a.split[EDGE ? 1 : 0];
+  const uint32_t n = a.split[EDGE ? 2 : 1] - first;
+  const uint32_t nchunk = ((n + 1) / 2 + 63) / 64;
+  if constexpr (PERSIST) {
+    for (;;) {
+      uint32_t c = 0;
+      if ((threadIdx.x & 63u) == 0) c = atomicAdd(a.chunk_next, 1u);
+      c = __builtin_amdgcn_readfirstlane(c);
+      if (c >= nchunk) return;
+      dp16_chunk<G, EDGE>(a, first, n, c);
+    }
+  } else {
+    const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (wv >= nchunk) return;
+    dp16_chunk<G, EDGE>(a, first, n, snake_chunk(wv, nchunk, a.round));
+  }
 }
 
 // ------------------------------------------- banded DP, one lane per band
@@ -2213,7 +2238,7 @@ void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
 // k_dp16 over order[split[0], split[1]) and, with edge, over
 // order[split[1], split[2]) (dp16.hip), k_dp_band over order[0, split[0])
 // (dp_band.hip); a.n is the task count.
-void launch_dp16(int gap, bool edge, const DPArgs& a, hipStream_t st);
+void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st);
 void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
 // DP task order by class (key bits from classbit up): 0 the long tasks

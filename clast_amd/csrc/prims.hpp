@@ -242,10 +242,12 @@ void device_scan(Runtime& rt, const Tin* in, uint64_t n, Tout* out, Tacc* total,
   rt.check_launch("device_scan");
 }
 
-// Stable sort of (keys, vals) by key bits [0, nbits).  Uses workspace
-// "sort.*"; results land back in keys/vals.
-inline void radix_sort_pairs(Runtime& rt, uint64_t* keys, uint32_t* vals, uint32_t n, int nbits) {
+// Stable sort of (keys, vals) by key bits [0, nbits), `digit` bits (1..8)
+// per pass.  Uses workspace "sort.*"; results land back in keys/vals.
+inline void radix_sort_pairs(Runtime& rt, uint64_t* keys, uint32_t* vals, uint32_t n, int nbits,
+                             int digit = kRadixBits) {
   if (n <= 1 || nbits <= 0) return;
+  if (digit < 1 || digit > kRadixBits) throw ClastError(1, "radix digit out of range");
   const uint32_t ntiles = grid_for(n, kSortTile);
   uint64_t* k2 = rt.ws().get<uint64_t>("sort.k2", n);
   uint32_t* v2 = rt.ws().get<uint32_t>("sort.v2", n);
@@ -255,11 +257,13 @@ inline void radix_sort_pairs(Runtime& rt, uint64_t* keys, uint32_t* vals, uint32
   uint64_t* kout = k2;
   uint32_t* vout = v2;
   int passes = 0;
-  for (int shift = 0; shift < nbits; shift += kRadixBits, ++passes) {
-    const int db = nbits - shift < kRadixBits ? nbits - shift : kRadixBits;
+  for (int shift = 0; shift < nbits; shift += digit, ++passes) {
+    const int db = nbits - shift < digit ? nbits - shift : digit;
     const uint32_t mask = (1u << db) - 1u;
     k_radix_hist<<<ntiles, kBlock, 0, rt.stream()>>>(kin, n, shift, mask, counts, ntiles);
-    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, counts, static_cast<uint64_t>(ntiles) * kRadixBins, counts,
+    // counts are digit-major: the first 2^db rows hold every used digit
+    // (the rest stay zero and are never read for a key)
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, counts, static_cast<uint64_t>(ntiles) << db, counts,
                                                      static_cast<uint32_t*>(nullptr), false);
     k_radix_scatter<<<ntiles, kBlock, 0, rt.stream()>>>(kin, vin, kout, vout, n, shift, db, mask, counts, ntiles);
     rt.check_launch("radix pass");

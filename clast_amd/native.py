@@ -101,6 +101,8 @@ def hip() -> C.CDLL:
             "clast_hits_stage_copy": (C.c_int, [vp, i32] + [vp] * 8),
             "clast_hits_stage_times": (C.c_int, [vp, vp]),
             "clast_hits_kernel_times": (C.c_int, [vp, vp]),
+            "clast_hits_dp_config": (C.c_int, [vp, vp]),
+            "clast_tuner_report": (i64, [C.c_char_p, i64]),
             "clast_mem_info": (C.c_int, [vp, vp]),
             "clast_workspace_bytes": (C.c_int, [vp]),
             "clast_hits_destroy": (C.c_int, [vp]),
@@ -153,6 +155,14 @@ def mem_info() -> tuple[int, int]:
     f, t = C.c_int64(0), C.c_int64(0)
     _check(hip().clast_mem_info(C.byref(f), C.byref(t)), "clast_mem_info")
     return int(f.value), int(t.value)
+
+
+def tuner_report() -> str:
+    """The DP autotuner's table: one line per kind of search tuned."""
+    n = int(hip().clast_tuner_report(None, 0))
+    buf = C.create_string_buffer(n + 1)
+    hip().clast_tuner_report(buf, n + 1)
+    return buf.value.decode()
 
 
 def workspace_bytes() -> int:
@@ -441,6 +451,13 @@ class Hits:
         t = np.zeros(8, np.float32)
         _check(hip().clast_hits_kernel_times(self.h, _ptr(t)), "clast_hits_kernel_times")
         return t
+
+    def dp_config(self) -> tuple[int, int, int]:
+        """(16-bit schedule, lane-per-band mode, radix digit bits) the
+        search's DP ran with."""
+        c = np.zeros(3, np.int32)
+        _check(hip().clast_hits_dp_config(self.h, _ptr(c)), "clast_hits_dp_config")
+        return int(c[0]), int(c[1]), int(c[2])
 
     def close(self) -> None:
         if self.h:

@@ -230,6 +230,21 @@ int clast_hits_counts(clast_hits h, int64_t* out8);
  * (as clast_hits_stage_times out[1]), [6] the result stage, [7] the whole
  * search.  0 for a kernel that did not run. */
 int clast_hits_kernel_times(clast_hits h, float* out8);
+/* No counterpart in the reference (the DP autotuner): the DP launch
+ * configuration the search ran with, out[0] schedule of the 16-bit kernel
+ * (0 static, 1 persistent work-stealing wavefronts), [1] extensions run one
+ * lane per band (0 none, 1 those the 16-bit kernel cannot take, 2 every one
+ * over twice a SIMD's share), [2] radix digit bits of the task-key sort.
+ * The configuration of each kind of search (gap, global / local, length
+ * bucket of its longest extension: < 256, < 2048, < 8192 rows, longer) is
+ * picked by timed trials on the first such search in the process (rows are
+ * the same under every configuration); CLAST_TUNE=0 turns trials off and
+ * CLAST_DP_SCHED=static|persist, CLAST_DP_BAND=off|rest|all and
+ * CLAST_RADIX_BITS=4..8 fix a knob.  clast_tuner_report writes the table
+ * (one line per kind, with the trial times) into buf (cap bytes, NUL
+ * terminated) and returns its full length. */
+int clast_hits_dp_config(clast_hits h, int32_t* out3);
+int64_t clast_tuner_report(char* buf, int64_t cap);
 /* No counterpart in the reference (memory-budgeted admission): free and
  * total device memory of the calling thread's current device
  * (hipMemGetInfo), and the device bytes the calling thread's search

@@ -276,6 +276,7 @@ def test_dp_16bit_lanes(gap, monkeypatch):
     # letters, reads from sequence ends, read lengths 60-1500 (two tasks of
     # unequal rows share a lane).  Rows equal the oracle's with the 16-bit
     # path on and off.
+    monkeypatch.setenv("CLAST_DP_BAND", "off")  # no extension leaves the 16-bit kernels for k_dp_band
     t, r = workload(n_t=5, t_len=20_000, n_r=240, len_mode=gen.LEN_LOGUNIFORM, read_len=60, len_b=1500,
                     indel=0.02, seed=11)
     tb = t.bases.copy()
@@ -325,7 +326,7 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     # the oracle's with every long extension on the lane-per-band kernel
     # (CLAST_DP_BAND=all), with only those the 16-bit kernel cannot take
     # (rest: here the reads with X letters and those running off a sequence
-    # end; in local mode all), and with none (off, the default).
+    # end; in local mode all), with none (off), and with the default (the autotuner's pick).
     t, _ = workload(n_t=3, t_len=50_000, n_r=1, seed=13)
     long_r = gen.make_reads(gen.ReadSpec(31, 16, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=6000,
                                          sub_rate=0.02, indel_rate=0.01), t)
@@ -364,7 +365,8 @@ def test_dp_lane_per_band(local, gap, monkeypatch):
     o = oracle.search(p, tl, twins, ql, qwins)
     assert len(o["final"]["tID"]) > 0
     c_all, c_rest, c_off, c_def = (runs[m][1] for m in ("all", "rest", "off", None))
-    assert c_all[6] >= c_rest[6] > 0 and c_off[6] == 0 and c_def[6] == 0, (c_all[6], c_rest[6], c_off[6])
+    # (the default is whatever the DP autotuner picked for this kind)
+    assert c_all[6] >= c_rest[6] > 0 and c_off[6] == 0, (c_all[6], c_rest[6], c_off[6])
     assert c_all[2] == c_rest[2] == c_off[2] == c_def[2]
     if local:
         assert c_rest[6] == c_all[6]  # no 16-bit kernel in local mode
@@ -503,6 +505,7 @@ def test_dp_16bit_past_sequence_end(gap, monkeypatch):
     # overhang fails the corner filter: no hit).  Those tasks run in the
     # 16-bit kernel with 'X' marked past the end: every task with rows runs
     # in 16-bit lanes, and the rows equal the oracle's and the int32 kernel's.
+    monkeypatch.setenv("CLAST_DP_BAND", "off")  # no extension leaves the 16-bit kernels for k_dp_band
     rng = np.random.default_rng(100 + gap)
     t, _ = workload(n_t=4, t_len=30_000, n_r=1, n_rate=0.0, seed=17)
     seqs = []
@@ -542,3 +545,84 @@ def test_dp_16bit_past_sequence_end(gap, monkeypatch):
         assert cnt[5] == cnt[2]
     else:
         assert 0 < cnt[5] < cnt[2]
+
+
+def _dp_mix_workload(seed):
+    """Short and long reads (1-3 kbp: extensions over 1024 rows), some off a
+    sequence end, some with X / star letters, over two target windows."""
+    t, _ = workload(n_t=4, t_len=40_000, n_r=1, seed=seed)
+    tb = t.bases.copy()
+    o1 = int(t.offsets[1])
+    tb[o1 + 2000:o1 + 40_000:1777] = ord("X")
+    t = gen.Target(tb, t.offsets)
+    long_r = gen.make_reads(gen.ReadSpec(seed + 1, 40, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=3000,
+                                         sub_rate=0.03, indel_rate=0.03), t)
+    short_r = gen.make_reads(gen.ReadSpec(seed + 2, 120, sub_rate=0.04, indel_rate=0.01), t)
+    seqs = [long_r.seq(i) for i in range(len(long_r))] + [short_r.seq(i) for i in range(len(short_r))]
+    seqs += [t.seq(0)[-1400:] + b"ACG", b"TTGA" + t.seq(2)[:1300], t.seq(3)[-200:]]
+    b = bytearray(seqs[3])
+    b[len(b) // 3] = ord("Q")
+    seqs[3] = bytes(b)
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    return t, gen.Reads(bases, off)
+
+
+def test_dp_every_configuration_gives_the_oracle_rows(monkeypatch):
+    # Each DP launch configuration the autotuner chooses between: the 16-bit
+    # kernel's schedule (static snake / persistent work-stealing), which
+    # long extensions run one lane per band (off / rest / all) and the radix
+    # digit of the task-key sort (8 / 5 bits).  Rows equal the oracle's under
+    # every one, and each search reports the configuration it ran with.
+    t, r = _dp_mix_workload(61)
+    p = native.Params(total_db_size=db_size(t))
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 90_000, query=False)
+    qwins = native.windows(ql.gateway, 80_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    exp = oracle.search(p, tl, twins, ql, qwins)["final"]
+    assert len(exp["tID"]) > 100
+    monkeypatch.setenv("CLAST_TUNE", "0")
+    for sched in ("static", "persist"):
+        for band in ("off", "rest", "all"):
+            for radix in (8, 5):
+                monkeypatch.setenv("CLAST_DP_SCHED", sched)
+                monkeypatch.setenv("CLAST_DP_BAND", band)
+                monkeypatch.setenv("CLAST_RADIX_BITS", str(radix))
+                print(f"dp config {sched} {band} {radix}", flush=True)
+                h = native.search_batch(p, ix, q)
+                assert h.dp_config() == (sched == "persist", ("off", "rest", "all").index(band), radix)
+                assert_rows_equal(h.arrays(), exp, what=f"dp config {sched}/{band}/{radix}")
+                h.close()
+    ix.close()
+    q.close()
+
+
+def test_dp_tuner_keeps_the_rows(monkeypatch):
+    # With trials on (the default), the first search of a kind times the
+    # configurations and keeps the fastest; its rows and those of the next
+    # search (which reuses the choice) equal the oracle's.
+    for k in ("CLAST_TUNE", "CLAST_DP_SCHED", "CLAST_DP_BAND", "CLAST_RADIX_BITS"):
+        monkeypatch.delenv(k, raising=False)
+    t, r = _dp_mix_workload(71)
+    p = native.Params(total_db_size=db_size(t), gap=7)  # a kind no other test tunes
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 90_000, query=False)
+    qwins = native.windows(ql.gateway, 80_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    exp = oracle.search(p, tl, twins, ql, qwins)["final"]
+    print("dp tuner", flush=True)
+    h1 = native.search_batch(p, ix, q)
+    rep = native.tuner_report()
+    line = [x for x in rep.splitlines() if x.startswith("gap=7 local=0 bucket=2")]
+    assert line, rep
+    h2 = native.search_batch(p, ix, q)
+    assert h1.dp_config() == h2.dp_config()
+    want = "sched=%d band=%d radix=%d" % h2.dp_config()
+    assert want in line[0], (want, line[0])
+    assert_rows_equal(h1.arrays(), exp, what="tuned first search")
+    assert_rows_equal(h2.arrays(), exp, what="tuned second search")
+    for h in (h1, h2, ix, q):
+        h.close()
