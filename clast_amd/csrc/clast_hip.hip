@@ -292,6 +292,14 @@ static bool dp16_disabled() {
   const char* f = std::getenv("CLAST_DP16");
   return f && !std::strcmp(f, "0");
 }
+// CLAST_GRAPH=0 launches the DP stage's task-ordering chain kernel by
+// kernel; by default it is captured once per shape into a hipGraph and
+// replayed.  Both give the same rows.
+static bool graphs_disabled() {
+  const char* f = std::getenv("CLAST_GRAPH");
+  return f && !std::strcmp(f, "0");
+}
+
 // Cell lookup over the top B key bits of the sorted entries (keys in ek,
 // u64): presence bitmap + rank directory + a record per present prefix, or a
 // direct record table; keeps the keys as u32 or u64.
@@ -948,10 +956,14 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
     thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
     om = ws.get<int32_t>("d.m", nhit); osc = ws.get<int32_t>("d.s", nhit);
-    // one task per (hit, direction), longest first
+    // one task per (hit, direction), longest first; the ordering chain runs
+    // over `nslot` task slots, the task count rounded up to a quarter of its
+    // power of two, so that a captured chain serves every search of that size
     const uint32_t ntask = 2 * nhit;
-    uint64_t* dk = ws.get<uint64_t>("d.key", ntask);
-    uint32_t* dporder = ws.get<uint32_t>("d.order", ntask);
+    const uint32_t qstep = std::max<uint32_t>(256u, (1u << (31 - __builtin_clz(ntask))) / 4u);
+    const uint32_t nslot = (ntask + qstep - 1) / qstep * qstep;
+    uint64_t* dk = ws.get<uint64_t>("d.key", nslot);
+    uint32_t* dporder = ws.get<uint32_t>("d.order", nslot);
     int4* dres = ws.get<int4>("d.res", ntask);
     const int maxrows = std::max(q.max_qlen - p.lmer, 0);
     const int rowbits = bits_for(static_cast<uint64_t>(maxrows) + 1);
@@ -959,6 +971,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* split = ws.get<uint32_t>("d.split", 3);
     int* band_rows = ws.get<int>("d.bandrows", 1);
     uint32_t* chunk_next = ws.get<uint32_t>("d.chunk", 1);
+    TaskKeyArgs* kargs = ws.get<TaskKeyArgs>("d.kargs", 1);
+    const SortSlots dsl{"dsort.k2", "dsort.v2", "dsort.counts", "dscan.tiles"};
+    const bool use_graph = !graphs_disabled() && !rows_out;
     const uint32_t cu = static_cast<uint32_t>(std::max(rt.compute_units(), 1));
     hipStream_t ss = rt.side_stream();
     // The DP stage under configuration c (tuner.hpp); `final` marks the run
@@ -978,25 +993,67 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       const int sortbits = std::max(d, (rowbits + d - 1) / d * d);
       const int classbit = sortbits - 2;  // four classes (k_dp_task_keys)
       const int rowshift = std::max(0, rowbits - classbit);
-      k_fill<uint64_t><<<1, 64, 0, st>>>(tot64 + 10, 26, 0ull);  // this stage's counters
-      if (bmode == 2) {
-        auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
-        k_dp_rows_sum<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(nhit, hs, hq, q.qlen.ptr, p.lmer, rsum);
-        k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * cu * 128ull, band_rows);
-      } else if (bmode == 1) {
-        k_fill<int><<<1, 64, 0, st>>>(band_rows, 1, kBandRows);
-      }
       const TaskKeyArgs ka{nhit, hs, hgs, ht, hq, q.qlen.ptr, q.qclean.ptr, ix.seq_len.ptr, ix.seq_win.ptr,
                            ix.seq_slot.ptr, ix.win_slot.ptr, ix.seq_clean.ptr, p.lmer, p.gap, packed ? 1 : 0,
                            bmode, band_rows, maxrows, rowshift, classbit, dk, dporder,
                            reinterpret_cast<unsigned long long*>(tot64 + 10),
                            reinterpret_cast<unsigned long long*>(tot64 + 20)};
-      k_dp_task_keys<<<grid_for(ntask, kBlock), kBlock, 0, st>>>(ka);
-      radix_sort_pairs(rt, dk, dporder, ntask, sortbits, d);
-      for (int k = 0; k < 3; ++k)
-        k_count_below<<<1, 64, 0, st>>>(dk, ntask, static_cast<uint64_t>(k + 1) << classbit, split + k, nullptr);
-      k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
-      if (c.sched == 1) k_fill<uint32_t><<<1, 64, 0, st>>>(chunk_next, 1, 0u);
+      k_store<TaskKeyArgs><<<1, 64, 0, st>>>(kargs, ka);
+      auto* rsum = reinterpret_cast<unsigned long long*>(tot64 + 28);
+      // the task-ordering chain: kernels only, every per-search value read
+      // from kargs, sized by nslot
+      auto order_chain = [&] {
+        k_fill<uint64_t><<<1, 64, 0, st>>>(tot64 + 10, 26, 0ull);  // this stage's counters
+        if (bmode == 2) {
+          k_dp_rows_sum<<<grid_for(nslot, kBlock), kBlock, 0, st>>>(kargs, rsum);
+          k_band_threshold<<<1, 64, 0, st>>>(rsum, 4ull * cu * 128ull, band_rows);
+        } else if (bmode == 1) {
+          k_fill<int><<<1, 64, 0, st>>>(band_rows, 1, kBandRows);
+        }
+        k_dp_task_keys<<<grid_for(nslot, kBlock), kBlock, 0, st>>>(kargs, nslot);
+        radix_sort_pairs(rt, dk, dporder, nslot, sortbits, d, dsl);
+        for (int k = 0; k < 3; ++k)
+          k_count_below<<<1, 64, 0, st>>>(dk, nslot, static_cast<uint64_t>(k + 1) << classbit, split + k, nullptr);
+        k_split_counts<<<1, 64, 0, st>>>(split, reinterpret_cast<unsigned long long*>(tot64 + 18));
+        if (c.sched == 1) k_fill<uint32_t><<<1, 64, 0, st>>>(chunk_next, 1, 0u);
+        rt.check_launch("DP task order");
+      };
+      if (!use_graph) {
+        order_chain();
+      } else {
+        // the sort's slots at full size first: no workspace growth (which
+        // synchronises and allocates) may happen while capturing
+        const uint64_t ntiles = grid_for(nslot, kSortTile);
+        uint64_t* k2 = ws.get<uint64_t>(dsl.k2, nslot);
+        uint32_t* v2 = ws.get<uint32_t>(dsl.v2, nslot);
+        uint32_t* cnts = ws.get<uint32_t>(dsl.counts, ntiles * kRadixBins);
+        uint32_t* tiles = ws.get<uint32_t>(dsl.tiles, (ntiles * kRadixBins + kScanTile - 1) / kScanTile + 1);
+        const std::vector<uint64_t> key{
+            static_cast<uint64_t>(bmode), static_cast<uint64_t>(sortbits), static_cast<uint64_t>(d),
+            static_cast<uint64_t>(c.sched), nslot, static_cast<uint64_t>(cu),
+            reinterpret_cast<uint64_t>(kargs), reinterpret_cast<uint64_t>(dk), reinterpret_cast<uint64_t>(dporder),
+            reinterpret_cast<uint64_t>(split), reinterpret_cast<uint64_t>(band_rows),
+            reinterpret_cast<uint64_t>(chunk_next), reinterpret_cast<uint64_t>(tot64), reinterpret_cast<uint64_t>(k2),
+            reinterpret_cast<uint64_t>(v2), reinterpret_cast<uint64_t>(cnts), reinterpret_cast<uint64_t>(tiles)};
+        hipGraphExec_t ex = rt.graph(key);
+        if (!ex) {
+          hipGraph_t g = nullptr;
+          hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+          try {
+            order_chain();
+          } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          hip_check(hipStreamEndCapture(st, &g), "hipStreamEndCapture");
+          const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          hip_check(e, "hipGraphInstantiate");
+          rt.add_graph(key, ex);
+        }
+        hip_check(hipGraphLaunch(ex, st), "hipGraphLaunch");
+      }
       DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
                 ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres,
                 q.bmax, ix.bmax, p.lmer, 4 * cu, chunk_next, ot, oq, thl, qhl, om, osc};

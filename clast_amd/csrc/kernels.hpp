@@ -2268,11 +2268,21 @@ struct TaskKeyArgs {
   unsigned long long* rowsband;  // 8 counters: rows of the long tasks
 };
 
-static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
+// The arguments come from device memory (k_store writes them per search),
+// so a captured launch of this kernel serves every search.  Slots
+// [2 nhit, nslots) get keys above every task of class 3 and below none of
+// the same key's tasks (stable sort: after them): the task count padded to
+// a bucket that a captured chain was built for.
+static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(const TaskKeyArgs* __restrict__ ap, uint32_t nslots) {
   __shared__ uint32_t wsum[2 * (kBlock / 64)];
+  const TaskKeyArgs& a = *ap;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   bool p16 = false, edge = false, long_task = false;
   int rows = 0;
+  if (t >= 2 * a.nhit && t < nslots) {
+    a.key[t] = (3ull << a.classbit) | static_cast<uint64_t>(a.maxrows >> a.rowshift);
+    a.task[t] = t;
+  }
   if (t < 2 * a.nhit) {
     const uint32_t i = t >> 1;
     const bool bwd = (t & 1u) == 0u;
@@ -2318,18 +2328,18 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(TaskKeyArgs a) {
   }
 }
 
-// Rows of every DP task, summed into 8 counters (by workgroup).
-static __global__ __launch_bounds__(kBlock) void k_dp_rows_sum(uint32_t nhit, const int32_t* __restrict__ hs,
-                                                               const int32_t* __restrict__ hq,
-                                                               const int32_t* __restrict__ qlen, int lmer,
+// Rows of every DP task, summed into 8 counters (by workgroup); arguments
+// from device memory as for k_dp_task_keys.
+static __global__ __launch_bounds__(kBlock) void k_dp_rows_sum(const TaskKeyArgs* __restrict__ ap,
                                                                unsigned long long* __restrict__ sums) {
   __shared__ uint32_t wsum[kBlock / 64];
+  const TaskKeyArgs& a = *ap;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   int rows = 0;
-  if (t < 2 * nhit) {
+  if (t < 2 * a.nhit) {
     const uint32_t i = t >> 1;
-    const int q = hq[i];
-    rows = (t & 1u) ? q : qlen[hs[i]] - (q + lmer);
+    const int q = a.hq[i];
+    rows = (t & 1u) ? q : a.qlen[a.hs[i]] - (q + a.lmer);
   }
   const uint32_t r = wave_sum_incl(static_cast<uint32_t>(max(rows, 0)));
   if (lane_id() == 63) wsum[wave_id()] = r;
@@ -2339,6 +2349,12 @@ static __global__ __launch_bounds__(kBlock) void k_dp_rows_sum(uint32_t nhit, co
     for (int w = 0; w < kBlock / 64; ++w) tot += wsum[w];
     if (tot) atomicAdd(&sums[blockIdx.x & 7u], tot);
   }
+}
+
+// One value into device memory (the arguments of a captured kernel).
+template <class T>
+static __global__ void k_store(T* __restrict__ dst, T v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
 }
 
 // Row threshold of the long tasks: a task over twice one SIMD's average

@@ -215,6 +215,15 @@ static __global__ void k_fill(T* __restrict__ out, uint64_t n, T v) {
   if (i < n) out[i] = v;
 }
 
+static __global__ void k_copy_pairs(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    kout[i] = kin[i];
+    vout[i] = vin[i];
+  }
+}
+
 template <class T>
 static __global__ void k_gather(const T* __restrict__ in, const uint32_t* __restrict__ perm, T* __restrict__ out, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,11 +237,12 @@ inline uint32_t grid_for(uint64_t n, uint32_t per_block) {
 // ---------------------------------------------------------------- host side
 
 // Exclusive (or inclusive) scan of n items into out; total (may be null) is
-// a device scalar of type Tacc.  Uses workspace slots "scan.*".
+// a device scalar of type Tacc.  Uses workspace slot `tiles_slot`.
 template <class Op, class Tin, class Tacc, class Tout>
-void device_scan(Runtime& rt, const Tin* in, uint64_t n, Tout* out, Tacc* total, bool inclusive) {
+void device_scan(Runtime& rt, const Tin* in, uint64_t n, Tout* out, Tacc* total, bool inclusive,
+                 const char* tiles_slot = "scan.tiles") {
   const uint64_t ntiles = n == 0 ? 1 : (n + kScanTile - 1) / kScanTile;
-  Tacc* tiles = rt.ws().get<Tacc>("scan.tiles", ntiles);
+  Tacc* tiles = rt.ws().get<Tacc>(tiles_slot, ntiles);
   if (n > 0) k_scan_reduce<Op, Tin, Tacc><<<static_cast<uint32_t>(ntiles), kBlock, 0, rt.stream()>>>(in, n, tiles);
   else k_fill<Tacc><<<1, 64, 0, rt.stream()>>>(tiles, 1, Op::template identity<Tacc>());
   k_scan_tiles<Op, Tacc><<<1, kBlock, 0, rt.stream()>>>(tiles, ntiles, total);
@@ -242,16 +252,27 @@ void device_scan(Runtime& rt, const Tin* in, uint64_t n, Tout* out, Tacc* total,
   rt.check_launch("device_scan");
 }
 
+// Workspace slots of a radix sort: the sorts of one stage can keep their
+// own (the DP stage's task sort, replayed from a captured graph, needs
+// addresses no other stage moves).
+struct SortSlots {
+  const char* k2 = "sort.k2";
+  const char* v2 = "sort.v2";
+  const char* counts = "sort.counts";
+  const char* tiles = "scan.tiles";
+};
+
 // Stable sort of (keys, vals) by key bits [0, nbits), `digit` bits (1..8)
-// per pass.  Uses workspace "sort.*"; results land back in keys/vals.
+// per pass; results land back in keys/vals.  Launches kernels only (no
+// copy or fill nodes when captured).
 inline void radix_sort_pairs(Runtime& rt, uint64_t* keys, uint32_t* vals, uint32_t n, int nbits,
-                             int digit = kRadixBits) {
+                             int digit = kRadixBits, const SortSlots& sl = SortSlots{}) {
   if (n <= 1 || nbits <= 0) return;
   if (digit < 1 || digit > kRadixBits) throw ClastError(1, "radix digit out of range");
   const uint32_t ntiles = grid_for(n, kSortTile);
-  uint64_t* k2 = rt.ws().get<uint64_t>("sort.k2", n);
-  uint32_t* v2 = rt.ws().get<uint32_t>("sort.v2", n);
-  uint32_t* counts = rt.ws().get<uint32_t>("sort.counts", static_cast<uint64_t>(ntiles) * kRadixBins);
+  uint64_t* k2 = rt.ws().get<uint64_t>(sl.k2, n);
+  uint32_t* v2 = rt.ws().get<uint32_t>(sl.v2, n);
+  uint32_t* counts = rt.ws().get<uint32_t>(sl.counts, static_cast<uint64_t>(ntiles) * kRadixBins);
   uint64_t* kin = keys;
   uint32_t* vin = vals;
   uint64_t* kout = k2;
@@ -264,15 +285,15 @@ inline void radix_sort_pairs(Runtime& rt, uint64_t* keys, uint32_t* vals, uint32
     // counts are digit-major: the first 2^db rows hold every used digit
     // (the rest stay zero and are never read for a key)
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, counts, static_cast<uint64_t>(ntiles) << db, counts,
-                                                     static_cast<uint32_t*>(nullptr), false);
+                                                     static_cast<uint32_t*>(nullptr), false, sl.tiles);
     k_radix_scatter<<<ntiles, kBlock, 0, rt.stream()>>>(kin, vin, kout, vout, n, shift, db, mask, counts, ntiles);
     rt.check_launch("radix pass");
     uint64_t* tk = kin; kin = kout; kout = tk;
     uint32_t* tv = vin; vin = vout; vout = tv;
   }
   if (passes & 1) {
-    hip_check(hipMemcpyAsync(keys, kin, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, rt.stream()), "sort copy");
-    hip_check(hipMemcpyAsync(vals, vin, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, rt.stream()), "sort copy");
+    k_copy_pairs<<<grid_for(n, kBlock), kBlock, 0, rt.stream()>>>(kin, vin, keys, vals, n);
+    rt.check_launch("sort copy");
   }
 }
 

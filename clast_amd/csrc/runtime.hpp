@@ -158,6 +158,7 @@ class Runtime {
     hip_check(hipDeviceGetAttribute(&ncu_, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
   }
   ~Runtime() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
     delete ws_;
     (void)hipHostFree(pinned_);
     for (auto& e : out_done_) (void)hipEventDestroy(e);
@@ -191,6 +192,24 @@ class Runtime {
   void set_out_used(int k) { out_used_[k] = true; }
   int device() const { return device_; }
   Workspace& ws() { return *ws_; }
+  // Executable graphs of captured launch chains, by a key of everything the
+  // launches were captured with (sizes, flags and every device address
+  // passed by value); at most kGraphs kept, the oldest dropped first.
+  hipGraphExec_t graph(const std::vector<uint64_t>& key) const {
+    auto it = graphs_.find(key);
+    return it == graphs_.end() ? nullptr : it->second;
+  }
+  void add_graph(const std::vector<uint64_t>& key, hipGraphExec_t g) {
+    if (graphs_.size() >= kGraphs) {
+      hip_check(hipStreamSynchronize(stream_), "graph cache sync");
+      (void)hipGraphExecDestroy(graphs_.at(graph_order_.front()));
+      graphs_.erase(graph_order_.front());
+      graph_order_.erase(graph_order_.begin());
+    }
+    graphs_[key] = g;
+    graph_order_.push_back(key);
+  }
+  size_t graph_count() const { return graphs_.size(); }
   std::mutex& mutex() { return mu_; }
 
   void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
@@ -208,6 +227,9 @@ class Runtime {
 
  private:
   static constexpr size_t kPinned = 4096;
+  static constexpr size_t kGraphs = 64;
+  std::map<std::vector<uint64_t>, hipGraphExec_t> graphs_;
+  std::vector<std::vector<uint64_t>> graph_order_;
   int device_;
   hipStream_t stream_ = nullptr;
   hipStream_t copy_stream_ = nullptr;

@@ -626,3 +626,40 @@ def test_dp_tuner_keeps_the_rows(monkeypatch):
     assert_rows_equal(h2.arrays(), exp, what="tuned second search")
     for h in (h1, h2, ix, q):
         h.close()
+
+
+def test_dp_order_graph_replays_equal_eager(monkeypatch):
+    # The DP stage's task-ordering chain is captured into a hipGraph per
+    # shape and replayed.  One query set searched four times (the first
+    # search captures, the others replay), between them searches of a
+    # second query set, with the graphs off (kernel by kernel) and on, with
+    # synchronisations in between: every replay's rows equal the
+    # kernel-by-kernel rows and the oracle's.
+    for k, v in (("CLAST_TUNE", "0"), ("CLAST_DP_SCHED", "persist"), ("CLAST_DP_BAND", "rest")):
+        monkeypatch.setenv(k, v)
+    t, r = _dp_mix_workload(81)
+    p = native.Params(total_db_size=db_size(t))
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 90_000, query=False)
+    qwins = native.windows(ql.gateway, 80_000, query=True)
+    ix = native.Index(p, tl, twins)
+    qa = native.Query(p, ql, qwins)
+    qb = native.Query(p, ql, qwins[:1])
+    exp_a = oracle.search(p, tl, twins, ql, qwins)["final"]
+    exp_b = oracle.search(p, tl, twins, ql, qwins[:1])["final"]
+    monkeypatch.setenv("CLAST_GRAPH", "0")
+    eager_a = native.search_batch(p, ix, qa).arrays()
+    eager_b = native.search_batch(p, ix, qb).arrays()
+    assert_rows_equal(eager_a, exp_a, what="eager a")
+    assert_rows_equal(eager_b, exp_b, what="eager b")
+    monkeypatch.delenv("CLAST_GRAPH")
+    print("dp order graph replays", flush=True)
+    for k in range(4):
+        got_a = native.search_batch(p, ix, qa).arrays()
+        assert_rows_equal(got_a, eager_a, what=f"graph replay {k} of a")
+        monkeypatch.setenv("CLAST_GRAPH", "0" if k % 2 else "1")
+        got_b = native.search_batch(p, ix, qb).arrays()
+        assert_rows_equal(got_b, eager_b, what=f"search {k} of b")
+        monkeypatch.delenv("CLAST_GRAPH")
+    for h in (qa, qb, ix):
+        h.close()
