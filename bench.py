@@ -35,7 +35,10 @@ Also reported (untimed checks and extra legs):
   request's rows equal at every rate;
 * large: the 1 Gbp target with 1,000,000 reads of 150 bp in 5 query RAM
   chunks, each step uploading and encoding every chunk's reads on the device
-  and copying the rows back, from 3 host threads.
+  and copying the rows back, from 3 host threads;
+* uneven_batched_vs_plain: long reads (100 bp-20 kbp) searched the
+  reference's way, one launch chain per (target window, query window) unit,
+  and batched, over the same timed span, rows equal.
 """
 from __future__ import annotations
 
@@ -353,6 +356,81 @@ def large_leg(steps: int, warmup: int, streams: int) -> dict:
     }
 
 
+# ---------------------------------------------------------------- uneven: batched vs plain
+
+def uneven_leg(n_chunks: int, streams: int) -> dict:
+    """The long-read configuration (1 Gbp target, 200,000 reads of 100 bp -
+    20 kbp log-uniform, 20% random), the first n_chunks query RAM chunks,
+    timed from reads in host memory to rows in host memory two ways over
+    the same resident target:
+    * plain: the reference's schedule (CHostSchedular.cu:55-127), one launch
+      chain per (target window, query window) unit: each unit's query
+      window uploaded (clast_query_create) and searched alone against that
+      window's index (clast_search), rows copied back, one unit at a time;
+    * batched: per query chunk, the reads uploaded and encoded on the device
+      once (clast_query_create_reads) and every unit searched in one launch
+      chain against the merged index (clast_search_batch: persistent /
+      autotuned DP, captured task ordering), chunks from `streams` threads.
+    The plain rows, appended in (target window, query window) order and
+    stable-sorted by (query pair, score descending) as fixResult does, must
+    equal the batched rows."""
+    from clast_amd import configs, native
+
+    t, r, p = configs.workload("uneven")
+    tch = configs.target_chunks(t, p)
+    rch = configs.read_chunks(r, p)[:n_chunks]
+    merged = native.Index.merged(p, tch)
+    units = [(tl, (a, b), native.Index(p, tl, [(a, b)])) for tl, wins in tch for a, b in wins]
+    mbp = sum(c.n_bases for c in rch) / 1e6
+    # warm both paths (kernel loading, autotuner trials, graph capture)
+    native.search_batch(p, merged, native.Query.from_reads(p, rch[0].bases, rch[0].offsets, rch[0].wins)).close()
+
+    def plain_chunk(ch):
+        parts = []
+        for tl, (a, b), ixu in units:
+            for qa, qb in ch.wins:
+                q = native.Query(p, ch.lst, [(qa, qb)])
+                h = native.search(p, ixu, q, a, 2 * qa)
+                parts.append(h.arrays())
+                h.close()
+                q.close()
+        cat = {k: np.concatenate([x[k] for x in parts]) for k in native.HIT_FIELDS + ("evalue",)}
+        order = np.lexsort((-cat["score"].astype(np.int64), cat["qID"] // 2))
+        return {k: v[order] for k, v in cat.items()}
+
+    for ch in rch:
+        _ = ch.lst  # host lists built before timing (the plain path uploads from them)
+    ts = time.perf_counter()
+    plain = [plain_chunk(ch) for ch in rch]
+    plain_s = time.perf_counter() - ts
+
+    def batched_chunk(ch):
+        q = native.Query.from_reads(p, ch.bases, ch.offsets, ch.wins)
+        h = native.search_batch(p, merged, q)
+        rows = h.arrays()
+        h.close()
+        q.close()
+        return rows
+
+    ts = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=streams) as pool:
+        batched = list(pool.map(batched_chunk, rch))
+    batched_s = time.perf_counter() - ts
+    equal = all(len(a["tID"]) == len(b["tID"]) and all(np.array_equal(a[k], b[k]) for k in native.HIT_FIELDS)
+                for a, b in zip(plain, batched))
+    merged.close()
+    for _, _, ixu in units:
+        ixu.close()
+    nunits = len(units) * sum(len(c.wins) for c in rch)
+    return {"workload": f"uneven: 1 Gbp target x the first {len(rch)} query RAM chunks of 200,000 reads "
+                        f"(100 bp-20 kbp log-uniform, 20% random): {sum(len(c.offsets) - 1 for c in rch)} reads, "
+                        f"{mbp:.1f} Mbp, {nunits} (target window, query window) units",
+            "plain_mbp_s": round(mbp / plain_s, 3), "batched_mbp_s": round(mbp / batched_s, 3),
+            "batched_over_plain": round(plain_s / batched_s, 2), "rows": int(sum(len(b["tID"]) for b in batched)),
+            "rows_equal": bool(equal),
+            "timed_span": "reads in host memory -> rows in host memory, target index resident in both"}
+
+
 # ---------------------------------------------------------------- main
 
 def main() -> int:
@@ -369,6 +447,8 @@ def main() -> int:
     ap.add_argument("--sweep-rates", default="10,25,50,100,200")
     ap.add_argument("--sweep-requests", type=int, default=80)
     ap.add_argument("--large-steps", type=int, default=3)
+    ap.add_argument("--uneven-chunks", type=int, default=2)
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-checks", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -485,6 +565,9 @@ def main() -> int:
     if not args.no_large:
         out["large"] = large_leg(args.large_steps, 1, 3)
         log(f"large: {out['large']['value']} Mbp/s ({out['large']['resident_value']} resident)")
+    if not args.no_uneven:
+        out["uneven_batched_vs_plain"] = uneven_leg(args.uneven_chunks, 3)
+        log(f"uneven: {out['uneven_batched_vs_plain']}")
     print(json.dumps(out), flush=True)
     return 0
 
