@@ -118,13 +118,46 @@ def request_checks(p, reqs, tch) -> dict:
         res.append(checks.check_rows(p, rows, qlens[rows["qID"]], tl))
     m = checks.merge(res)
     m["passed"] = checks.passed(m)
-    return m
+    return m, res
 
 
-def rows_equal(a: dict, b: dict) -> bool:
+def outside_row_vs_oracle(p, tch, reqs, res) -> dict:
+    """The smallest request holding a row that runs past an end of its
+    target sequence (a row the reference's rules allow, counted by the
+    checks): that request's rows in that target window against the oracle."""
+    from clast_amd import configs, native
+
+    cand = [(len(r.offsets), k) for k, (r, c) in enumerate(zip(reqs, res)) if c["rows_outside_target"]]
+    if not cand:
+        return {"checked": False, "reason": "no such row"}
+    k = min(cand)[1]
+    req, i = reqs[k], res[k]["first_outside_target"]
+    twin = int(req.rows["twin"][i])
+    off = 0
+    for lst, wins in tch:
+        if twin < off + len(wins):
+            tl, tw = lst, wins[twin - off]
+            break
+        off += len(wins)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    ql = native.build_list(req.bases, req.offsets, p.lmer, True)
+    qwins = native.windows(ql.gateway, configs.Q_VRAM, True)
+    exp = oracle.search(p, tl, [tw], ql, qwins)["final"]
+    mine = {key: v[req.rows["twin"] == twin] for key, v in req.rows.items()}
+    row = {key: int(req.rows[key][i]) for key in ("tID", "tIdx", "tHL", "qID", "qIdx", "qHL", "score")}
+    return {"checked": True, "request_reads": len(req.offsets) - 1, "target_window": twin, "row": row,
+            "target_length": int(tl.length[row["tID"]]), "rows": int(len(exp["tID"])),
+            "bit_exact": rows_equal(mine, exp, windows=False)}
+
+
+def rows_equal(a: dict, b: dict, windows: bool = True) -> bool:
+    """Rows equal (integers exactly, E-values within 1e-9 relative), with
+    the target window column unless windows is False."""
     from clast_amd import native
 
-    keys = native.HIT_FIELDS + ("twin",)
+    keys = native.HIT_FIELDS + (("twin",) if windows else ())
     if len(a["tID"]) != len(b["tID"]):
         return False
     if not all(np.array_equal(a[k], b[k]) for k in keys):
@@ -235,7 +268,7 @@ def cpu_baseline_serving(p, tch, reqs, sample_mbp: float, threads: int):
             {"rows": int(len(exp["tID"])), "bit_exact": bool(exact), "evalue_max_rel": rel})
 
 
-def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int) -> dict:
+def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int, part_positions: int) -> dict:
     """The seed-pressure request stream at each arrival rate: the same
     requests, arrival gaps scaled; each request's rows must be equal at
     every rate (bit-exact under any admission pattern)."""
@@ -250,7 +283,7 @@ def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int) -> dict:
     exact = True
     for rate in rates:
         reqs = [serve.Request(r.rid, r.bases, r.offsets, float(a / rate)) for r, a in zip(base, arrivals)]
-        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=workers)
+        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=workers, part_positions=part_positions)
         st = srv.run(reqs, realtime=True)
         if ref is None:
             ref = [r.rows for r in reqs]
@@ -258,7 +291,8 @@ def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int) -> dict:
             exact &= all(rows_equal(r.rows, e) for r, e in zip(reqs, ref))
         points.append({"rate_req_s": rate, "offered_mbp_s": round(mbp * rate / max(arrivals[-1], 1e-9), 2),
                        "served_mbp_s": st["mbp_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-                       "batches": st["batches"], "mean_batch": st["mean_batch"], "wall_s": st["wall_s"]})
+                       "batches": st["batches"], "mean_batch_parts": st["mean_batch_parts"],
+                       "wall_s": st["wall_s"]})
         log(f"pressure {rate} req/s: {points[-1]}")
     return {"requests": n_requests, "query_mbp": round(mbp, 2),
             "rows": int(sum(len(r["tID"]) for r in ref)), "rows_equal_at_every_rate": bool(exact),
@@ -440,6 +474,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3, help="warmup seconds of arrivals")
     ap.add_argument("--rate", type=float, default=50.0, help="requests per second")
     ap.add_argument("--workers", type=int, default=2, help="device threads draining the batch queue")
+    ap.add_argument("--part-positions", type=int, default=16_000_000,
+                    help="requests are cut into parts of at most this many query k-mer positions")
     ap.add_argument("--capacity-requests", type=int, default=300)
     ap.add_argument("--alone-sample", type=int, default=12)
     ap.add_argument("--cpu-sample-mbp", type=float, default=12.0)
@@ -494,10 +530,10 @@ def main() -> int:
     log(f"{n_warm} warmup + {n_req} timed requests ({mbp:.0f} Mbp timed) generated in "
         f"{time.perf_counter() - t2:.1f} s")
     if n_warm:
-        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers)
+        srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers, part_positions=args.part_positions)
         ws = srv.run(warm, realtime=True)
         log(f"warmup: {ws['requests']} requests, {ws['mbp_per_s']} Mbp/s, p99 {ws['p99_ms']} ms")
-    srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers)
+    srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers, part_positions=args.part_positions)
     stats = srv.run(reqs, realtime=True, record=True)
     log(f"timed: {stats}")
     wall = stats["wall_s"]
@@ -537,7 +573,8 @@ def main() -> int:
     }
     if not args.no_checks:
         t3 = time.perf_counter()
-        out["full_size_checks"] = request_checks(p, reqs, tch)
+        out["full_size_checks"], per_req = request_checks(p, reqs, tch)
+        out["outside_target_row_vs_oracle"] = outside_row_vs_oracle(p, tch, reqs, per_req)
         order = sorted(range(len(reqs)), key=lambda k: len(reqs[k].offsets))
         pick = [order[int(i)] for i in np.linspace(0, len(order) - 1, min(args.alone_sample, len(order)))]
         same = 0
@@ -548,7 +585,8 @@ def main() -> int:
         log(f"checks in {time.perf_counter() - t3:.1f} s: {out['full_size_checks']}, alone {same}/{len(pick)}")
     if not args.no_capacity:
         cap = [serve.Request(r.rid, r.bases, r.offsets, 0.0) for r in reqs[:args.capacity_requests]]
-        cs = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers).run(cap, realtime=False)
+        cs = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers,
+                          part_positions=args.part_positions).run(cap, realtime=False)
         out["capacity"] = {"requests": cs["requests"], "query_mbp": cs["query_mbp"], "mbp_per_s": cs["mbp_per_s"],
                            "wall_s": cs["wall_s"], "batches": cs["batches"],
                            "note": "the first requests of the timed stream, all queued at once"}
@@ -560,7 +598,8 @@ def main() -> int:
         log(f"cpu baseline: {cb}; parity {parity}")
     if not args.no_sweep:
         rates = [float(x) for x in args.sweep_rates.split(",") if x]
-        out["pressure_sweep"] = pressure_sweep(p, ix, t, rates, args.sweep_requests, args.workers)
+        out["pressure_sweep"] = pressure_sweep(p, ix, t, rates, args.sweep_requests, args.workers,
+                                              args.part_positions)
     ix.close()
     if not args.no_large:
         out["large"] = large_leg(args.large_steps, 1, 3)

@@ -12,16 +12,22 @@ rows are split back per request.  Because the only coupling between queries
 is per (target window, query window) unit and every request keeps its own
 windows, each request's rows are identical alone and in the mix.
 
+A request is cut at query-window boundaries into parts of a bounded number
+of query k-mer positions, so that the device threads share a large request
+instead of one of them running it alone; its rows are its parts' rows in
+window order, equal to the request's rows alone.
+
 Admission is by device memory.  At start the server reads the free device
 memory (hipMemGetInfo, with the index already resident) and keeps a budget of
 90% of it less a reserve; each of the `workers` device threads may hold one
 batch, so a batch is closed before its estimated device bytes pass the
-budget's share of one worker, or the free memory left at that moment.  The
+budget's share of one worker, or the free memory left at that moment
+(parts are admitted in order, whole).  The
 estimate is bytes per query k-mer position (the query set's own tables plus
 the search workspace), learned from the workers: after each search a worker
 reads its workspace size (it grows to the largest batch it has run and is
 kept), and the estimate is the largest ratio of workspace to positions seen.
-A request larger than a whole share is admitted alone.
+A part larger than a whole share is admitted alone.
 
 Host preparation of the next batch (list building, windows) overlaps the
 device search of the current one: a preparation thread feeds a queue and
@@ -31,6 +37,7 @@ during native calls.
 """
 from __future__ import annotations
 
+import collections
 import dataclasses
 import queue
 import threading
@@ -51,6 +58,8 @@ class Request:
     # filled by the server
     done: float = 0.0
     rows: dict = field(default_factory=dict)
+    _parts: dict = field(default_factory=dict, repr=False)
+    _nparts: int = 0
 
     @property
     def n_bases(self) -> int:
@@ -58,15 +67,26 @@ class Request:
 
 
 @dataclass
-class Prepared:
-    requests: list
-    wins: list               # query windows (read pair ranges) per request
+class Part:
+    """Consecutive query windows of one request (read pair ranges in the
+    request's own numbering), searched within one batch."""
+    req: Request
+    k: int
+    wins: list
     positions: int
+    slots: int
 
 
 class Server:
     """Continuous-batching search over a resident index, admitted by a device
-    memory budget."""
+    memory budget.
+
+    A request is cut at query-window boundaries into parts of at most
+    part_positions query k-mer positions (a window is never cut), so that the
+    device threads can share a large request; a request's rows are its parts'
+    rows in window order, which is the order the request alone gets (each
+    part holds whole read pairs and every target window, and the rows of a
+    batch come pair by pair)."""
 
     # starting estimate of device bytes per query k-mer position, before any
     # search has run: 16 B cell record, 4 B strand, and room for seeds and hits
@@ -77,12 +97,13 @@ class Server:
 
     def __init__(self, params: native.Params, index: native.Index, q_vram: float = 2e6,
                  workers: int = 2, mem_fraction: float = 0.9, reserve_bytes: int = 2 << 30,
-                 max_positions: int | None = None):
+                 max_positions: int | None = None, part_positions: int | None = None):
         self.p = params
         self.index = index
         self.q_vram = q_vram
         self.workers = max(1, workers)
         self.max_positions = max_positions
+        self.part_positions = part_positions
         free, total = native.mem_info()
         self.reserve = reserve_bytes
         self.mem_budget = max(0, int((free - reserve_bytes) * mem_fraction))
@@ -90,7 +111,7 @@ class Server:
         self.bytes_per_position = self.START_BYTES_PER_POSITION
         self._ws = {}        # worker -> (largest positions run, workspace bytes)
         self._lock = threading.Lock()
-        self.records = []    # per batch: (requests, positions, kernel times, counts)
+        self.records = []    # per batch: (parts, positions, kernel times, counts)
 
     def estimate(self, positions: int, slots: int) -> float:
         """Device bytes of a batch: the workspace estimate with a quarter of
@@ -120,53 +141,91 @@ class Server:
         ln = np.diff(r.offsets)
         return int(2 * np.clip(ln - self.p.lmer + 1, 0, None).sum())
 
-    def prepare(self, reqs: list) -> Prepared:
-        wins, npos = [], 0
-        for r in reqs:
-            g = native.read_gateway(np.diff(r.offsets), self.p.lmer)
-            wins.append(native.windows(g, self.q_vram, True))
-            npos += self.positions(r)
-        return Prepared(reqs, wins, npos)
-
-    def slots(self, r: Request) -> int:
+    def split(self, r: Request, whole: bool = False) -> list:
+        """The request's parts (one part when whole or when no part size is set)."""
         ln = np.diff(r.offsets)
-        return int(2 * (ln + max(self.p.lmer - 1, 0)).sum())
+        g = native.read_gateway(ln, self.p.lmer)
+        wins = native.windows(g, self.q_vram, True)
+        pos = np.concatenate([[0], np.cumsum(2 * np.clip(ln - self.p.lmer + 1, 0, None))])
+        slot = np.concatenate([[0], np.cumsum(2 * (ln + max(self.p.lmer - 1, 0)))])
+        parts, cur = [], []
+        cap = None if whole else self.part_positions
 
-    def search(self, prep: Prepared, worker: int = 0, record: bool = False) -> None:
-        """One launch chain for the whole batch; rows split per request."""
-        owner = [k for k, wl in enumerate(prep.wins) for _ in wl]
+        def close():
+            a, b = cur[0][0], cur[-1][1]
+            parts.append(Part(r, len(parts), list(cur), int(pos[b] - pos[a]), int(slot[b] - slot[a])))
+            cur.clear()
+
+        for a, b in wins:
+            if cur and cap is not None and pos[b] - pos[cur[0][0]] > cap:
+                close()
+            cur.append((a, b))
+        if cur:
+            close()
+        r._nparts = len(parts)
+        r._parts = {}
+        if not parts:  # a request with no reads
+            r.rows = {}
+        return parts
+
+    def search(self, parts: list, worker: int = 0, record: bool = False) -> list:
+        """One launch chain for the batch; returns each part's rows."""
+        owner = [k for k, pt in enumerate(parts) for _ in pt.wins]
         if not owner:
-            for r in prep.requests:
-                r.rows = {}
-            return
-        # one upload of the batch's reads; each request's windows shifted to
-        # its reads' place in the batch, list IDs numbered per request
-        bases = np.concatenate([r.bases[r.offsets[0]:r.offsets[-1]] for r in prep.requests])
-        offs = [np.zeros(1, np.int64)]
-        bwins, ids, first = [], [], 0
-        for r, wl in zip(prep.requests, prep.wins):
-            offs.append((r.offsets[1:] - r.offsets[0]).astype(np.int64) + offs[-1][-1])
-            bwins += [(first + a, first + b) for a, b in wl]
-            ids += [2 * a for a, _ in wl]
-            first += len(r.offsets) - 1
-        q = native.Query.from_reads(self.p, bases, np.concatenate(offs), bwins, ids)
+            return [{} for _ in parts]
+        # one upload of the batch's reads; each part's windows shifted to its
+        # reads' place in the batch, list IDs numbered per request
+        chunks, offs, bwins, ids, first = [], [np.zeros(1, np.int64)], [], [], 0
+        for pt in parts:
+            r = pt.req
+            a, b = pt.wins[0][0], pt.wins[-1][1]
+            lo, hi = int(r.offsets[a]), int(r.offsets[b])
+            chunks.append(r.bases[lo:hi])
+            offs.append((r.offsets[a + 1:b + 1] - lo).astype(np.int64) + offs[-1][-1])
+            bwins += [(first + wa - a, first + wb - a) for wa, wb in pt.wins]
+            ids += [2 * wa for wa, _ in pt.wins]
+            first += b - a
+        q = native.Query.from_reads(self.p, np.concatenate(chunks), np.concatenate(offs), bwins, ids)
+        npos = sum(pt.positions for pt in parts)
         try:
             hits = native.search_batch(self.p, self.index, q)
             rows = hits.arrays()
             if record:
                 with self._lock:
-                    self.records.append((len(prep.requests), prep.positions, hits.kernel_times(), hits.counts()))
+                    self.records.append((len(parts), npos, hits.kernel_times(), hits.counts()))
             hits.close()
         finally:
             q.close()
-        self._learn(worker, prep.positions)
-        req_of_row = np.asarray(owner, dtype=np.int64)[rows["qwin"]] if len(rows["qwin"]) else np.zeros(0, np.int64)
-        for k, r in enumerate(prep.requests):
-            sel = np.nonzero(req_of_row == k)[0]
-            r.rows = {key: v[sel] for key, v in rows.items() if key not in ("qwin",)}
+        self._learn(worker, npos)
+        part_of_row = np.asarray(owner, dtype=np.int64)[rows["qwin"]] if len(rows["qwin"]) else np.zeros(0, np.int64)
+        out = []
+        for k in range(len(parts)):
+            sel = np.nonzero(part_of_row == k)[0]
+            out.append({key: v[sel] for key, v in rows.items() if key != "qwin"})
+        return out
+
+    def _deliver(self, parts: list, rows: list, t: float) -> int:
+        """Stores each part's rows; a request whose parts are all in gets its
+        rows (parts in order) and its completion time.  Returns the number of
+        requests completed."""
+        done = 0
+        with self._lock:
+            for pt, rw in zip(parts, rows):
+                r = pt.req
+                r._parts[pt.k] = rw
+                if len(r._parts) == r._nparts:
+                    ps = [r._parts[k] for k in range(r._nparts)]
+                    r.rows = {key: np.concatenate([x[key] for x in ps]) for key in ps[0]} if ps[0] else {}
+                    r._parts = {}
+                    r.done = t
+                    done += 1
+        return done
 
     def search_alone(self, r: Request) -> dict:
-        self.search(self.prepare([r]))
+        """The request as one batch of its own, unsplit."""
+        parts = self.split(r, whole=True)
+        if parts:
+            self._deliver(parts, self.search(parts), 0.0)
         return r.rows
 
     def run(self, requests: list, realtime: bool = True, record: bool = False) -> dict:
@@ -178,37 +237,44 @@ class Server:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         ready: queue.Queue = queue.Queue(maxsize=self.workers + 1)
-        pending = list(requests)
-        lock = threading.Lock()
+        admitted, caps, nparts = [0], [], [0]
 
         def producer():
             i = 0
-            while i < len(pending):
-                # wait for the next arrival
-                if realtime:
-                    wait = pending[i].arrival - now()
+            pend: collections.deque = collections.deque()
+            while i < len(requests) or pend:
+                if realtime and not pend:  # wait for the next arrival
+                    wait = requests[i].arrival - now()
                     if wait > 0:
                         time.sleep(wait)
-                # admit what has arrived while its estimated device bytes fit
+                while i < len(requests) and (not realtime or requests[i].arrival <= now()):
+                    ps = self.split(requests[i])
+                    if not ps:
+                        requests[i].done = now()
+                    pend.extend(ps)
+                    nparts[0] += len(ps)
+                    i += 1
+                if not pend:
+                    continue
+                # admit pending parts in order while the batch's estimated
+                # device bytes fit
                 cap = self._cap()
                 batch, npos, nslot = [], 0, 0
-                while i < len(pending) and (not realtime or pending[i].arrival <= now()):
-                    need, sl = self.positions(pending[i]), self.slots(pending[i])
-                    over = self.estimate(npos + need, nslot + sl) > cap or (
-                        self.max_positions is not None and npos + need > self.max_positions)
+                while pend:
+                    pt = pend[0]
+                    over = self.estimate(npos + pt.positions, nslot + pt.slots) > cap or (
+                        self.max_positions is not None and npos + pt.positions > self.max_positions)
                     if batch and over:
                         break
-                    batch.append(pending[i])
-                    npos += need
-                    nslot += sl
-                    i += 1
+                    batch.append(pend.popleft())
+                    npos += pt.positions
+                    nslot += pt.slots
                 with self._lock:
                     admitted[0] = max(admitted[0], npos)
                     caps.append(cap)
-                ready.put(self.prepare(batch))
+                ready.put(batch)
             ready.put(None)
 
-        admitted, caps = [0], []
         th = threading.Thread(target=producer, daemon=True)
         th.start()
         batches = [0]
@@ -216,18 +282,17 @@ class Server:
 
         def worker(k):
             while True:
-                prep = ready.get()
-                if prep is None:
+                batch = ready.get()
+                if batch is None:
                     ready.put(None)  # for the other workers
                     return
                 try:
-                    self.search(prep, worker=k, record=record)
+                    rows = self.search(batch, worker=k, record=record)
                 except Exception as e:  # noqa: BLE001 - reported after the run
                     errors.append(e)
-                t = now()
-                with lock:
-                    for r in prep.requests:
-                        r.done = t
+                    rows = [{} for _ in batch]
+                self._deliver(batch, rows, now())
+                with self._lock:
                     batches[0] += 1
 
         ws = [threading.Thread(target=worker, args=(k,), daemon=True) for k in range(self.workers)]
@@ -241,14 +306,15 @@ class Server:
         wall = now()
         lat = np.array([(r.done - r.arrival) * 1e3 for r in requests])
         mbp = sum(r.n_bases for r in requests) / 1e6
-        return {"requests": len(requests), "batches": batches[0], "workers": self.workers, "wall_s": round(wall, 3),
-                "query_mbp": round(mbp, 3), "mbp_per_s": round(mbp / wall, 3),
+        return {"requests": len(requests), "parts": nparts[0], "batches": batches[0], "workers": self.workers,
+                "wall_s": round(wall, 3), "query_mbp": round(mbp, 3), "mbp_per_s": round(mbp / wall, 3),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "mean_batch": round(len(requests) / max(batches[0], 1), 2),
+                "mean_batch_parts": round(nparts[0] / max(batches[0], 1), 2),
                 "admission": {"mem_budget_bytes": self.mem_budget, "per_batch_share_bytes": self.share,
                               "min_batch_cap_bytes": int(min(caps)) if caps else 0,
                               "bytes_per_position": round(self.bytes_per_position, 1),
                               "largest_batch_positions": admitted[0],
+                              "part_positions": self.part_positions,
                               "workspace_bytes": {k: w for k, (_, w) in sorted(self._ws.items())}}}
 
 

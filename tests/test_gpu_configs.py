@@ -177,7 +177,9 @@ def _serve_case(big_target, name, n_reads, alone_every):
     t, p, tch, ix = big_target
     _, rspec = gen.workload(name)
     r = gen.make_reads(dataclasses.replace(rspec, n_reads=n_reads), t)
-    srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=2)
+    # parts of at most 4 M positions: the large requests are split over
+    # several batches, the alone runs below are not
+    srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=2, part_positions=4_000_000)
     reqs = serve.poisson_requests(r.bases, r.offsets, rate=50.0, seed=11, min_reads=1, max_reads=10_000)
     _say(f"{name}: {len(reqs)} requests, {len(r)} reads ({r.bases.shape[0] / 1e6:.1f} Mbp)")
     stats = srv.run(reqs, realtime=False)
@@ -196,15 +198,15 @@ def _serve_case(big_target, name, n_reads, alone_every):
         assert_rows_equal(req.rows, alone, what=f"{name} request {req.rid} mix vs alone")
         np.testing.assert_array_equal(req.rows["twin"], alone["twin"])
     # one small request's unit (the target window with most of its rows),
-    # and the window of the first row that ends past its target sequence (a
+    # and the window of the first row that runs past an end of its target sequence (a
     # row the reference allows; counted above), each against the oracle
     small = min(reqs, key=lambda q: (len(q.offsets) > 200, -len(q.rows["tID"])))
     twin = int(np.bincount(small.rows["twin"]).argmax()) if len(small.rows["twin"]) else 0
     _request_window_vs_oracle(p, tch, small, twin, name)
-    past = [(req, c["first_past_target_end"]) for req, c in zip(reqs, res) if c["rows_past_target_end"]]
+    past = [(req, c["first_outside_target"]) for req, c in zip(reqs, res) if c["rows_outside_target"]]
     if past:
         req, i = min(past, key=lambda x: len(x[0].offsets))
-        _say(f"{name}: request {req.rid} row {i} ends past its target sequence: "
+        _say(f"{name}: request {req.rid} row {i} runs past an end of its target sequence: "
              f"{ {k: int(req.rows[k][i]) for k in ('twin', 'tID', 'tIdx', 'tHL', 'qID', 'qIdx', 'qHL', 'score')} }")
         _request_window_vs_oracle(p, tch, req, int(req.rows["twin"][i]), name)
     return m
