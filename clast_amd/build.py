@@ -93,6 +93,23 @@ def build_host() -> str:
     return exe
 
 
+SANITIZE = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+
+
+def build_host_selftest(sanitize: bool = True) -> str:
+    """bin/host_selftest_san: the host driver's CPU side (host/selftest.cpp)
+    built with the address and undefined-behaviour sanitizers (g++, host
+    code only; the HIP library it links is not instrumented)."""
+    os.makedirs(BIN, exist_ok=True)
+    srcs = [os.path.join(HOST, u) for u in HOST_UNITS] + [os.path.join(HOST, "selftest.cpp")]
+    exe = os.path.join(BIN, "host_selftest_san" if sanitize else "host_selftest")
+    flags = SANITIZE if sanitize else ["-O2"]
+    _run(["g++", *flags, "-std=c++17", f"-I{INCLUDE}", *srcs, f"-L{LIB}", "-lclast_hip",
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_all() -> None:
     with ThreadPoolExecutor(max_workers=2) as ex:
         f_hip = ex.submit(build_hip)

@@ -856,7 +856,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       ng = static_cast<uint32_t>(t2);
       skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
       sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
-      k_seed_write<<<novf, kWriteBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2, skey, sval);
+      // at most 2^16 workgroups, striding over the listed strands
+      k_seed_write<<<std::min<uint32_t>(novf, 1u << 16), kWriteBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2,
+                                                                               skey, sval);
       rt.check_launch("k_seed_write");
     }
     if (!keep_stages) {

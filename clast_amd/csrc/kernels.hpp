@@ -1143,17 +1143,19 @@ static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, S
                                                                    uint32_t* __restrict__ sval) {
   constexpr int kW = kWriteBlock / 64;
   __shared__ uint32_t wsum[2][kW];
-  const uint32_t k = blockIdx.x;
-  if (k >= nlist) return;  // the whole workgroup
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  // workgroups stride over the listed strands (the grid is capped); the
+  // partial-sum buffers alternate across strands as across chunks, so one
+  // barrier per chunk keeps a buffer from being rewritten while read
+  int buf = 0;
+  for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
   const uint32_t s = list[k];
   const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
   const int qLen = a.qlen[s];
   const int qw = a.qwin[s];
   const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
   uint32_t base = off[s];
-  int buf = 0;
   for (uint32_t c0 = p0; c0 < p1; c0 += kWriteBlock) {
     const uint32_t p = c0 + threadIdx.x;
     const uint4 cp = p < p1 ? cellp[p] : make_uint4(0, 0, 0, 0);
@@ -1183,6 +1185,7 @@ static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, S
     }
     base += total;
     buf ^= 1;
+  }
   }
 }
 

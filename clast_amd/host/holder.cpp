@@ -39,6 +39,15 @@ void ResultHolder::add(clast_hits h, const SeqList& t) {
   }
 }
 
+void ResultHolder::add_rows(const int32_t* const cols[8], const double* evalue, int64_t n, const SeqList& t) {
+  for (int64_t i = 0; i < n; ++i) {
+    for (int c = 0; c < 8; ++c) col_[c].push_back(cols[c][i]);
+    ev_.push_back(evalue[i]);
+    tlabel_.push_back(t.label.at(cols[0][i]));
+    tstart_.push_back(t.start_idx.at(cols[0][i]));
+  }
+}
+
 // fixResult (:75-128): the device stable sort by (qID/2, score desc); as in
 // the reference, the tID column carries the row index so that the target
 // labels and offsets follow their rows.

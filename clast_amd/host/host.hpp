@@ -82,6 +82,8 @@ class ResultHolder {
  public:
   explicit ResultHolder(const std::vector<Fasta>& queries);
   void add(clast_hits h, const SeqList& targets);  // addResult + addLabel + addStartIdx
+  // the same from host arrays (n rows; cols: tID tIdx qID qIdx tHL qHL match score)
+  void add_rows(const int32_t* const cols[8], const double* evalue, int64_t n, const SeqList& targets);
   void fix();                                      // fixResult (device stable sort)
   int64_t print(int num_out, const std::string& path) const;  // printResult; returns rows printed
   int64_t size() const { return static_cast<int64_t>(col_[0].size()); }

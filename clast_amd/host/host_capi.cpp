@@ -79,4 +79,20 @@ int32_t clast_host_chunks(const int64_t* lengths, int32_t n, int64_t avail, int3
   return chunks + 1;
 }
 
+// The FASTA loader over one file with a RAM budget of `avail` bases, load
+// after load (CFASTALoader::loadFASTA, main.cu:40-67): the number of
+// records of each load into counts (at most max_chunks loads); returns the
+// number of loads.
+int32_t clast_host_fasta_chunks(const char* path, int64_t avail, int32_t target, int32_t* counts,
+                                int32_t max_chunks) {
+  FastaLoader ld({std::string(path)}, static_cast<int>(avail), target != 0);
+  int32_t n = 0;
+  while (ld.file_index() != -1 && n < max_chunks) {
+    std::vector<Fasta> out;
+    ld.load(out);
+    counts[n++] = static_cast<int32_t>(out.size());
+  }
+  return n;
+}
+
 }  // extern "C"

@@ -112,6 +112,29 @@ def test_chunks_follow_loader_rule():
     assert keep.tolist() == [1, 1, 1, 1, 0, 1, 1, 1]
 
 
+def test_loader_skips_a_record_of_exactly_the_budget(tmp_path):
+    # judgeContinueReading (CFASTALoader.cpp:43-87) adds a record while
+    # sum + len < budget and skips one longer than the budget; a record of
+    # exactly the budget at the start of a load fits neither way, and the
+    # reference's loader would return to it forever.  This loader skips it
+    # with a warning, like an over-long one: loads [r0], [r2 r3].
+    import ctypes
+
+    f = tmp_path / "q.fa"
+    f.write_bytes(b">r0\n" + b"A" * 10 + b"\n>r1\n" + b"C" * 100 + b"\n>r2\n" + b"G" * 20 + b"\n>r3\n" + b"T" * 30
+                  + b"\n")
+    L = native.host()
+    L.clast_host_fasta_chunks.restype = ctypes.c_int32
+    L.clast_host_fasta_chunks.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32,
+                                          np.ctypeslib.ndpointer(np.int32), ctypes.c_int32]
+    counts = np.zeros(8, np.int32)
+    n = L.clast_host_fasta_chunks(str(f).encode(), 100, 0, counts, 8)
+    assert counts[:n].tolist() == [1, 2]
+    # and the in-memory chunking the package uses agrees
+    ch, keep = native.chunks(np.array([10, 100, 20, 30], np.int64), 100)
+    assert ch == [(0, 1), (1, 4)] and keep.tolist() == [1, 0, 1, 1]
+
+
 def run_clast(args, cwd):
     return subprocess.run([CLAST, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                           timeout=60)
