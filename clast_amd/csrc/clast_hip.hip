@@ -1862,4 +1862,59 @@ int clast_result_sort(int64_t n, int32_t* tID, int32_t* tIdx, int32_t* qID, int3
   });
 }
 
+int clast_result_finalize(int64_t n, const int32_t* tIdx, const int32_t* qID, const int32_t* qIdx,
+                          const int32_t* tHL, const int32_t* qHL, const int32_t* match, const int32_t* score,
+                          const double* evalue, const int32_t* qlabel, const int32_t* tlabel, const int32_t* tstart,
+                          int32_t num_out, int32_t* order, uint8_t* printed, int64_t* n_printed) {
+  return guard([&] {
+    if (n_printed) *n_printed = 0;
+    if (n < 0 || n >= (1ll << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "bad row count");
+    if (num_out < -1) throw ClastError(CLAST_ERR_ARG, "num_out must be -1 or >= 0");
+    if (n == 0) return;
+    const void* in[] = {tIdx, qID, qIdx, tHL, qHL, match, score, evalue, qlabel, tlabel, tstart, order, printed};
+    for (auto* c : in)
+      if (!c) throw ClastError(CLAST_ERR_ARG, "null column");
+    Runtime& rt = runtime();
+    std::lock_guard<std::mutex> lk(rt.mutex());
+    hipStream_t st = rt.stream();
+    Workspace& ws = rt.ws();
+    const uint32_t m = static_cast<uint32_t>(n);
+    DevArray<int32_t> d[10];
+    const int32_t* cols[10] = {tIdx, qID, qIdx, tHL, qHL, match, score, qlabel, tlabel, tstart};
+    for (int k = 0; k < 10; ++k) d[k].upload(cols[k], m, st);
+    DevArray<double> dev;
+    dev.upload(evalue, m, st);
+    // fixResult's stable order: (qID / 2 ascending, score descending)
+    uint64_t* w = ws.get<uint64_t>("fin.w", m);
+    uint32_t* perm = ws.get<uint32_t>("fin.perm", m);
+    k_result_keys<<<grid_for(m, kBlock), kBlock, 0, st>>>(m, d[1].ptr, d[6].ptr, w);
+    const uint64_t* words[1] = {w};
+    const int bits[1] = {64};
+    sort_perm(rt, m, 1, words, bits, perm);
+    // printResult's row selection in that order
+    uint32_t* dflag = ws.get<uint32_t>("fin.d", m);
+    uint8_t* nflag = ws.get<uint8_t>("fin.n", m);
+    uint32_t* head = ws.get<uint32_t>("fin.h", m);
+    uint32_t* dpre = ws.get<uint32_t>("fin.dp", m);
+    uint32_t* run = ws.get<uint32_t>("fin.r", m);
+    uint8_t* pr = ws.get<uint8_t>("fin.p", m);
+    uint32_t* pflag = ws.get<uint32_t>("fin.pf", m);
+    uint32_t* tot = ws.get<uint32_t>("fin.tot", 1);
+    const FinalCols fc{perm, d[7].ptr, d[1].ptr, d[2].ptr, d[4].ptr, d[8].ptr, d[0].ptr, d[9].ptr, d[3].ptr,
+                       d[5].ptr, d[6].ptr, dev.ptr};
+    const uint32_t g = grid_for(m, kBlock);
+    k_final_flags<<<g, kBlock, 0, st>>>(fc, m, dflag, nflag, head);
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, m, dpre, static_cast<uint32_t*>(nullptr), false);
+    device_scan<OpMax, uint32_t, uint32_t, uint32_t>(rt, head, m, run, static_cast<uint32_t*>(nullptr), true);
+    k_final_print<<<g, kBlock, 0, st>>>(m, num_out, dflag, nflag, dpre, run, pr, pflag);
+    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, pflag, m, dflag, tot, false);
+    rt.check_launch("clast_result_finalize");
+    hip_check(hipMemcpyAsync(order, perm, 4ull * m, hipMemcpyDeviceToHost, st), "D2H order");
+    hip_check(hipMemcpyAsync(printed, pr, m, hipMemcpyDeviceToHost, st), "D2H flags");
+    uint32_t np = 0;
+    rt.read_scalars(tot, &np, 1);  // waits for the copies too (same stream)
+    if (n_printed) *n_printed = np;
+  });
+}
+
 }  // extern "C"

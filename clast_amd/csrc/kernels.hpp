@@ -2712,6 +2712,74 @@ static __global__ __launch_bounds__(kBlock) void k_score_range(uint32_t n, const
   }
 }
 
+// ---------------------------------------------------------------- printResult row selection
+//
+// printResult (CHostResultHolder.cu:131-243) walks the fixResult order with
+// the last printed row's ten fields and a per-label count: a row equal to
+// the last printed one is skipped, and after a label's first row at most
+// numOut rows of that label print.  The last printed row always equals the
+// row just before in every field that decides: a row not printed was either
+// equal to the last printed row (then it has its fields) or past its label's
+// limit (then it has its label, and so does the next row of that label, also
+// past the limit).  So, in sorted order, with row i-1 for the last printed
+// row: distinct(i) = i is first or differs from i-1 in a printed field;
+// new(i) = its query label differs from i-1's (row 0: from the empty label
+// the walk starts with); printed(i) = distinct(i) and (new(i) or numOut = -1
+// or rank(i) < numOut), rank(i) = distinct rows of i's label run before i.
+struct FinalCols {
+  const uint32_t* perm;     // sorted position -> row
+  const int32_t* qlabel;    // per row: query label id (0: the empty label)
+  const int32_t* qid;
+  const int32_t* qidx;
+  const int32_t* qhl;
+  const int32_t* tlabel;    // per row: target label id
+  const int32_t* tidx;
+  const int32_t* tstart;    // per row: the target's start offset ('@' line), added to tIdx
+  const int32_t* thl;
+  const int32_t* match;
+  const int32_t* score;
+  const double* evalue;
+};
+
+static __global__ __launch_bounds__(kBlock) void k_final_flags(FinalCols c, uint32_t n, uint32_t* __restrict__ dflag,
+                                                               uint8_t* __restrict__ nflag,
+                                                               uint32_t* __restrict__ head) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = c.perm[i];
+  bool nw, dist;
+  if (i == 0) {
+    nw = c.qlabel[r] != 0;
+    dist = true;
+  } else {
+    const uint32_t o = c.perm[i - 1];
+    nw = c.qlabel[r] != c.qlabel[o];
+    dist = nw || c.qidx[r] != c.qidx[o] || ((c.qid[r] ^ c.qid[o]) & 1) != 0 || c.tlabel[r] != c.tlabel[o] ||
+           c.tidx[r] + c.tstart[r] != c.tidx[o] + c.tstart[o] || c.thl[r] != c.thl[o] || c.qhl[r] != c.qhl[o] || c.match[r] != c.match[o] ||
+           c.score[r] != c.score[o] || c.evalue[r] != c.evalue[o];
+  }
+  dflag[i] = dist ? 1u : 0u;
+  nflag[i] = nw ? 1 : 0;
+  head[i] = nw ? i : 0u;  // max-scanned: the start of i's label run
+}
+
+static __global__ __launch_bounds__(kBlock) void k_final_print(uint32_t n, int num_out,
+                                                               const uint32_t* __restrict__ dflag,
+                                                               const uint8_t* __restrict__ nflag,
+                                                               const uint32_t* __restrict__ dpre,
+                                                               const uint32_t* __restrict__ run,
+                                                               uint8_t* __restrict__ printed,
+                                                               uint32_t* __restrict__ pflag) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = run[i];                  // start of i's label run (0 before the first new label)
+  const bool nw = nflag[i] != 0;
+  const uint32_t rank = dpre[i] - dpre[s];    // distinct rows of the run before i
+  const bool p = dflag[i] && (nw || num_out == -1 || rank < static_cast<uint32_t>(num_out));
+  printed[i] = p ? 1 : 0;
+  pflag[i] = p ? 1u : 0u;
+}
+
 // resultSorting key for clast_result_sort: (qID/2, -score) biased to u64.
 static __global__ void k_result_keys(uint32_t n, const int32_t* __restrict__ qid, const int32_t* __restrict__ score,
                               uint64_t* __restrict__ w) {

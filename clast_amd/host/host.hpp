@@ -4,7 +4,9 @@
 // (include/clast_hip.h) only.
 #pragma once
 #include <cstdint>
+#include <ostream>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -77,23 +79,33 @@ struct SeqList {
 std::vector<std::pair<int, int>> target_windows(const std::vector<int32_t>& gateway, double vram_slots);
 std::vector<std::pair<int, int>> query_windows(const std::vector<int32_t>& gateway, double vram_slots);
 
-// CHostResultHolder (CHostResultHolder.cuh:8-44)
+// CHostResultHolder (CHostResultHolder.cuh:8-44).  Labels are kept as
+// integer ids (equal strings, equal ids) so that fixResult's order and
+// printResult's row selection run on the device (clast_result_finalize);
+// only the text of the printed rows is written on the host.
 class ResultHolder {
  public:
   explicit ResultHolder(const std::vector<Fasta>& queries);
   void add(clast_hits h, const SeqList& targets);  // addResult + addLabel + addStartIdx
   // the same from host arrays (n rows; cols: tID tIdx qID qIdx tHL qHL match score)
   void add_rows(const int32_t* const cols[8], const double* evalue, int64_t n, const SeqList& targets);
-  void fix();                                      // fixResult (device stable sort)
-  int64_t print(int num_out, const std::string& path) const;  // printResult; returns rows printed
+  // fixResult + printResult: order and selection on the device, the printed
+  // rows appended to `path`; returns rows printed
+  int64_t print(int num_out, const std::string& path) const;
+  // the text of rows order[k] with printed[k] set, in that order
+  void write_rows(std::ostream& os, const int32_t* order, const uint8_t* printed, int64_t n) const;
   int64_t size() const { return static_cast<int64_t>(col_[0].size()); }
 
  private:
-  std::vector<std::string> qlabel_;
+  int32_t target_label_id(const std::string& label);
+  std::vector<std::string> qlabel_;   // per query strand (qID)
+  std::vector<int32_t> qlabel_id_;    // per query strand; 0: the empty label
   std::vector<char> qstrand_;
-  std::vector<std::string> tlabel_;
-  std::vector<int32_t> tstart_;
-  std::vector<int32_t> col_[8];  // tID tIdx qID qIdx tHL qHL match score
+  std::vector<std::string> tname_;    // target labels by id
+  std::unordered_map<std::string, int32_t> tname_id_;
+  std::vector<int32_t> tlabel_;       // per row: target label id
+  std::vector<int32_t> tstart_;       // per row: target start offset
+  std::vector<int32_t> col_[8];       // tID tIdx qID qIdx tHL qHL match score
   std::vector<double> ev_;
 };
 

@@ -44,8 +44,7 @@ int main(int argc, const char** argv) {
         }
       }
       std::cout << " Fixing and printing the results." << std::endl;
-      holder.fix();
-      holder.print(s.num_out, s.output);
+      holder.print(s.num_out, s.output);  // fixResult + printResult
       std::cout << " ...finished." << std::endl;
       if (s.sleep > 0) {  // main.cu:94-103 cool-down, as a plain wait
         const std::clock_t c0 = std::clock();

@@ -107,6 +107,7 @@ def hip() -> C.CDLL:
             "clast_workspace_bytes": (C.c_int, [vp]),
             "clast_hits_destroy": (C.c_int, [vp]),
             "clast_result_sort": (C.c_int, [i64] + [vp] * 9),
+            "clast_result_finalize": (C.c_int, [i64] + [vp] * 11 + [i32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -518,3 +519,22 @@ def result_sort(rows: dict[str, np.ndarray]) -> None:
     for k, c in zip(HIT_FIELDS, cols):
         rows[k] = c
     rows["evalue"] = ev
+
+
+def result_finalize(rows: dict[str, np.ndarray], qlabel: np.ndarray, tlabel: np.ndarray, tstart: np.ndarray,
+                    num_out: int) -> tuple[np.ndarray, np.ndarray, int]:
+    """fixResult's order and printResult's row selection on the device
+    (clast_result_finalize): (order, printed flags per ordered position,
+    rows printed).  qlabel / tlabel / tstart are per-row label ids and
+    target start offsets."""
+    n = len(rows["qID"])
+    c = {k: np.ascontiguousarray(rows[k], dtype=np.int32) for k in HIT_FIELDS}
+    ev = np.ascontiguousarray(rows["evalue"], dtype=np.float64)
+    ql, tl, ts = (np.ascontiguousarray(a, dtype=np.int32) for a in (qlabel, tlabel, tstart))
+    order = np.zeros(max(n, 1), np.int32)
+    printed = np.zeros(max(n, 1), np.uint8)
+    npr = C.c_int64(0)
+    _check(hip().clast_result_finalize(n, *[_ptr(c[k]) for k in HIT_FIELDS[1:]], _ptr(ev), _ptr(ql), _ptr(tl),
+                                       _ptr(ts), num_out, _ptr(order), _ptr(printed), C.byref(npr)),
+           "clast_result_finalize")
+    return order[:n], printed[:n], int(npr.value)

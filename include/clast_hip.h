@@ -258,6 +258,23 @@ int clast_hits_destroy(clast_hits h);
 int clast_result_sort(int64_t n, int32_t* tID, int32_t* tIdx, int32_t* qID, int32_t* qIdx, int32_t* tHL,
                       int32_t* qHL, int32_t* match, int32_t* score, double* evalue);
 
+/* CHostResultHolder::fixResult + the row selection of printResult
+ * (CHostResultHolder.cu:75-128, 131-243) on the device, for the n rows the
+ * holder accumulated (its append order): order[k] receives the row index of
+ * position k of fixResult's stable order (query pair ascending, score
+ * descending), and printed[k] whether printResult writes that row: a row
+ * equal to the last printed one in all ten printed fields is skipped, and
+ * after the first row of a query label at most num_out rows of it print
+ * (-1: no limit).  Labels are compared as integers: qlabel / tlabel give each
+ * row's query / target label id (equal strings, equal ids; query id 0 must
+ * be the empty label, which printResult's walk starts from), tstart each
+ * row's target start offset ('@' line, added to tIdx as the row prints it).
+ * *n_printed receives the number of printed rows. */
+int clast_result_finalize(int64_t n, const int32_t* tIdx, const int32_t* qID, const int32_t* qIdx,
+                          const int32_t* tHL, const int32_t* qHL, const int32_t* match, const int32_t* score,
+                          const double* evalue, const int32_t* qlabel, const int32_t* tlabel, const int32_t* tstart,
+                          int32_t num_out, int32_t* order, uint8_t* printed, int64_t* n_printed);
+
 #ifdef __cplusplus
 }
 #endif
