@@ -27,24 +27,87 @@ inline void hip_check(hipError_t e, const char* what) {
   throw ClastError(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// A device allocation owned by a handle.
+// Device blocks of released handles, kept per device for the next handle:
+// hipFree waits for the whole device, so a query set or result buffer freed
+// by one host thread would stall the searches other threads have in flight.
+// Blocks come in size classes (powers of two from 64 KiB, quarter steps
+// above 64 MiB); at most kCacheBytes per device are kept, the rest is freed.
+// A block is only released once no queued work uses it (every entry point
+// that frees synchronises its stream first).
+class DeviceCache {
+ public:
+  static DeviceCache& get() {
+    static DeviceCache* c = new DeviceCache();  // never destroyed: blocks may outlive static teardown
+    return *c;
+  }
+  static uint64_t size_class(uint64_t bytes) {
+    uint64_t c = 64ull << 10;
+    while (c < bytes && c < (64ull << 20)) c <<= 1;
+    if (c >= bytes) return c;
+    const uint64_t q = 16ull << 20;  // quarter of 64 MiB steps, then of each power of two
+    uint64_t p = 64ull << 20;
+    while (p * 2 < bytes) p <<= 1;
+    const uint64_t step = std::max(q, p / 4);
+    return (bytes + step - 1) / step * step;
+  }
+  void* take(int dev, uint64_t bytes) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto& fl = free_[dev];
+      auto it = fl.lower_bound(bytes);
+      if (it != fl.end() && it->first <= bytes + bytes / 4) {
+        void* p = it->second;
+        cached_[dev] -= it->first;
+        fl.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+  }
+  void give(int dev, void* p, uint64_t bytes) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (cached_[dev] + bytes <= kCacheBytes) {
+        free_[dev].emplace(bytes, p);
+        cached_[dev] += bytes;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+
+ private:
+  static constexpr uint64_t kCacheBytes = 16ull << 30;
+  std::mutex mu_;
+  std::map<int, std::multimap<uint64_t, void*>> free_;
+  std::map<int, uint64_t> cached_;
+};
+
+// A device allocation owned by a handle (through DeviceCache).
 template <class T>
 struct DevArray {
   T* ptr = nullptr;
   uint64_t n = 0;
+  uint64_t bytes = 0;
+  int dev = 0;
   DevArray() = default;
   DevArray(const DevArray&) = delete;
   DevArray& operator=(const DevArray&) = delete;
   ~DevArray() { reset(); }
   void reset() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) DeviceCache::get().give(dev, ptr, bytes);
     ptr = nullptr;
     n = 0;
+    bytes = 0;
   }
   void alloc(uint64_t count) {
     reset();
     if (count == 0) count = 1;
-    hip_check(hipMalloc(&ptr, sizeof(T) * count), "hipMalloc");
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    bytes = DeviceCache::size_class(sizeof(T) * count);
+    ptr = static_cast<T*>(DeviceCache::get().take(dev, bytes));
     n = count;
   }
   void upload(const T* host, uint64_t count, hipStream_t s) {
