@@ -62,7 +62,13 @@ N_CU = 256
 # 128 16-bit lane-ops when packed (two per 32-bit lane)
 VALU_INT32_PEAK_TOPS = N_CU * 64 * CLOCK_GHZ / 1e3
 VALU_INT16_PEAK_TOPS = N_CU * 128 * CLOCK_GHZ / 1e3
-OPS_PER_CELL = 14  # int32-equivalent operations per DP cell (v, h, diagonal, max, match-count choice)
+# int32-equivalent operations per DP cell: vertical and horizontal gap terms
+# (add + max each, and the add of the open penalty), diagonal (substitution
+# choice + add), best of three (2 max), match count (2 choices + add + the
+# compare that picks the winning term); the score pass of a two-pass DP
+# stage computes no match counts: 10
+OPS_PER_CELL = 14
+OPS_PER_CELL_SCORE = 10
 METRIC = "query Mbp/s aligned (bit-exact hits) + p50/p99 request latency, 1 MI355X"
 
 
@@ -172,16 +178,20 @@ def kernel_roofline(records, name: str) -> tuple[dict, dict]:
     """Per-kernel device time over the timed batches and the roofline of the
     kernel with the most time."""
     kt = np.array([r[2] for r in records], dtype=np.float64).reshape(-1, 8)
-    cnt = np.array([r[3] for r in records], dtype=np.float64).reshape(-1, 8)
+    dpk = np.array([r[4] for r in records], dtype=np.float64).reshape(-1, 12)
     pos = np.array([r[1] for r in records], dtype=np.float64)
-    cells, cells16, cellsband = cnt[:, 2], cnt[:, 5], cnt[:, 7]
-    cells32 = cells - cells16 - cellsband
+    # each DP kernel's own cells over its own event-timed span (the
+    # past-the-end k_dp16 launch, k_dp and k_dp_band run beside k_dp16 on a
+    # second stream)
+    ops16 = np.where(dpk[:, 11] > 0, OPS_PER_CELL_SCORE, OPS_PER_CELL)
     kernels = {
-        # name: (ms, launches, work, unit, peak, bound)
+        # name: (ms, work, unit, peak, bound)
         "k_seed_count": (kt[:, 0], 16.0 * pos, "GB/s", HBM_PEAK_GBS, "hbm"),
-        "k_dp16<8>": (kt[:, 1], OPS_PER_CELL * cells16, "Tops/s", VALU_INT16_PEAK_TOPS, "valu_int16"),
-        "k_dp<8,false>": (kt[:, 2], OPS_PER_CELL * cells32, "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
-        "k_dp_band<8,false>": (kt[:, 3], OPS_PER_CELL * cellsband, "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
+        "k_dp16<8>": (dpk[:, 4], ops16 * dpk[:, 0], "Tops/s", VALU_INT16_PEAK_TOPS, "valu_int16"),
+        "k_dp16<8,edge>": (dpk[:, 5], ops16 * dpk[:, 1], "Tops/s", VALU_INT16_PEAK_TOPS, "valu_int16"),
+        "dp_second_pass": (dpk[:, 10], OPS_PER_CELL * dpk[:, 9], "Tops/s", VALU_INT16_PEAK_TOPS, "valu_int16"),
+        "k_dp<8,false>": (dpk[:, 6], OPS_PER_CELL * dpk[:, 2], "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
+        "k_dp_band<8,false>": (dpk[:, 7], OPS_PER_CELL * dpk[:, 3], "Tops/s", VALU_INT32_PEAK_TOPS, "valu_int32"),
     }
     per = {}
     for k, (ms, work, unit, peak, bound) in kernels.items():
@@ -206,7 +216,9 @@ def kernel_roofline(records, name: str) -> tuple[dict, dict]:
             "avg_launch_us": per[dom]["avg_us"],
             "traffic": measured_traffic(name, dom.split("<")[0]),
             "work_per_unit": ("16 B per query k-mer position" if unit == "GB/s" else
-                              f"{OPS_PER_CELL} int32-equivalent ops per DP cell (band width x rows)")}
+                              f"{OPS_PER_CELL} int32-equivalent ops per DP cell (band width x rows), "
+                              f"{OPS_PER_CELL_SCORE} in the score pass of a two-pass DP stage"),
+            "two_pass_searches": int((dpk[:, 11] > 0).sum()), "searches": int(dpk.shape[0])}
     if bound == "valu_int16":
         roof["peak_note"] = (f"packed 16-bit VALU: {N_CU} CU x 128 lane-ops per clock at {CLOCK_GHZ} GHz; "
                              f"the int32 peak is {VALU_INT32_PEAK_TOPS:.1f}")
@@ -325,7 +337,7 @@ def large_leg(steps: int, warmup: int, streams: int) -> dict:
         h.wait()
         n = len(h)
         if record is not None:
-            record.append((h.kernel_times(), h.counts(), ch))
+            record.append((h.kernel_times(), h.counts(), ch, h.dp_kernels()))
         h.close()
         q.close()
         return n
@@ -362,10 +374,11 @@ def large_leg(steps: int, warmup: int, streams: int) -> dict:
     ix.close()
     kt = np.array([x[0] for x in rec], dtype=np.float64)
     cnt = np.array([x[1] for x in rec], dtype=np.float64)
+    dpk = np.array([x[3] for x in rec], dtype=np.float64)
     npos = np.array([float(2 * np.clip(np.diff(x[2].offsets) - p.lmer + 1, 0, None).sum()) for x in rec])
     seed_ms = float(kt[:, 0].sum())
     achieved = 16.0 * npos.sum() / (seed_ms / 1e3) / 1e9
-    dp16_ms = float(kt[:, 1].sum())
+    dp16_ms = float(dpk[:, 4].sum())
     step_ms = float(np.mean(ms))
     return {
         "workload": f"large: {t.bases.shape[0] / 1e9:.3g} Gbp target ({len(t)} seqs, 10% repeats) x {len(r)} reads "
@@ -385,7 +398,7 @@ def large_leg(steps: int, warmup: int, streams: int) -> dict:
         "seed_lookup_roofline": {"kernel": "k_seed_count", "bound": "hbm", "achieved": round(achieved, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                                  "traffic": measured_traffic("large", "k_seed_count")},
-        "dp16": {"GCUPS": round(float(cnt[:, 5].sum()) / (dp16_ms / 1e3) / 1e9, 1) if dp16_ms else 0.0,
+        "dp16": {"GCUPS": round(float(dpk[:, 0].sum()) / (dp16_ms / 1e3) / 1e9, 1) if dp16_ms else 0.0,
                  "cells_16bit_frac": round(float(cnt[:, 5].sum()) / max(float(cnt[:, 2].sum()), 1.0), 4)},
     }
 
@@ -490,6 +503,8 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-capacity", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving headline (only the other legs, e.g. to profile them alone)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only copy MiB of device memory once (the known-size kernel that scales the HBM "
                          "traffic counters) and exit")
@@ -511,6 +526,14 @@ def main() -> int:
     from clast_amd import configs, gen, native, serve
 
     native.init(0)
+    if args.no_serve:
+        out = {"metric": METRIC, "value": None, "note": "serving headline skipped (--no-serve)"}
+        if not args.no_large:
+            out["large"] = large_leg(args.large_steps, 1, 3)
+        if not args.no_uneven:
+            out["uneven_batched_vs_plain"] = uneven_leg(args.uneven_chunks, 3)
+        print(json.dumps(out), flush=True)
+        return 0
     t0 = time.perf_counter()
     t, p, tch = serving_target()
     n_twin = sum(len(w) for _, w in tch)
@@ -551,10 +574,10 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16" if cells16_frac >= 0.5 else "int32",
         "dp_cells_int16_frac": round(cells16_frac, 4),
-        "dtype_note": "integer scores; DP cells whose range proof allows it run in packed 16-bit lanes with "
-                      "bit-identical results (dp_cells_int16_frac of all cells)",
+        "dtype_note": "integer scores: DP cells whose range proof allows it (dp_cells_int16_frac of all cells) run "
+                      "in packed 16-bit lanes with results identical to the int32 kernel's, the rest in int32",
         "data": "synthetic",
         "config": {
             "workload": f"serving: {t.bases.shape[0] / 1e9:.2f} Gbp target ({len(t)} seqs, 40% repeats, 1% low "

@@ -35,7 +35,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-HIP_UNITS = ["clast_hip.hip", "dp_global.hip", "dp_local.hip", "dp16.hip", "dp_band.hip"]
+HIP_UNITS = ["clast_hip.hip", "dp_global.hip", "dp_local.hip", "dp16.hip", "dp16_score.hip", "dp_band.hip"]
 HOST_UNITS = ["settings.cpp", "fasta.cpp", "seqlist.cpp", "holder.cpp", "scheduler.cpp", "host_capi.cpp"]
 
 

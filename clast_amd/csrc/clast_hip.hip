@@ -245,6 +245,10 @@ struct clast_hits_s {
   clast::DPConfig dpcfg;  // DP configuration the search ran with
   // raw seeds, DP hits, DP cells, rows, 16-bit tasks, their cells, long tasks, their cells
   int64_t counts[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // clast_hits_dp_kernels: DP cells of k_dp16, of its past-the-end launch, of
+  // k_dp, of k_dp_band, the four kernels' device times (ms), then the second
+  // pass's tasks, cells and time (ms), and 1 for a two-pass DP stage
+  double dpk[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -660,7 +664,9 @@ enum {
   EV_START, EV_COUNT0, EV_COUNT1, EV_WRITE0, EV_WRITE1, EV_SORT0, EV_SORT1, EV_FILT0, EV_FILT1, EV_DP0, EV_DP1,
   EV_RES0, EV_RES1,
   // the three DP kernels, each bracketed on the stream it runs on
-  EV_D16_0, EV_D16_1, EV_D32_0, EV_D32_1, EV_DBAND0, EV_DBAND1, kEvents
+  EV_D16_0, EV_D16_1, EV_D32_0, EV_D32_1, EV_DBAND0, EV_DBAND1, EV_E16_0, EV_E16_1,
+  // the second pass of a two-pass DP stage (side stream)
+  EV_P2_0, EV_P2_1, kEvents
 };
 
 struct StageEvents {
@@ -726,10 +732,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const int d_off = q.max_qlen;
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
   // [2, 10): DP cell counters, [10, 18): rows of 16-bit DP tasks, [18]: their
-  // count, [19]: long task count, [20, 28): their rows, [28, 36): all DP rows
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 36);
-  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 36, 0ull);
-  uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 8);
+  // count, [19]: long task count, [20, 28): their rows, [28, 36): all DP rows,
+  // [36, 44): rows of the second DP pass's tasks
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 44);
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 44, 0ull);
+  // [8, 11): task counts of the second DP pass's three lists
+  uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 12);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
 
@@ -792,7 +800,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* ovf_list = ws.get<uint32_t>("s.ovf", q.nseq);
     uint8_t* ovf_flag = ws.get<uint8_t>("s.ovfflag", q.nseq);
     uint32_t novf = 0;
-    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 8, 0u);
+    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 12, 0u);
     const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
     const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
     const bool packed = wbits + gbits + dbits + qbits <= 63;
@@ -954,6 +962,15 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
 
   // banded DP
   int32_t *ot = nullptr, *oq = nullptr, *thl = nullptr, *qhl = nullptr, *om = nullptr, *osc = nullptr;
+  int4* dres = nullptr;
+  // two-pass DP stage (tuner.hpp): the first pass's arguments and task
+  // order, which the second pass reads
+  struct SecondPass {
+    bool on = false;
+    DPArgs da{};
+    uint32_t nslot = 0;
+    DPTuner::Key key{0, 0, 0};
+  } sp;
   if (nhit) {
     ot = ws.get<int32_t>("d.t", nhit); oq = ws.get<int32_t>("d.q", nhit);
     thl = ws.get<int32_t>("d.thl", nhit); qhl = ws.get<int32_t>("d.qhl", nhit);
@@ -966,10 +983,14 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     const uint32_t nslot = (ntask + qstep - 1) / qstep * qstep;
     uint64_t* dk = ws.get<uint64_t>("d.key", nslot);
     uint32_t* dporder = ws.get<uint32_t>("d.order", nslot);
-    int4* dres = ws.get<int4>("d.res", ntask);
+    dres = ws.get<int4>("d.res", ntask);
     const int maxrows = std::max(q.max_qlen - p.lmer, 0);
     const int rowbits = bits_for(static_cast<uint64_t>(maxrows) + 1);
     const bool packed = !p.local && !dp16_disabled();
+    const DPTuner::Key tkey{p.gap, p.local ? 1 : 0, dp_bucket(maxrows)};
+    // (the allocation-free entry point and the stage read-back run one pass)
+    sp.on = packed && use_cut && !keep_stages && !rows_out && DPTuner::get().two_pass(tkey, forced.two_pass);
+    auto* const launch16 = sp.on ? launch_dp16_score : launch_dp16;
     uint32_t* split = ws.get<uint32_t>("d.split", 3);
     int* band_rows = ws.get<int>("d.bandrows", 1);
     uint32_t* chunk_next = ws.get<uint32_t>("d.chunk", 1);
@@ -1070,12 +1091,14 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
           launch_dp_band(p.gap, false, da, ss);
           fmark_on(EV_DBAND1, ss);
         }
-        launch_dp16(p.gap, true, false, 0, da, ss);
+        fmark_on(EV_E16_0, ss);
+        launch16(p.gap, true, false, 0, da, ss);
+        fmark_on(EV_E16_1, ss);
         fmark_on(EV_D32_0, ss);
         launch_dp_global(p.gap, da, ss);
         fmark_on(EV_D32_1, ss);
         fmark(EV_D16_0);
-        launch_dp16(p.gap, false, c.sched == 1, 4 * cu, da, st);
+        launch16(p.gap, false, c.sched == 1, 4 * cu, da, st);
         fmark(EV_D16_1);
         rt.join(0);
       } else {
@@ -1098,7 +1121,6 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     // configuration: the tuner's for this kind of search, found now by timed
     // trials (coordinate descent over sched, band, radix; every trial writes
     // every task's result, so the trials change no row) if it has none yet
-    const DPTuner::Key tkey{p.gap, p.local ? 1 : 0, dp_bucket(maxrows)};
     DPConfig cfg;
     const bool tuned = DPTuner::get().lookup(tkey, &cfg);
     cfg = forced.apply(tuned ? cfg : DPConfig{});
@@ -1142,6 +1164,11 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     }
     const DPArgs da = dp_chain(cfg, true);
     out.dpcfg = cfg;
+    if (sp.on) {
+      sp.da = da;
+      sp.nslot = nslot;
+      sp.key = tkey;
+    }
     k_dp_combine<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(da, nhit);
     rt.check_launch("k_dp_combine");
   }
@@ -1164,6 +1191,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   // strand order, so a pair's hits are contiguous)
   mark(EV_RES0);
   uint32_t nfinal = 0;
+  uint32_t p2n[3] = {0, 0, 0};  // second DP pass: lane-per-band, k_dp16, past-the-end tasks
+  bool p2_pending = false;
   uint32_t* order = nullptr;
   double* hev = nullptr;
   if (nhit) {
@@ -1180,12 +1209,64 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
                                        kflag, rk, p.lmer, 2 * p.gap + 1,
                                        reinterpret_cast<unsigned long long*>(tot64 + 2));
     rt.check_launch("k_hit_filter");
+    uint32_t* order2 = nullptr;
+    uint32_t* split2 = nullptr;
+    if (sp.on) {
+      // the second pass's three task lists (k_redo_flags), counts to tot32[8, 11)
+      const uint64_t n3 = 3ull * sp.nslot;
+      uint32_t* rfl = ws.get<uint32_t>("p2.flags", n3);
+      uint32_t* rof = ws.get<uint32_t>("p2.offs", n3);
+      order2 = ws.get<uint32_t>("p2.order", sp.nslot);
+      split2 = ws.get<uint32_t>("p2.split", 3);
+      const uint32_t gs2 = grid_for(sp.nslot, kBlock);
+      k_redo_flags<<<gs2, kBlock, 0, st>>>(sp.da.order, sp.da.split, kflag, hs, hq, q.qlen.ptr, p.lmer, sp.nslot,
+                                           rfl, reinterpret_cast<unsigned long long*>(tot64 + 36));
+      for (int c = 0; c < 3; ++c)
+        device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, rfl + c * static_cast<uint64_t>(sp.nslot), sp.nslot,
+                                                         rof + c * static_cast<uint64_t>(sp.nslot), tot32 + 8 + c,
+                                                         false, "p2.tiles");
+      k_redo_scatter<<<gs2, kBlock, 0, st>>>(sp.da.order, rfl, rof, sp.nslot, tot32 + 8, order2, split2);
+      rt.check_launch("k_redo_flags/scatter");
+    }
     PairArgs pa{nhit, q.nseq, hbase, hs, hgs, ot, oq, kflag, rk};
     k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 4, 1, 0u);
     k_pair_dedup<<<g, kBlock, 0, st>>>(pa, dflag, gflag, tot32 + 4);
     rt.check_launch("k_pair_dedup");
-    uint32_t nbig = 0;
-    rt.read_scalars(tot32 + 4, &nbig, 1);
+    // tot32[4] and the second pass's list counts, tot32[8, 11)
+    uint32_t rb[7];
+    rt.read_scalars(tot32 + 4, rb, 7);
+    const uint32_t nbig = rb[0];
+    if (sp.on) {
+      p2n[0] = rb[4];
+      p2n[1] = rb[5];
+      p2n[2] = rb[6];
+    }
+    if (p2n[0] + p2n[1] + p2n[2] > 0) {
+      // the second pass on the side stream, beside the row ordering below;
+      // joined before the rows are gathered
+      DPArgs d2 = sp.da;
+      d2.order = order2;
+      d2.split = split2;
+      hipStream_t ss = rt.side_stream();
+      rt.fork(1);
+      mark_on(EV_P2_0, ss);
+      if (p2n[0]) {
+        d2.n = p2n[0];
+        launch_dp_band(p.gap, false, d2, ss);
+      }
+      if (p2n[1]) {
+        d2.n = p2n[1];
+        launch_dp16(p.gap, false, false, 0, d2, ss);
+      }
+      if (p2n[2]) {
+        d2.n = p2n[2];
+        launch_dp16(p.gap, true, false, 0, d2, ss);
+      }
+      k_match_patch<<<g, kBlock, 0, ss>>>(nhit, kflag, dres, hs, hq, q.qlen.ptr, p.lmer, om);
+      mark_on(EV_P2_1, ss);
+      rt.check_launch("second DP pass");
+      p2_pending = true;
+    }
     uint32_t* grow = nullptr;
     uint32_t* gfirst = nullptr;
     uint32_t ngrow = 0;
@@ -1260,6 +1341,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       rt.check_launch("k_pair_place");
     }
   }
+  if (p2_pending) rt.join(1);
   // gather, then the copy to pinned host memory on the copy stream (the
   // caller's reads wait for it; the next search's kernels do not)
   out.wait();
@@ -1334,13 +1416,36 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       hip_check(hipEventElapsedTime(&t, ev->e[kpairs[k][0]], ev->e[kpairs[k][1]]), "elapsed");
     out.ktimes[k] = t;
   }
-  uint64_t cnt[26];
-  rt.read_scalars(tot64 + 2, cnt, 26);
+  uint64_t cnt[42];
+  rt.read_scalars(tot64 + 2, cnt, 42);
   uint64_t cells = 0, rows16 = 0, rowsband = 0;
+  uint64_t rows_main = 0;
   for (int k = 0; k < 8; ++k) {
     cells += cnt[k];
     rows16 += cnt[8 + k];
     rowsband += cnt[18 + k];
+    if (k < 4) rows_main += cnt[8 + k];
+  }
+  {
+    const double w = 2 * p.gap + 1;
+    out.dpk[0] = w * static_cast<double>(rows_main);
+    out.dpk[1] = w * static_cast<double>(rows16 - rows_main);
+    out.dpk[3] = w * static_cast<double>(rowsband);
+    out.dpk[2] = static_cast<double>(cells) - out.dpk[0] - out.dpk[1] - out.dpk[3];
+    const int dk[5][2] = {{EV_D16_0, EV_D16_1}, {EV_E16_0, EV_E16_1}, {EV_D32_0, EV_D32_1}, {EV_DBAND0, EV_DBAND1},
+                          {EV_P2_0, EV_P2_1}};
+    for (int k = 0; k < 5; ++k) {
+      float t = 0;
+      if (timed[dk[k][0]] && timed[dk[k][1]])
+        hip_check(hipEventElapsedTime(&t, ev->e[dk[k][0]], ev->e[dk[k][1]]), "elapsed");
+      out.dpk[k < 4 ? 4 + k : 10] = t;
+    }
+    uint64_t rows2 = 0;
+    for (int k = 34; k < 42; ++k) rows2 += cnt[k];
+    out.dpk[8] = static_cast<double>(p2n[0]) + p2n[1] + p2n[2];
+    out.dpk[9] = sp.on ? w * static_cast<double>(rows2) : 0.0;
+    out.dpk[11] = sp.on ? 1.0 : 0.0;
+    if (sp.on) DPTuner::get().survived(sp.key, static_cast<uint64_t>(out.dpk[8]), cnt[16]);
   }
   out.counts[0] = nseed;
   out.counts[1] = nhit;
@@ -1775,6 +1880,13 @@ int clast_hits_kernel_times(clast_hits h, float* out8) {
   return guard([&] {
     if (!h || !out8) throw ClastError(CLAST_ERR_ARG, "null argument");
     std::memcpy(out8, h->ktimes, sizeof h->ktimes);
+  });
+}
+
+int clast_hits_dp_kernels(clast_hits h, double* out12) {
+  return guard([&] {
+    if (!h || !out12) throw ClastError(CLAST_ERR_ARG, "null argument");
+    std::memcpy(out12, h->dpk, sizeof h->dpk);
   });
 }
 

@@ -1772,8 +1772,9 @@ struct PairRows {
 // One query row (k_dp's dp_row, global mode) for two tasks.  Z holds S - 7:
 // both gap terms then read it unchanged, and the diagonal term is
 // S + (4 mc - 3) = Z + 4 + 4 mc.  T[b-1] and Q hold the target and query
-// codes of both halves; mc = 1 - min(t ^ q, 1).
-template <int W, int O, bool EDGE>
+// codes of both halves; mc = 1 - min(t ^ q, 1).  SCORE: scores only (the
+// match counts M are left as they are; they do not feed the scores).
+template <int W, int O, bool EDGE, bool SCORE>
 __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
                                          const uint32_t (&T)[W + 4], uint32_t Q) {
   s16x2 hp = pk_splat(-30000);
@@ -1790,18 +1791,51 @@ __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s
       pt = pt - xb * pk_splat(12);
     }
     const s16x2 vh = __builtin_elementwise_max(v, h);
-    const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
-    const uint32_t nm = pk_pick(pk_less(vh, pt), pk_bits(M[b] + mc), ngap);
+    if constexpr (!SCORE) {
+      const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
+      const uint32_t nm = pk_pick(pk_less(vh, pt), pk_bits(M[b] + mc), ngap);
+      M[b] = pk_val(nm);
+    }
     Z[b] = __builtin_elementwise_max(pt, vh) + pk_splat(kGapOpen + kGapExt);
     V[b] = v;
-    M[b] = pk_val(nm);
+    hp = h;
+  }
+}
+
+// One query row of the score pass (SCORE, not EDGE) in drifted units: every
+// value of band b at row r is held as value + d(r, b), d = 2 b + 4 (r mod 16)
+// (Z as Z + 2 + d, i.e. S - 5 + d), which the gap terms' -2 steps cancel:
+// v(r, b) reads band b + 1 of row r - 1 (d two lower, Z two higher) and
+// h(r, b) band b - 1 of row r (the same), so
+//   v~ = max(Z~[b+1], V~[b+1]),  h~ = max(Z~[b-1], h~[b-1]),
+//   pt~ = Z~[b] + 6 + 4 mc  (the diagonal: d four lower, Z two higher),
+//   Z~ = max(pt~, v~, h~) - 5,
+// eight instructions per cell pair.  6 + 4 mc comes from one byte permute of
+// the class codes' xor (Qs: the query codes with 0x0C in bytes 1 and 3, which
+// select 0 for the high bytes).  Band 1's horizontal term and band W's
+// vertical term are constants of k_dp's edges (-30002 and the band W + 1
+// constants): h1 and vw, drifted by the caller.  The values change only by
+// d, and |d| <= 2 W + 60, inside the 16-bit margins of k_dp16's range proof.
+template <int W, int O>
+__device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
+                                               uint32_t Qs, s16x2 h1, s16x2 vw) {
+  s16x2 hp = h1;
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const s16x2 v = b == W ? vw : __builtin_elementwise_max(Z[b + 1], V[b + 1]);
+    const s16x2 h = b == 1 ? h1 : __builtin_elementwise_max(Z[b - 1], hp);
+    const s16x2 w = pk_val(__builtin_amdgcn_perm(0x06060606u, 0x0606060Au, T[O + b - 1] ^ Qs));
+    const s16x2 pt = Z[b] + w;
+    Z[b] = __builtin_elementwise_max(pt, __builtin_elementwise_max(v, h)) + pk_splat(-5);
+    V[b] = v;
     hp = h;
   }
 }
 
 // End node of the last row (k_dp's global selection: strict '<' from the
-// constant band 0) of both halves; half h goes to dres[task] when store.
-template <int G>
+// constant band 0) of half `half`, written to dres[task].  SCORE: the match
+// count is written as 0.
+template <int G, bool SCORE>
 __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x2 (&M)[2 * G + 3], int4* dres,
                                          uint32_t task, int half, int rows) {
   constexpr int W = 2 * G + 1;
@@ -1811,13 +1845,34 @@ __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x
   for (int b = 1; b <= W; ++b) {
     const uint32_t m = pk_less(bz, Z[b]);
     bz = __builtin_elementwise_max(bz, Z[b]);
-    bm = pk_pick(m, pk_bits(M[b]), bm);
+    if constexpr (!SCORE) bm = pk_pick(m, pk_bits(M[b]), bm);
     be = pk_pick(m, pk_bits(pk_splat(b - 1)), be);
   }
   const int sc = (half ? bz.y : bz.x) - (kGapOpen + kGapExt);
-  const int mt = half ? pk_val(bm).y : pk_val(bm).x;
+  const int mt = SCORE ? 0 : half ? pk_val(bm).y : pk_val(bm).x;
   const int e = (half ? pk_val(be).y : pk_val(be).x) - G;
   dres[task] = make_int4(rows + e, rows, mt, sc);
+}
+
+// dp16_end of the score pass in drifted units (dp_row16_drift): band b's Z
+// is Z~[b] - 2 - 2 b - d4 at the row whose drift is 2 b + d4.
+template <int G>
+__device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], int4* dres, uint32_t task, int half,
+                                               int rows, int d4) {
+  constexpr int W = 2 * G + 1;
+  // compared with the common d4 left in (band 0, never drifted, gets it)
+  s16x2 bz = Z[0] + pk_splat(d4);
+  uint32_t be = pk_bits(pk_splat(-1));
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const s16x2 z = Z[b] - pk_splat(2 + 2 * b);
+    const uint32_t m = pk_less(bz, z);
+    bz = __builtin_elementwise_max(bz, z);
+    be = pk_pick(m, pk_bits(pk_splat(b - 1)), be);
+  }
+  const int sc = (half ? bz.y : bz.x) - d4 - (kGapOpen + kGapExt);
+  const int e = (half ? pk_val(be).y : pk_val(be).x) - G;
+  dres[task] = make_int4(rows + e, rows, 0, sc);
 }
 
 // Tasks order[2t] and order[2t + 1] (neighbours in the longest-first order,
@@ -1826,9 +1881,11 @@ __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x
 // query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
 // One chunk of 64 task pairs: lane l runs pair 64 chunk + l of the
 // kernel's task range [first, first + n).
-template <int G, bool EDGE>
+template <int G, bool EDGE, bool SCORE>
 __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint32_t n, uint32_t chunk) {
   constexpr int W = 2 * G + 1;
+  // the score pass outside EDGE runs in drifted units (dp_row16_drift)
+  constexpr bool DRIFT = SCORE && !EDGE;
   // the target streams start kLead positions early, so that both streams
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
@@ -1870,10 +1927,17 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     } else {
       sc = (b - half) * kGapExt + kGapOpen; vv = sc + kGapOpen; mm = 0;
     }
-    Z[b] = pk_splat(sc + kGapOpen + kGapExt);
-    V[b] = pk_splat(vv);
+    // drifted (dp_row16_drift): as the last row of a group, d = 2 b + 60
+    const int dz = DRIFT && b >= 1 && b <= W ? 2 + 2 * b + 60 : 0;
+    const int dv = DRIFT && b >= 1 && b <= W ? 2 * b + 60 : 0;
+    Z[b] = pk_splat(sc + kGapOpen + kGapExt + dz);
+    V[b] = pk_splat(vv + dv);
     M[b] = pk_splat(mm);
   }
+  // k_dp's edges: band 1's horizontal term, band W's vertical term (the band
+  // W + 1 constants above)
+  constexpr int kH1 = -30000 + kGapExt;
+  constexpr int kVW = (W + 1 - half) * kGapExt + kGapOpen + kGapOpen + kGapExt;
   // target ring: stream positions kLead .. kLead + W - 1
   PairRows tp, qp;
   uint32_t T[W + 4];
@@ -1905,18 +1969,40 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
   // U once per U rows and, with U = 4, the code words shift at fixed rows.
   // Wider bands keep U = 1 (four rows' ring would not fit the registers).
   constexpr int U = G <= 8 ? 4 : 1;
+  // the end node of half h at its last row rws - 1 (drifted values back to
+  // k_dp's first)
+  auto end_node = [&](uint32_t task, int h, int rws) {
+    if constexpr (DRIFT) {
+      dp16_end_drift<G>(Z, a.dres, task, h, rws, 4 * ((rws - 1) & 15));
+    } else {
+      dp16_end<G, SCORE>(Z, M, a.dres, task, h, rws);
+    }
+  };
   auto row = [&](auto oi, int j, int r) {
     constexpr int i = decltype(oi)::value;
     const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
     T[W + i] = tp.next(j);          // target position r + W + kLead = r (mod 16)
     if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
-    dp_row16<W, i, EDGE>(Z, V, M, T, Q);
+    if constexpr (DRIFT) {
+      const int d4 = 4 * (r & 15);
+      dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
+    } else {
+      dp_row16<W, i, EDGE, SCORE>(Z, V, M, T, Q);
+    }
     if (r + 1 == rsplit) {
       const int h = rows[0] == rsplit ? 0 : 1;
-      dp16_end<G>(Z, M, a.dres, h ? task1 : task0, h, rsplit);
+      end_node(h ? task1 : task0, h, rsplit);
     }
   };
   auto rows16 = [&](int r0) {
+    if constexpr (DRIFT) {
+      // a new group: d drops by 64 (row r0 - 1 becomes row -1 of the group)
+#pragma unroll
+      for (int b = 1; b <= W; ++b) {
+        Z[b] = Z[b] - pk_splat(64);
+        V[b] = V[b] - pk_splat(64);
+      }
+    }
 #pragma unroll 1
     for (int j = 0; j < 16; j += U) {
       const int ju = __builtin_amdgcn_readfirstlane(j);
@@ -1962,8 +2048,8 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     rows16(rg + 16);
   }
   // the longer half (both when equal; a lone task's copy writes the same)
-  if (rows[0] >= rows[1] && rows[0] > 0) dp16_end<G>(Z, M, a.dres, task0, 0, rows[0]);
-  if (rows[1] >= rows[0] && rows[1] > 0) dp16_end<G>(Z, M, a.dres, task1, 1, rows[1]);
+  if (rows[0] >= rows[1] && rows[0] > 0) end_node(task0, 0, rows[0]);
+  if (rows[1] >= rows[0] && rows[1] > 0) end_node(task1, 1, rows[1]);
 }
 
 // The chunks of the range go out longest first.  Static (PERSIST false):
@@ -1973,8 +2059,10 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
 // device counter (a.chunk_next, zero before the launch) until none is left,
 // so a wavefront that finishes early takes over the remaining work instead
 // of waiting for its workgroup; every wavefront exits once the counter
-// passes the last chunk.
-template <int G, bool EDGE, bool PERSIST>
+// passes the last chunk.  SCORE: scores and end nodes only, match counts 0
+// (the score pass of a two-pass DP stage: the tasks of the hits that pass
+// the bad-hit filters run again without SCORE).
+template <int G, bool EDGE, bool PERSIST, bool SCORE>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
   const uint32_t first = a.split[EDGE ? 1 : 0];
   const uint32_t n = a.split[EDGE ? 2 : 1] - first;
@@ -1985,12 +2073,12 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
       if ((threadIdx.x & 63u) == 0) c = atomicAdd(a.chunk_next, 1u);
       c = __builtin_amdgcn_readfirstlane(c);
       if (c >= nchunk) return;
-      dp16_chunk<G, EDGE>(a, first, n, c);
+      dp16_chunk<G, EDGE, SCORE>(a, first, n, c);
     }
   } else {
     const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (wv >= nchunk) return;
-    dp16_chunk<G, EDGE>(a, first, n, snake_chunk(wv, nchunk, a.round));
+    dp16_chunk<G, EDGE, SCORE>(a, first, n, snake_chunk(wv, nchunk, a.round));
   }
 }
 
@@ -2239,9 +2327,10 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
 void launch_dp_global(int gap, const DPArgs& a, hipStream_t st);
 void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
 // k_dp16 over order[split[0], split[1]) and, with edge, over
-// order[split[1], split[2]) (dp16.hip), k_dp_band over order[0, split[0])
-// (dp_band.hip); a.n is the task count.
+// order[split[1], split[2]) (dp16.hip; its score-only form, dp16_score.hip),
+// k_dp_band over order[0, split[0]) (dp_band.hip); a.n bounds the task count.
 void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st);
+void launch_dp16_score(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st);
 void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
 // DP task order by class (key bits from classbit up): 0 the long tasks
@@ -2267,7 +2356,7 @@ struct TaskKeyArgs {
   int maxrows, rowshift, classbit;
   uint64_t* key;
   uint32_t* task;
-  unsigned long long* rows16;    // 8 counters (by workgroup): rows of the 16-bit tasks
+  unsigned long long* rows16;    // 8 counters: rows of the k_dp16 tasks [0, 4), past-the-end ones [4, 8)
   unsigned long long* rowsband;  // 8 counters: rows of the long tasks
 };
 
@@ -2277,7 +2366,7 @@ struct TaskKeyArgs {
 // the same key's tasks (stable sort: after them): the task count padded to
 // a bucket that a captured chain was built for.
 static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(const TaskKeyArgs* __restrict__ ap, uint32_t nslots) {
-  __shared__ uint32_t wsum[2 * (kBlock / 64)];
+  __shared__ uint32_t wsum[3 * (kBlock / 64)];
   const TaskKeyArgs& a = *ap;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   bool p16 = false, edge = false, long_task = false;
@@ -2317,17 +2406,22 @@ static __global__ __launch_bounds__(kBlock) void k_dp_task_keys(const TaskKeyArg
     a.key[t] = (cls << a.classbit) | static_cast<uint64_t>((a.maxrows - rows) >> a.rowshift);
     a.task[t] = t;
   }
-  const uint32_t r16 = wave_sum_incl(p16 || edge ? static_cast<uint32_t>(rows) : 0u);
+  const uint32_t r16 = wave_sum_incl(p16 ? static_cast<uint32_t>(rows) : 0u);
+  const uint32_t re = wave_sum_incl(edge ? static_cast<uint32_t>(rows) : 0u);
   const uint32_t rb = wave_sum_incl(long_task ? static_cast<uint32_t>(rows) : 0u);
   if (lane_id() == 63) {
     wsum[wave_id()] = r16;
-    wsum[kBlock / 64 + wave_id()] = rb;
+    wsum[kBlock / 64 + wave_id()] = re;
+    wsum[2 * (kBlock / 64) + wave_id()] = rb;
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 3) {
     unsigned long long tot = 0;
     for (int w = 0; w < kBlock / 64; ++w) tot += wsum[threadIdx.x * (kBlock / 64) + w];
-    if (tot) atomicAdd(&(threadIdx.x ? a.rowsband : a.rows16)[blockIdx.x & 7u], tot);
+    // rows16[0, 4): k_dp16 tasks, rows16[4, 8): its past-the-end tasks
+    unsigned long long* dst = threadIdx.x == 2 ? a.rowsband + (blockIdx.x & 7u)
+                                               : a.rows16 + 4 * threadIdx.x + (blockIdx.x & 3u);
+    if (tot) atomicAdd(dst, tot);
   }
 }
 
@@ -2414,6 +2508,93 @@ static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t
 //   c = (target sequence << 32) | diagonal order
 // (a, b, c) is the CDeviceHitList order of one unit; (a.low, a.high, b, c)
 // is the row order fixResult leaves among the pair's rows.
+// ----------------------------------------- two-pass DP stage, second pass
+//
+// The first pass ran the 16-bit tasks (order[split[0], split[2])) through
+// k_dp16's score pass: scores and end nodes, which is all the bad-hit
+// filters read.  The second pass runs the 16-bit tasks of the hits that
+// passed (kflag) again with match counts, in three lists that keep the
+// task order: tasks of at least kBandRows rows (k_dp_band, for latency),
+// k_dp16's others, and its past-the-end others.  flags holds one 0/1 array
+// of nslot entries per list; the rows of the listed tasks are summed into
+// rows_sum[0..8).
+static __global__ __launch_bounds__(kBlock) void k_redo_flags(const uint32_t* __restrict__ order,
+                                                              const uint32_t* __restrict__ split,
+                                                              const uint32_t* __restrict__ kflag,
+                                                              const int32_t* __restrict__ hs,
+                                                              const int32_t* __restrict__ hq,
+                                                              const int32_t* __restrict__ qlen, int lmer,
+                                                              uint32_t nslot, uint32_t* __restrict__ flags,
+                                                              unsigned long long* __restrict__ rows_sum) {
+  __shared__ unsigned long long wsum[kBlock / 64];
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t cat = 0;
+  int rows = 0;
+  if (j < nslot && j >= split[0] && j < split[2]) {
+    const uint32_t task = order[j];
+    const uint32_t i = task >> 1;
+    if (kflag[i]) {
+      rows = (task & 1u) == 0u ? qlen[hs[i]] - (hq[i] + lmer) : hq[i];
+      cat = rows >= kBandRows ? 1u : j < split[1] ? 2u : 3u;
+    }
+  }
+  if (j < nslot) {
+    flags[j] = cat == 1u;
+    flags[nslot + j] = cat == 2u;
+    flags[2ull * nslot + j] = cat == 3u;
+  }
+  unsigned long long c = cat ? static_cast<unsigned long long>(rows) : 0ull;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (lane_id() == 0) wsum[wave_id()] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kBlock / 64; ++k) t += wsum[k];
+    if (t) atomicAdd(&rows_sum[blockIdx.x & 7u], t);
+  }
+}
+
+// The three lists into order2 (list k at the sum of the counts before it;
+// offs: exclusive scans of the flags, cnt: their totals) and their bounds
+// into split2 in k_dp16 / k_dp_band's layout: [0, split2[0]) lane per band,
+// [split2[0], split2[1]) k_dp16, [split2[1], split2[2]) past the end.
+static __global__ __launch_bounds__(kBlock) void k_redo_scatter(const uint32_t* __restrict__ order,
+                                                                const uint32_t* __restrict__ flags,
+                                                                const uint32_t* __restrict__ offs, uint32_t nslot,
+                                                                const uint32_t* __restrict__ cnt,
+                                                                uint32_t* __restrict__ order2,
+                                                                uint32_t* __restrict__ split2) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t b1 = cnt[0], b2 = cnt[0] + cnt[1];
+  if (j == 0) {
+    split2[0] = b1;
+    split2[1] = b2;
+    split2[2] = b2 + cnt[2];
+  }
+  if (j >= nslot) return;
+  if (flags[j]) order2[offs[j]] = order[j];
+  else if (flags[nslot + j]) order2[b1 + offs[nslot + j]] = order[j];
+  else if (flags[2ull * nslot + j]) order2[b2 + offs[2ull * nslot + j]] = order[j];
+}
+
+// Match counts of the hits that passed the filters from the second pass's
+// results (k_dp_combine's sum; the other fields are the first pass's and
+// unchanged).
+static __global__ __launch_bounds__(kBlock) void k_match_patch(uint32_t nhit, const uint32_t* __restrict__ kflag,
+                                                               const int4* __restrict__ dres,
+                                                               const int32_t* __restrict__ hs,
+                                                               const int32_t* __restrict__ hq,
+                                                               const int32_t* __restrict__ qlen, int lmer,
+                                                               int32_t* __restrict__ om) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nhit || !kflag[i]) return;
+  const int qIdx = hq[i];
+  const int b = qlen[hs[i]] - (qIdx + lmer) > 0 ? dres[2 * i].z : 0;
+  const int f = qIdx > 0 ? dres[2 * i + 1].z : 0;
+  om[i] = lmer + b + f;
+}
+
 struct RowKey {
   uint64_t a, b, c;
 };

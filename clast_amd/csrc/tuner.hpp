@@ -16,6 +16,15 @@
 // radix 8, are then used); CLAST_DP_SCHED=static|persist, CLAST_DP_BAND=
 // off|rest|all and CLAST_RADIX_BITS=4..8 fix a knob (the tuner varies only
 // the others).
+//
+// Two-pass DP stage (global mode with the E-value cut): the 16-bit tasks run
+// first as scores only, and the tasks of the hits that pass the bad-hit
+// filters run again with match counts.  It pays while few 16-bit tasks
+// survive (the score pass issues 11 of the full kernel's 18 instructions per
+// cell pair), so the tuner keeps, per kind, the surviving share of the
+// 16-bit tasks of the searches so far, and a search runs two passes while
+// that share is below kTwoPassMax (and on the first search of its kind).
+// CLAST_DP_TWO_PASS=0|1 fixes the choice.
 #pragma once
 #include <cstdio>
 #include <cstdlib>
@@ -42,7 +51,7 @@ inline int dp_bucket(int maxrows) { return maxrows < 256 ? 0 : maxrows < 2048 ? 
 
 // Forced knobs from the environment (-1: free).
 struct DPForced {
-  int sched = -1, band = -1, radix = -1;
+  int sched = -1, band = -1, radix = -1, two_pass = -1;
   bool tune = true;
   static DPForced from_env() {
     DPForced f;
@@ -57,6 +66,11 @@ struct DPForced {
       else if (!std::strcmp(v, "rest")) f.band = 1;
       else if (!std::strcmp(v, "all")) f.band = 2;
       else throw ClastError(1, "CLAST_DP_BAND must be off, rest or all");
+    }
+    if (const char* v = std::getenv("CLAST_DP_TWO_PASS")) {
+      if (!std::strcmp(v, "0")) f.two_pass = 0;
+      else if (!std::strcmp(v, "1")) f.two_pass = 1;
+      else if (std::strcmp(v, "auto")) throw ClastError(1, "CLAST_DP_TWO_PASS must be 0, 1 or auto");
     }
     if (const char* v = std::getenv("CLAST_RADIX_BITS")) {
       const int b = std::atoi(v);
@@ -87,6 +101,25 @@ class DPTuner {
     *out = it->second.best;
     return true;
   }
+  // Two passes for the next search of kind k (forced: -1 free, 0, 1)?
+  bool two_pass(const Key& k, int forced) {
+    if (forced >= 0) return forced == 1;
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = survive_.find(k);
+    return it == survive_.end() || it->second < kTwoPassMax;
+  }
+  // A search of kind k ran `total` 16-bit tasks of which `kept` belong to
+  // hits that passed the filters (running mean, recent searches weighted).
+  void survived(const Key& k, uint64_t kept, uint64_t total) {
+    if (total == 0) return;
+    const double f = static_cast<double>(kept) / static_cast<double>(total);
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = survive_.find(k);
+    if (it == survive_.end()) survive_[k] = f;
+    else it->second = 0.75 * it->second + 0.25 * f;
+  }
+  static constexpr double kTwoPassMax = 0.4;
+
   void store(const Key& k, const DPConfig& best, const std::vector<std::pair<DPConfig, float>>& trials) {
     std::lock_guard<std::mutex> lk(mu_);
     if (table_.count(k)) return;  // another thread tuned it first
@@ -109,6 +142,11 @@ class DPTuner {
       }
       s += "\n";
     }
+    for (auto& kv : survive_) {
+      std::snprintf(buf, sizeof buf, "gap=%d local=%d bucket=%d: 16-bit tasks surviving the filters %.4f\n",
+                    std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
+      s += buf;
+    }
     return s;
   }
 
@@ -119,6 +157,7 @@ class DPTuner {
   };
   std::mutex mu_;
   std::map<Key, Entry> table_;
+  std::map<Key, double> survive_;
 };
 
 }  // namespace clast

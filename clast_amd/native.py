@@ -101,6 +101,7 @@ def hip() -> C.CDLL:
             "clast_hits_stage_copy": (C.c_int, [vp, i32] + [vp] * 8),
             "clast_hits_stage_times": (C.c_int, [vp, vp]),
             "clast_hits_kernel_times": (C.c_int, [vp, vp]),
+            "clast_hits_dp_kernels": (C.c_int, [vp, vp]),
             "clast_hits_dp_config": (C.c_int, [vp, vp]),
             "clast_tuner_report": (i64, [C.c_char_p, i64]),
             "clast_mem_info": (C.c_int, [vp, vp]),
@@ -452,6 +453,15 @@ class Hits:
         t = np.zeros(8, np.float32)
         _check(hip().clast_hits_kernel_times(self.h, _ptr(t)), "clast_hits_kernel_times")
         return t
+
+    def dp_kernels(self) -> np.ndarray:
+        """DP cells of k_dp16, of its past-the-end launch, of k_dp and of
+        k_dp_band, the same four kernels' device times (ms), then the second
+        DP pass's tasks, cells and time (ms), and 1.0 for a two-pass stage
+        (whose first-pass 16-bit kernels computed scores only)."""
+        d = np.zeros(12, np.float64)
+        _check(hip().clast_hits_dp_kernels(self.h, _ptr(d)), "clast_hits_dp_kernels")
+        return d
 
     def dp_config(self) -> tuple[int, int, int]:
         """(16-bit schedule, lane-per-band mode, radix digit bits) the

@@ -111,7 +111,7 @@ class Server:
         self.bytes_per_position = self.START_BYTES_PER_POSITION
         self._ws = {}        # worker -> (largest positions run, workspace bytes)
         self._lock = threading.Lock()
-        self.records = []    # per batch: (parts, positions, kernel times, counts)
+        self.records = []    # per batch: (parts, positions, kernel times, counts, DP kernels)
 
     def estimate(self, positions: int, slots: int) -> float:
         """Device bytes of a batch: the workspace estimate with a quarter of
@@ -192,7 +192,7 @@ class Server:
             rows = hits.arrays()
             if record:
                 with self._lock:
-                    self.records.append((len(parts), npos, hits.kernel_times(), hits.counts()))
+                    self.records.append((len(parts), npos, hits.kernel_times(), hits.counts(), hits.dp_kernels()))
             hits.close()
         finally:
             q.close()

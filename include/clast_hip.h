@@ -230,6 +230,16 @@ int clast_hits_counts(clast_hits h, int64_t* out8);
  * (as clast_hits_stage_times out[1]), [6] the result stage, [7] the whole
  * search.  0 for a kernel that did not run. */
 int clast_hits_kernel_times(clast_hits h, float* out8);
+/* No counterpart in the reference (rooflines): the DP cells (band width x
+ * rows) and device times (ms, hipEvent pairs on each kernel's stream) of the
+ * DP kernels: out[0] cells of k_dp16, [1] cells of the k_dp16 launch for
+ * bands past a sequence end, [2] cells of k_dp, [3] cells of k_dp_band,
+ * [4..8) the same four kernels' times; [8] tasks, [9] cells and [10] time of
+ * the second pass of a two-pass DP stage, [11] 1 if the stage ran two passes
+ * (the 16-bit kernels of the first then computed scores only).  The
+ * past-the-end launch, k_dp, k_dp_band and the second pass run on a second
+ * stream. */
+int clast_hits_dp_kernels(clast_hits h, double* out12);
 /* No counterpart in the reference (the DP autotuner): the DP launch
  * configuration the search ran with, out[0] schedule of the 16-bit kernel
  * (0 static, 1 persistent work-stealing wavefronts), [1] extensions run one

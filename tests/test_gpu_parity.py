@@ -663,3 +663,68 @@ def test_dp_order_graph_replays_equal_eager(monkeypatch):
         monkeypatch.delenv("CLAST_GRAPH")
     for h in (qa, qb, ix):
         h.close()
+
+
+@pytest.mark.parametrize("sched", ["static", "persist"])
+def test_dp_two_pass_equals_one_pass(sched, monkeypatch):
+    # Two-pass DP stage: the 16-bit tasks first as scores only, then the
+    # tasks of the hits that pass the bad-hit filters again with match
+    # counts (long ones one lane per band, the others in k_dp16, past-the-end
+    # ones with 'X').  A target with repeats and reads from clean to noisy
+    # (20% substitutions: extensions whose scores fail the E-value cut), some
+    # over 1024 rows, some off a sequence end, some with X letters: the
+    # two-pass rows equal the one-pass rows and the oracle's, the second pass
+    # runs some but not all 16-bit tasks, and without the E-value cut
+    # (cutOff -1) the stage runs one pass.
+    monkeypatch.setenv("CLAST_TUNE", "0")
+    monkeypatch.setenv("CLAST_DP_SCHED", sched)
+    monkeypatch.setenv("CLAST_DP_BAND", "off")
+    t, _ = workload(n_t=4, t_len=40_000, n_r=1, seed=91, repeat_frac=0.3)
+    tb = t.bases.copy()
+    o1 = int(t.offsets[1])
+    tb[o1 + 2000:o1 + 40_000:1777] = ord("X")
+    t = gen.Target(tb, t.offsets)
+    parts = [gen.make_reads(gen.ReadSpec(92, 40, len_mode=gen.LEN_LOGUNIFORM, len_a=1100, len_b=3000,
+                                         sub_rate=0.03, indel_rate=0.02), t),
+             gen.make_reads(gen.ReadSpec(93, 40, len_mode=gen.LEN_LOGUNIFORM, len_a=300, len_b=2500,
+                                         sub_rate=0.2, indel_rate=0.02), t),
+             gen.make_reads(gen.ReadSpec(94, 150, sub_rate=0.05, indel_rate=0.01, repeat_read_frac=0.5), t)]
+    seqs = [x.seq(i) for x in parts for i in range(len(x))]
+    seqs += [t.seq(0)[-1400:] + b"ACG", b"TTGA" + t.seq(2)[:1300], t.seq(3)[-200:]]
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    for cut in (10.0, -1.0):
+        p = native.Params(total_db_size=db_size(t), cut_off=cut)
+        tl, ql = lists(t, r, p.lmer)
+        twins = native.windows(tl.gateway, 90_000, query=False)
+        qwins = native.windows(ql.gateway, 80_000, query=True)
+        ix = native.Index(p, tl, twins)
+        q = native.Query(p, ql, qwins)
+        exp = oracle.search(p, tl, twins, ql, qwins)["final"]
+        print(f"dp two pass {sched} cutOff {cut}", flush=True)
+        got = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("CLAST_DP_TWO_PASS", mode)
+            h = native.search_batch(p, ix, q)
+            got[mode] = h.arrays()
+            d, c = h.dp_kernels(), h.counts()
+            assert d[11] == (1.0 if mode == "1" and cut > 0 else 0.0)
+            if d[11]:
+                assert 0 < d[8] < c[4], (d[8], c[4])
+                assert d[9] > 0
+            h.close()
+        for mode in ("1", "0"):
+            assert_rows_equal(got[mode], exp, what=f"two pass={mode} {sched} cutOff {cut}")
+        ix.close()
+        q.close()
+    monkeypatch.setenv("CLAST_DP_TWO_PASS", "2")
+    with pytest.raises(native.ClastError, match="CLAST_DP_TWO_PASS"):
+        native.search_batch(p, *_tiny_pair(p))
+
+
+def _tiny_pair(p):
+    t, r = workload(n_t=1, t_len=5_000, n_r=5, seed=95)
+    tl, ql = lists(t, r, p.lmer)
+    return (native.Index(p, tl, native.windows(tl.gateway, 90_000, query=False)),
+            native.Query(p, ql, native.windows(ql.gateway, 80_000, query=True)))
