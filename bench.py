@@ -62,13 +62,12 @@ N_CU = 256
 # 128 16-bit lane-ops when packed (two per 32-bit lane)
 VALU_INT32_PEAK_TOPS = N_CU * 64 * CLOCK_GHZ / 1e3
 VALU_INT16_PEAK_TOPS = N_CU * 128 * CLOCK_GHZ / 1e3
-# int32-equivalent operations per DP cell: vertical and horizontal gap terms
-# (add + max each, and the add of the open penalty), diagonal (substitution
-# choice + add), best of three (2 max), match count (2 choices + add + the
-# compare that picks the winning term); the score pass of a two-pass DP
-# stage computes no match counts: 10
+# int32 operations per DP cell: vertical and horizontal gap terms (2 adds +
+# max each), diagonal (add of the substitution score), max(v, h), max with
+# the diagonal, and for the match count 2 compares, 2 selects and an add;
+# the score pass of a two-pass DP stage computes no match counts: 9
 OPS_PER_CELL = 14
-OPS_PER_CELL_SCORE = 10
+OPS_PER_CELL_SCORE = 9
 METRIC = "query Mbp/s aligned (bit-exact hits) + p50/p99 request latency, 1 MI355X"
 
 
@@ -76,16 +75,21 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(name: str, kernel: str):
+def measured_traffic(name: str, kernel: str, cells_per_launch: float = 0.0):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE scaled by a
-    plain device copy's counters) from profiles/<name>_hbm_traffic.txt."""
+    plain device copy's counters) from profiles/<name>_hbm_traffic.txt: a
+    per-launch figure, or a per-DP-cell one times this run's cells per
+    launch (serving launches differ in size from run to run)."""
     path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.txt")
     try:
         with open(path) as f:
             for line in f:
                 part = line.split()
-                if len(part) == 3 and part[0] == "traffic_bytes_per_launch" and part[1] == kernel:
-                    return float(part[2])
+                if len(part) == 3 and part[1] == kernel:
+                    if part[0] == "traffic_bytes_per_launch":
+                        return float(part[2])
+                    if part[0] == "traffic_bytes_per_cell" and cells_per_launch > 0:
+                        return round(float(part[2]) * cells_per_launch)
     except OSError:
         pass
     return None
@@ -184,6 +188,8 @@ def kernel_roofline(records, name: str) -> tuple[dict, dict]:
     # past-the-end k_dp16 launch, k_dp and k_dp_band run beside k_dp16 on a
     # second stream)
     ops16 = np.where(dpk[:, 11] > 0, OPS_PER_CELL_SCORE, OPS_PER_CELL)
+    cells = {"k_dp16<8>": dpk[:, 0], "k_dp16<8,edge>": dpk[:, 1], "dp_second_pass": dpk[:, 9],
+             "k_dp<8,false>": dpk[:, 2], "k_dp_band<8,false>": dpk[:, 3]}
     kernels = {
         # name: (ms, work, unit, peak, bound)
         "k_seed_count": (kt[:, 0], 16.0 * pos, "GB/s", HBM_PEAK_GBS, "hbm"),
@@ -214,7 +220,9 @@ def kernel_roofline(records, name: str) -> tuple[dict, dict]:
     roof = {"kernel": dom, "bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": unit,
             "frac": round(achieved / peak, 4), "launches": int(launched.sum()),
             "avg_launch_us": per[dom]["avg_us"],
-            "traffic": measured_traffic(name, dom.split("<")[0]),
+            "traffic": measured_traffic(name, dom.split("<")[0],
+                                        float(cells[dom][launched].sum()) / max(int(launched.sum()), 1)
+                                        if dom in cells else 0.0),
             "work_per_unit": ("16 B per query k-mer position" if unit == "GB/s" else
                               f"{OPS_PER_CELL} int32-equivalent ops per DP cell (band width x rows), "
                               f"{OPS_PER_CELL_SCORE} in the score pass of a two-pass DP stage"),
@@ -503,6 +511,9 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-capacity", action="store_true")
+    ap.add_argument("--dp-log", default="", metavar="PATH",
+                    help="write every serving search's DP kernel figures (clast_hits_dp_kernels), in search "
+                         "order, one JSON list a line (for matching a profile's dispatches; use --workers 1)")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving headline (only the other legs, e.g. to profile them alone)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -552,12 +563,19 @@ def main() -> int:
     mbp = sum(r.n_bases for r in reqs) / 1e6
     log(f"{n_warm} warmup + {n_req} timed requests ({mbp:.0f} Mbp timed) generated in "
         f"{time.perf_counter() - t2:.1f} s")
+    dp_log = []
     if n_warm:
         srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers, part_positions=args.part_positions)
-        ws = srv.run(warm, realtime=True)
+        ws = srv.run(warm, realtime=True, record=bool(args.dp_log))
+        dp_log += [list(map(float, r[4])) for r in srv.records]
         log(f"warmup: {ws['requests']} requests, {ws['mbp_per_s']} Mbp/s, p99 {ws['p99_ms']} ms")
     srv = serve.Server(p, ix, q_vram=configs.Q_VRAM, workers=args.workers, part_positions=args.part_positions)
     stats = srv.run(reqs, realtime=True, record=True)
+    if args.dp_log:
+        dp_log += [list(map(float, r[4])) for r in srv.records]
+        with open(args.dp_log, "w") as f:
+            for row in dp_log:
+                f.write(json.dumps(row) + "\n")
     log(f"timed: {stats}")
     wall = stats["wall_s"]
     roof, per_kernel = kernel_roofline(srv.records, "serving")
