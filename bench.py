@@ -486,6 +486,45 @@ def uneven_leg(n_chunks: int, streams: int) -> dict:
             "timed_span": "reads in host memory -> rows in host memory, target index resident in both"}
 
 
+# ---------------------------------------------------------------- random-gather calibration
+
+def calibrate_gather(n: int, reps: int = 10) -> dict:
+    """The seed lookup's random loads measured apart from it: n uniform
+    random 8-byte reads (torch's index_select over a 1-D int64 array, one
+    dispatch each) from arrays of 128 MiB (the presence bitmap's size,
+    resident in the 256 MB Infinity Cache), 2 GiB and 16 GiB (the direct
+    cell table's size: its pages outrun the address-translation caches).
+    Each dispatch also streams its index array (8 B per access) and writes
+    its output (8 B per access).  Reported per array: the fastest and the
+    mean of `reps` hipEvent-timed dispatches and random reads per second."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    out = {"accesses_per_dispatch": n}
+    for name, log2_words in (("gather_128MiB", 24), ("gather_2GiB", 28), ("gather_16GiB", 31)):
+        src = torch.zeros(1 << log2_words, dtype=torch.int64, device=dev)
+        idx = torch.randint(0, 1 << log2_words, (n,), device=dev, generator=gen)
+        for _ in range(2):
+            torch.index_select(src, 0, idx)
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            torch.index_select(src, 0, idx)
+            e1.record()
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        best = min(ms)
+        out[name] = {"best_ms": round(best, 4), "mean_ms": round(float(np.mean(ms)), 4),
+                     "G_reads_per_s": round(n / (best / 1e3) / 1e9, 2)}
+        log(f"{name}: {out[name]}")
+        del src, idx
+        torch.cuda.empty_cache()
+    return out
+
+
 # ---------------------------------------------------------------- main
 
 def main() -> int:
@@ -519,6 +558,9 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only copy MiB of device memory once (the known-size kernel that scales the HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="N",
+                    help="only time N random 8-byte reads from arrays of 128 MiB, 2 GiB and 16 GiB "
+                         "(the seed lookup's random loads) and exit")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -528,6 +570,9 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copied {args.calibrate_copy} MiB")
+        return 0
+    if args.calibrate_gather:
+        print(json.dumps(calibrate_gather(args.calibrate_gather)), flush=True)
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: there is no multi-GPU path (the search would shard by query "
