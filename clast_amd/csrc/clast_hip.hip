@@ -733,9 +733,10 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   const uint64_t nunits = static_cast<uint64_t>(ix.nwin) * q.nwin;
   // [2, 10): DP cell counters, [10, 18): rows of 16-bit DP tasks, [18]: their
   // count, [19]: long task count, [20, 28): their rows, [28, 36): all DP rows,
-  // [36, 44): rows of the second DP pass's tasks
-  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 44);
-  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 44, 0ull);
+  // [36, 44): rows of the second DP pass's tasks, [44, 52): the same rows
+  // summed before they are listed (the second pass's long-task threshold)
+  uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 52);
+  k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 52, 0ull);
   // [8, 11): task counts of the second DP pass's three lists
   uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 12);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
@@ -1219,8 +1220,17 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       order2 = ws.get<uint32_t>("p2.order", sp.nslot);
       split2 = ws.get<uint32_t>("p2.split", 3);
       const uint32_t gs2 = grid_for(sp.nslot, kBlock);
+      // long tasks to k_dp_band only while they would set the pass's end by
+      // themselves (over twice one SIMD's share of the pass's rows): many
+      // long survivors run faster in k_dp16
+      auto* rsum2 = reinterpret_cast<unsigned long long*>(tot64 + 44);
+      int* band_rows2 = ws.get<int>("p2.bandrows", 1);
+      k_redo_rows<<<gs2, kBlock, 0, st>>>(sp.da.order, sp.da.split, kflag, hs, hq, q.qlen.ptr, p.lmer, sp.nslot,
+                                          rsum2);
+      k_band_threshold<<<1, 64, 0, st>>>(rsum2, 4ull * static_cast<uint64_t>(std::max(rt.compute_units(), 1)) * 128ull,
+                                         band_rows2);
       k_redo_flags<<<gs2, kBlock, 0, st>>>(sp.da.order, sp.da.split, kflag, hs, hq, q.qlen.ptr, p.lmer, sp.nslot,
-                                           rfl, reinterpret_cast<unsigned long long*>(tot64 + 36));
+                                           band_rows2, rfl, reinterpret_cast<unsigned long long*>(tot64 + 36));
       for (int c = 0; c < 3; ++c)
         device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, rfl + c * static_cast<uint64_t>(sp.nslot), sp.nslot,
                                                          rof + c * static_cast<uint64_t>(sp.nslot), tot32 + 8 + c,

@@ -2514,20 +2514,51 @@ static __global__ void k_count_below(const uint64_t* __restrict__ keys, uint32_t
 // k_dp16's score pass: scores and end nodes, which is all the bad-hit
 // filters read.  The second pass runs the 16-bit tasks of the hits that
 // passed (kflag) again with match counts, in three lists that keep the
-// task order: tasks of at least kBandRows rows (k_dp_band, for latency),
-// k_dp16's others, and its past-the-end others.  flags holds one 0/1 array
-// of nslot entries per list; the rows of the listed tasks are summed into
-// rows_sum[0..8).
+// task order: tasks of at least *band_rows rows (k_dp_band, for latency;
+// the threshold comes from the listed tasks' rows, k_redo_rows and
+// k_band_threshold, as the first pass's long tasks do), k_dp16's others,
+// and its past-the-end others.  flags holds one 0/1 array of nslot entries
+// per list; the rows of the listed tasks are summed into rows_sum[0..8).
+
+// Rows of the second pass's tasks (the 16-bit tasks of the hits that passed
+// the filters), summed into 8 counters (by workgroup).
+static __global__ __launch_bounds__(kBlock) void k_redo_rows(const uint32_t* __restrict__ order,
+                                                             const uint32_t* __restrict__ split,
+                                                             const uint32_t* __restrict__ kflag,
+                                                             const int32_t* __restrict__ hs,
+                                                             const int32_t* __restrict__ hq,
+                                                             const int32_t* __restrict__ qlen, int lmer,
+                                                             uint32_t nslot, unsigned long long* __restrict__ sums) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  int rows = 0;
+  if (j < nslot && j >= split[0] && j < split[2]) {
+    const uint32_t task = order[j];
+    const uint32_t i = task >> 1;
+    if (kflag[i]) rows = (task & 1u) == 0u ? qlen[hs[i]] - (hq[i] + lmer) : hq[i];
+  }
+  const uint32_t r = wave_sum_incl(static_cast<uint32_t>(max(rows, 0)));
+  if (lane_id() == 63) wsum[wave_id()] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) tot += wsum[w];
+    if (tot) atomicAdd(&sums[blockIdx.x & 7u], tot);
+  }
+}
+
 static __global__ __launch_bounds__(kBlock) void k_redo_flags(const uint32_t* __restrict__ order,
                                                               const uint32_t* __restrict__ split,
                                                               const uint32_t* __restrict__ kflag,
                                                               const int32_t* __restrict__ hs,
                                                               const int32_t* __restrict__ hq,
                                                               const int32_t* __restrict__ qlen, int lmer,
-                                                              uint32_t nslot, uint32_t* __restrict__ flags,
+                                                              uint32_t nslot, const int* __restrict__ band_rows,
+                                                              uint32_t* __restrict__ flags,
                                                               unsigned long long* __restrict__ rows_sum) {
   __shared__ unsigned long long wsum[kBlock / 64];
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  const int brows = *band_rows;
   uint32_t cat = 0;
   int rows = 0;
   if (j < nslot && j >= split[0] && j < split[2]) {
@@ -2535,7 +2566,7 @@ static __global__ __launch_bounds__(kBlock) void k_redo_flags(const uint32_t* __
     const uint32_t i = task >> 1;
     if (kflag[i]) {
       rows = (task & 1u) == 0u ? qlen[hs[i]] - (hq[i] + lmer) : hq[i];
-      cat = rows >= kBandRows ? 1u : j < split[1] ? 2u : 3u;
+      cat = rows >= brows ? 1u : j < split[1] ? 2u : 3u;
     }
   }
   if (j < nslot) {
