@@ -738,7 +738,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   uint64_t* tot64 = ws.get<uint64_t>("s.tot64", 52);
   k_fill<uint64_t><<<1, 64, 0, st>>>(tot64, 52, 0ull);
   // [8, 11): task counts of the second DP pass's three lists
-  uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 12);
+  uint32_t* tot32 = ws.get<uint32_t>("s.tot32", 16);
   uint32_t* unit_first = ws.get<uint32_t>("s.unit", nunits);
   k_fill<uint32_t><<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, nunits, 0xFFFFFFFFu);
 
@@ -763,7 +763,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
   a.local = p.local;
   a.unit_first = unit_first;
 
-  // lookup + count: raw seed count per strand, strands' seed ranges
+  // lookup + count: raw seed count per strand, strands' seed ranges (the
+  // seed total sizes the seed and hit arrays: the search's first read-back)
   uint32_t nseed = 0;
   uint4* cellp = nullptr;
   uint32_t* scnt = ws.get<uint32_t>("s.scnt", q.nseq);
@@ -800,8 +801,12 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* hbase = ws.get<uint32_t>("h.base", q.nseq);
     uint32_t* ovf_list = ws.get<uint32_t>("s.ovf", q.nseq);
     uint8_t* ovf_flag = ws.get<uint8_t>("s.ovfflag", q.nseq);
+    uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", q.nseq);
+    uint32_t* off2 = ws.get<uint32_t>("g.off", q.nseq);
+    uint32_t* fmark = ws.get<uint32_t>("s.fmark", q.nseq);
+    uint32_t* flist = ws.get<uint32_t>("s.flist", nunits);
     uint32_t novf = 0;
-    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 12, 0u);
+    k_fill<uint32_t><<<1, 64, 0, st>>>(tot32, 16, 0u);
     const int qbits = bits_for(static_cast<uint64_t>(q.max_qlen));
     const int wbits = bits_for(static_cast<uint64_t>(ix.nwin));
     const bool packed = wbits + gbits + dbits + qbits <= 63;
@@ -820,6 +825,17 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       }
       rt.check_launch("k_seed_segments");
     };
+    // unit_first is final: run each unit's first strand again with its
+    // element 0 (gate: nothing while *gate, i.e. before the global path)
+    auto rerun = [&](const uint32_t* gate) {
+      k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(fmark, q.nseq, 0u);
+      k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 5, 1, 0u);
+      k_fix_list<<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, static_cast<uint32_t>(nunits), ovf_flag,
+                                                              fmark, flist, tot32 + 5, gate);
+      segments(false, static_cast<uint32_t>(nunits), flist, tot32 + 5, kSegRerun);
+    };
+    uint64_t t2 = 0;
+    bool hits_known = false;
     mark(EV_WRITE0);
     if (!keep_stages) {
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 0);
@@ -837,30 +853,42 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         rt.check_launch("k_unit_first");
         a.wmask = nullptr;  // the later passes (global path, rerun) read unit_first
       }
-      rt.read_scalars(tot32 + 1, &novf, 1);
+      // with no strand on the global path (the usual case) unit_first is
+      // final now: the re-run and the hit count go ahead of the read-back,
+      // and a nonzero novf (tot32[1]) stops the re-run's list
+      rerun(tot32 + 1);
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
+      mark(EV_WRITE1);
+      // the global path's seeds per strand (zero without such strands)
+      k_ovf_strand_counts<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, scnt, q.nseq, cnt2);
+      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, q.nseq, off2, tot64 + 1, false);
+      // one read-back: the global path's seeds and strands, and the hits
+      uint32_t r32[4];
+      rt.read_regions({{tot64 + 1, &t2, sizeof t2}, {tot32, r32, sizeof r32}});
+      novf = r32[1];
+      if (novf == 0) {
+        nhit = r32[3];
+        hits_known = true;
+      }
     } else {
       // every strand on the global path: list them all
       k_iota_u32<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_list, q.nseq);
       k_fill<uint8_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, q.nseq, 1);
       k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(hcnt, q.nseq, 0u);
       novf = q.nseq;
+      k_ovf_strand_counts<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, scnt, q.nseq, cnt2);
+      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, q.nseq, off2, tot64 + 1, false);
+      rt.read_scalars(tot64 + 1, &t2, 1);
     }
     SeedPack pk{gbits, dbits, d_off};
     SeedDecode dc{gbits, dbits, d_off};
     uint32_t ng = 0;
-    uint32_t* off2 = nullptr;
     uint64_t* skey = nullptr;
     uint32_t* sval = nullptr;
     if (novf > 0) {
       if (sbits + gbits + dbits > 64) throw ClastError(CLAST_ERR_LIMIT, "seed key over 64 bits: split the query set");
       // the listed strands' seeds in strand order (off2[s]: strand s's first
       // seed, before and after the sort, which keys on the strand first)
-      uint32_t* cnt2 = ws.get<uint32_t>("g.cnt", q.nseq);
-      off2 = ws.get<uint32_t>("g.off", q.nseq);
-      k_ovf_strand_counts<<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(ovf_flag, scnt, q.nseq, cnt2);
-      device_scan<OpAdd, uint32_t, uint64_t, uint32_t>(rt, cnt2, q.nseq, off2, tot64 + 1, false);
-      uint64_t t2 = 0;
-      rt.read_scalars(tot64 + 1, &t2, 1);
       if (t2 >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "over 2^32 seeds: split the query set");
       ng = static_cast<uint32_t>(t2);
       skey = ws.get<uint64_t>("g.key", std::max<uint32_t>(ng, 1));
@@ -869,18 +897,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       k_seed_write<<<std::min<uint32_t>(novf, 1u << 16), kWriteBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2,
                                                                                skey, sval);
       rt.check_launch("k_seed_write");
-    }
-    if (!keep_stages) {
-      // unit_first is final: run each unit's first strand again with its element 0
-      uint32_t* fmark = ws.get<uint32_t>("s.fmark", q.nseq);
-      uint32_t* flist = ws.get<uint32_t>("s.flist", nunits);
-      k_fill<uint32_t><<<grid_for(q.nseq, kBlock), kBlock, 0, st>>>(fmark, q.nseq, 0u);
-      k_fix_list<<<grid_for(nunits, kBlock), kBlock, 0, st>>>(unit_first, static_cast<uint32_t>(nunits), ovf_flag,
-                                                              fmark, flist, tot32 + 5);
-      segments(false, static_cast<uint32_t>(nunits), flist, tot32 + 5, kSegRerun);
-    }
-    mark(EV_WRITE1);
-    if (novf > 0) {
+      if (!keep_stages) rerun(nullptr);
+      mark(EV_WRITE1);
       auto dump_seeds = [&](int slot) {
         std::vector<uint64_t> hk;
         std::vector<uint32_t> hv;
@@ -914,6 +932,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         mark(EV_SORT1);
       }
       if (keep_stages) dump_seeds(CLAST_STAGE_SORTED);
+      k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 2, 1, 0u);
       if (ng > 0) {
         mark(EV_FILT0);
         uint32_t* kidx = ws.get<uint32_t>("g.kidx", ng);
@@ -933,17 +952,17 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
         device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, ng, hoff, tot32 + 2, false);
         k_hits_scatter<<<g, kBlock, 0, st>>>(skey, sval, ng, dc, dflag, hoff, soff, off2, ss, sgs, stt, sq);
         rt.check_launch("k_hits_scatter");
-        // per-strand hit counts of the listed strands (nh2 read on device)
-        uint32_t nh2 = 0;
-        rt.read_scalars(tot32 + 2, &nh2, 1);
-        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, ng, hoff, nh2, hcnt);
+        // per-strand hit counts of the listed strands (their total read on device)
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, ng, hoff, tot32 + 2, hcnt);
         mark(EV_FILT1);
       } else {
-        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, 0, hcnt, 0, hcnt);
+        k_ovf_hcnt<<<grid_for(novf, kBlock), kBlock, 0, st>>>(ovf_list, novf, q.nseq, off2, 0, hcnt, tot32 + 2, hcnt);
       }
     }
-    device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
-    rt.read_scalars(tot32 + 3, &nhit, 1);
+    if (!hits_known) {
+      device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, hcnt, q.nseq, hbase, tot32 + 3, false);
+      rt.read_scalars(tot32 + 3, &nhit, 1);
+    }
     if (nhit)
       k_hits_compact<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(nhit, q.nseq, soff, hbase, ss, sgs, stt, sq, hs, hgs,
                                                                  ht, hq);
@@ -1204,6 +1223,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     uint32_t* gflag = ws.get<uint32_t>("r.gflag", nhit);
     uint32_t* doff = ws.get<uint32_t>("r.doff", nhit);
     uint32_t* hbase = ws.get<uint32_t>("h.base", q.nseq);
+    order = ws.get<uint32_t>("r.order", nhit);
     const uint32_t g = grid_for(nhit, kBlock);
     k_hit_filter<<<g, kBlock, 0, st>>>(nhit, hs, hgs, ht, hq, osc, qhl, q.qlen.ptr, ix.seq_win.ptr, q.smin.ptr,
                                        use_cut ? 1 : 0, p.lmer + p.stride, p.K * p.total_db_size, p.lambda, hev,
@@ -1242,7 +1262,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     k_fill<uint32_t><<<1, 64, 0, st>>>(tot32 + 4, 1, 0u);
     k_pair_dedup<<<g, kBlock, 0, st>>>(pa, dflag, gflag, tot32 + 4);
     rt.check_launch("k_pair_dedup");
-    // tot32[4] and the second pass's list counts, tot32[8, 11)
+    // tot32[4]: rows of pairs over kPairSmall hits, and the second pass's
+    // list sizes, tot32[8, 11)
     uint32_t rb[7];
     rt.read_scalars(tot32 + 4, rb, 7);
     const uint32_t nbig = rb[0];
@@ -1341,24 +1362,21 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       }
       }
     }
+    // the row count stays on the device (tot32[3]); order has room for every hit
     device_scan<OpAdd, uint32_t, uint32_t, uint32_t>(rt, dflag, nhit, doff, tot32 + 3, false);
-    rt.read_scalars(tot32 + 3, &nfinal, 1);
-    if (nfinal) {
-      order = ws.get<uint32_t>("r.order", nfinal);
-      k_pair_place<<<g, kBlock, 0, st>>>(pa, dflag, doff, order);
-      if (ngrow)
-        k_big_place<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hbase, gfirst, doff, order);
-      rt.check_launch("k_pair_place");
-    }
+    k_pair_place<<<g, kBlock, 0, st>>>(pa, dflag, doff, order);
+    if (ngrow)
+      k_big_place<<<grid_for(ngrow, kBlock), kBlock, 0, st>>>(ngrow, grow, hs, hbase, gfirst, doff, order);
+    rt.check_launch("k_pair_place");
   }
   if (p2_pending) rt.join(1);
-  // gather, then the copy to pinned host memory on the copy stream (the
-  // caller's reads wait for it; the next search's kernels do not)
+  // the handle's earlier rows (if any) are released
   out.wait();
   clast::pinned_pool().release(out.rows);
   out.rows = {};
   out.n = 0;
   if (rows_out) {
+    if (nhit) rt.read_scalars(tot32 + 3, &nfinal, 1);
     rows_out->needed = nfinal;
     if (nfinal > rows_out->capacity)
       throw ClastError(CLAST_ERR_LIMIT, "result rows exceed the caller's capacity (" + std::to_string(nfinal) +
@@ -1368,7 +1386,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       int32_t* o32 = reinterpret_cast<int32_t*>(ob);
       double* oev = reinterpret_cast<double*>(ob + static_cast<uint64_t>(nfinal) * 40);
       OutArgs oa{nfinal, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
-                 ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
+                 ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev, nullptr, nullptr};
       k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
       rt.check_launch("k_gather_out");
       for (int c = 0; c < 10; ++c)
@@ -1379,35 +1397,43 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       if (rows_out->evalue)
         hip_check(hipMemcpyAsync(rows_out->evalue, oev, 8ull * nfinal, hipMemcpyDeviceToHost, st), "D2H rows");
     }
-  } else if (nfinal) {
-    const int slot = rt.next_out_slot();
+  }
+  uint8_t* ob = nullptr;  // staged rows (planes laid out for the row count the gather reads)
+  int slot = 0;
+  if (!rows_out && nhit) {
+    // the gather reads the row count on the device, so it runs before the
+    // read-back below: the search's kernels are done when it returns
+    slot = rt.next_out_slot();
     const std::string tag = slot ? "o1" : "o0";
     // staging buffer `slot` is free once the copy that last read it is done
     if (rt.out_used(slot)) hip_check(hipStreamWaitEvent(st, rt.out_done(slot), 0), "hipStreamWaitEvent");
-    uint8_t* ob = ws.get<uint8_t>(tag, static_cast<uint64_t>(nfinal) * 48);
-    int32_t* o32 = reinterpret_cast<int32_t*>(ob);
-    double* oev = reinterpret_cast<double*>(ob + static_cast<uint64_t>(nfinal) * 40);
-    OutArgs oa{nfinal, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
-               ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, o32, oev};
-    k_gather_out<<<grid_for(nfinal, kBlock), kBlock, 0, st>>>(oa);
+    ob = ws.get<uint8_t>(tag, static_cast<uint64_t>(nhit) * 48);
+    OutArgs oa{0, order, nullptr, hs, hgs, ot, oq, thl, qhl, om, osc, hev,
+               ix.seq_ref.ptr, q.qref.ptr, ix.seq_win.ptr, q.qwin.ptr, nullptr, nullptr, tot32 + 3, ob};
+    k_gather_out<<<grid_for(nhit, kBlock), kBlock, 0, st>>>(oa);
     rt.check_launch("k_gather_out");
-    out.rows = clast::pinned_pool().acquire(static_cast<uint64_t>(nfinal) * 48);
-    out.n = nfinal;
-    hipEvent_t gathered;
-    hip_check(hipEventCreateWithFlags(&gathered, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipEventRecord(gathered, st), "hipEventRecord");
-    hipStream_t cs = rt.copy_stream();
-    hip_check(hipStreamWaitEvent(cs, gathered, 0), "hipStreamWaitEvent");
-    (void)hipEventDestroy(gathered);  // released once the wait is satisfied
-    hip_check(hipMemcpyAsync(out.rows.ptr, ob, static_cast<uint64_t>(nfinal) * 48, hipMemcpyDeviceToHost, cs),
-              "D2H rows");
-    hip_check(hipEventRecord(rt.out_done(slot), cs), "hipEventRecord");
-    rt.set_out_used(slot);
-    hip_check(hipEventCreateWithFlags(&out.ready, hipEventDisableTiming), "hipEventCreate");
-    hip_check(hipEventRecord(out.ready, cs), "hipEventRecord");
   }
   mark(EV_RES1);
-  rt.sync();
+  // the search's last wait: row count and work counters in one read-back
+  uint64_t cnt[42];
+  uint32_t r32[16] = {};
+  rt.read_regions({{tot64 + 2, cnt, sizeof cnt}, {tot32, r32, sizeof r32}});
+  if (!rows_out && nhit) {
+    nfinal = r32[3];
+    if (nfinal) {
+      // the copy to pinned host memory on the copy stream (the caller's
+      // reads wait for it; the next search's kernels do not)
+      out.rows = clast::pinned_pool().acquire(static_cast<uint64_t>(nfinal) * 48);
+      out.n = nfinal;
+      hipStream_t cs = rt.copy_stream();
+      hip_check(hipMemcpyAsync(out.rows.ptr, ob, static_cast<uint64_t>(nfinal) * 48, hipMemcpyDeviceToHost, cs),
+                "D2H rows");
+      hip_check(hipEventRecord(rt.out_done(slot), cs), "hipEventRecord");
+      rt.set_out_used(slot);
+      hip_check(hipEventCreateWithFlags(&out.ready, hipEventDisableTiming), "hipEventCreate");
+      hip_check(hipEventRecord(out.ready, cs), "hipEventRecord");
+    }
+  }
   const int pairs[7][2] = {{EV_COUNT0, EV_COUNT1}, {EV_WRITE0, EV_WRITE1}, {EV_SORT0, EV_SORT1}, {EV_FILT0, EV_FILT1},
                            {EV_DP0, EV_DP1},       {EV_RES0, EV_RES1},     {EV_START, EV_RES1}};
   for (int k = 0; k < 7; ++k) {
@@ -1426,8 +1452,6 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       hip_check(hipEventElapsedTime(&t, ev->e[kpairs[k][0]], ev->e[kpairs[k][1]]), "elapsed");
     out.ktimes[k] = t;
   }
-  uint64_t cnt[42];
-  rt.read_scalars(tot64 + 2, cnt, 42);
   uint64_t cells = 0, rows16 = 0, rowsband = 0;
   uint64_t rows_main = 0;
   for (int k = 0; k < 8; ++k) {

@@ -1,16 +1,20 @@
 // Instantiations of the lane-per-band DP kernel for long extensions (both
 // modes, gap 2..16).
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace clast {
 
 template <int G, bool LOCAL>
 static void launch(const DPArgs& a, hipStream_t st) {
-  // an upper bound on the pairs (the band task count is on the device):
-  // workgroups past the last pair find no pair and skip the steps
+  // a.n bounds the tasks (the band task count is on the device):
+  // workgroups past the last pair exit at once; at most kMaxBlocks
+  // workgroups, which stride over the pairs
   constexpr uint32_t kPairsPerBlock = (kBlock / 64) * (64 / (2 * G + 1));
+  constexpr uint64_t kMaxBlocks = 4096;
   const uint64_t pairs = (static_cast<uint64_t>(a.n) + 1) / 2;
-  const uint32_t blocks = static_cast<uint32_t>((pairs + kPairsPerBlock - 1) / kPairsPerBlock);
+  const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>((pairs + kPairsPerBlock - 1) / kPairsPerBlock, kMaxBlocks));
   if (blocks) k_dp_band<G, LOCAL><<<blocks, kBlock, 0, st>>>(a);
 }
 

@@ -1079,12 +1079,15 @@ static __global__ __launch_bounds__(kBlock) void k_seed_segments(SeedArgs a, con
 
 // Strands to run again once unit_first is final: each unit's first strand,
 // unless it took the global path (whose filters read unit_first directly).
+// With `gate`, nothing is listed while *gate is nonzero (strands on the
+// global path: unit_first is final only after their seeds are written).
 static __global__ __launch_bounds__(kBlock) void k_fix_list(const uint32_t* __restrict__ unit_first, uint32_t nunits,
                                                             const uint8_t* __restrict__ ovf_flag,
                                                             uint32_t* __restrict__ mark, uint32_t* __restrict__ list,
-                                                            uint32_t* __restrict__ nlist) {
+                                                            uint32_t* __restrict__ nlist,
+                                                            const uint32_t* __restrict__ gate) {
   const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
-  if (u >= nunits) return;
+  if (u >= nunits || (gate && *gate)) return;
   const uint32_t s = unit_first[u];
   if (s == 0xFFFFFFFFu || ovf_flag[s]) return;
   if (atomicExch(&mark[s], 1u) == 0u) list[atomicAdd(nlist, 1u)] = s;
@@ -1285,10 +1288,12 @@ static __global__ __launch_bounds__(kBlock) void k_hits_scatter(const uint64_t* 
 // strand's.
 static __global__ __launch_bounds__(kBlock) void k_ovf_hcnt(const uint32_t* __restrict__ ovf, uint32_t novf,
                                                             uint32_t nseq, const uint32_t* __restrict__ goff,
-                                                            uint32_t ng, const uint32_t* __restrict__ hoff, uint32_t nh,
+                                                            uint32_t ng, const uint32_t* __restrict__ hoff,
+                                                            const uint32_t* __restrict__ nh_dev,
                                                             uint32_t* __restrict__ hcnt) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= novf) return;
+  const uint32_t nh = *nh_dev;
   const uint32_t s = ovf[k];
   const uint32_t g0 = goff[s];
   const uint32_t g1 = s + 1u < nseq ? goff[s + 1u] : ng;
@@ -2202,7 +2207,11 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
   const int grp = lane / W;
   const int b = lane - grp * W;
   const uint32_t nband = a.split[0];
-  const uint32_t pair = (blockIdx.x * (kBlock / 64) + wave_id()) * kPairs + static_cast<uint32_t>(grp);
+  // workgroups stride over the task pairs (the grid is an upper bound, or
+  // capped); the loop bound is the same for every wavefront of a workgroup
+  constexpr uint32_t kBlockPairs = (kBlock / 64) * kPairs;
+  for (uint32_t blk = blockIdx.x; blk * kBlockPairs < (nband + 1) / 2; blk += gridDim.x) {
+  const uint32_t pair = (blk * (kBlock / 64) + wave_id()) * kPairs + static_cast<uint32_t>(grp);
   // every lane stays to the workgroup barrier; lanes without a pair run
   // task 0's setup and no steps
   const bool live = grp < kPairs && 2 * pair < nband;
@@ -2319,6 +2328,8 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
       endp = e - G;
     }
     a.dres[k ? taskB : taskA] = make_int4(move + endp, move, bestM, bestS);
+  }
+  __syncthreads();  // red is rewritten by the next round
   }
 }
 
@@ -2777,15 +2788,21 @@ struct OutArgs {
   const int32_t* qwin;
   int32_t* out_i32;   // 10 planes of n: tID tIdx qID qIdx tHL qHL match score twin qwin
   double* out_ev;
+  // with n_dev: n = *n_dev (the grid covers an upper bound), and the planes
+  // and E-values are laid out from out_base as above (10 x 4 n bytes, then
+  // 8 n)
+  const uint32_t* n_dev;
+  uint8_t* out_base;
 };
 
 static __global__ __launch_bounds__(kBlock) void k_gather_out(OutArgs a) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
+  const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+  if (i >= n) return;
   const uint32_t r = a.order[a.perm ? a.perm[i] : i];
   const int gs = a.hgs[r], s = a.hs[r];
-  int32_t* o = a.out_i32;
-  const uint32_t n = a.n;
+  int32_t* o = a.n_dev ? reinterpret_cast<int32_t*>(a.out_base) : a.out_i32;
+  double* oev = a.n_dev ? reinterpret_cast<double*>(a.out_base + 40ull * n) : a.out_ev;
   o[0 * n + i] = a.tref[gs];
   o[1 * n + i] = a.ot[r];
   o[2 * n + i] = a.qref[s];
@@ -2796,7 +2813,7 @@ static __global__ __launch_bounds__(kBlock) void k_gather_out(OutArgs a) {
   o[7 * n + i] = a.score[r];
   o[8 * n + i] = a.twin[gs];
   o[9 * n + i] = a.qwin[s];
-  a.out_ev[i] = a.ev[r];
+  oev[i] = a.ev[r];
 }
 
 // ---------------------------------------------------------------- per-pair results

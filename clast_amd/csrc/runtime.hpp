@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -286,6 +287,28 @@ class Runtime {
     hip_check(hipMemcpyAsync(pinned_, dev, sizeof(T) * count, hipMemcpyDeviceToHost, stream_), "scalar D2H");
     sync();
     for (int i = 0; i < count; ++i) host[i] = static_cast<const T*>(pinned_)[i];
+  }
+  // Several device regions in one wait: {device pointer, host pointer,
+  // bytes} each (at most 4 KiB in all).
+  struct Region {
+    const void* dev;
+    void* host;
+    size_t bytes;
+  };
+  void read_regions(std::initializer_list<Region> regions) {
+    size_t off = 0;
+    for (const Region& r : regions) {
+      if (off + r.bytes > kPinned) throw ClastError(4, "read-back over the staging block");
+      hip_check(hipMemcpyAsync(static_cast<uint8_t*>(pinned_) + off, r.dev, r.bytes, hipMemcpyDeviceToHost, stream_),
+                "scalar D2H");
+      off = (off + r.bytes + 15) & ~size_t{15};
+    }
+    sync();
+    off = 0;
+    for (const Region& r : regions) {
+      std::memcpy(r.host, static_cast<const uint8_t*>(pinned_) + off, r.bytes);
+      off = (off + r.bytes + 15) & ~size_t{15};
+    }
   }
 
  private:
