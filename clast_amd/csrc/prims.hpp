@@ -141,7 +141,10 @@ static __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint64_t* __
 
 // Stable scatter of one tile.  Item k of lane l of wave w is tile element
 // w*64*kSortItems + k*64 + l; within a wave, items are ranked in (k, lane)
-// order from a 8-ballot peer mask, so the sort is stable.
+// order from a 8-ballot peer mask, so the sort is stable.  The tile is
+// first put in digit order in LDS, then written out with consecutive
+// threads on consecutive slots of each digit's run, so that the global
+// writes of a digit coalesce.
 static __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
                                                           uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -149,6 +152,9 @@ static __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t*
                                                           const uint32_t* __restrict__ offsets, uint32_t ntiles) {
   __shared__ uint32_t wcnt[kBlock / 64][kRadixBins];
   __shared__ uint32_t gbase[kRadixBins];
+  __shared__ uint32_t tstart[kRadixBins];
+  __shared__ uint64_t lkey[kSortTile];
+  __shared__ uint32_t lval[kSortTile];
   volatile uint32_t* vw = &wcnt[0][0];
   for (int i = threadIdx.x; i < (kBlock / 64) * kRadixBins; i += kBlock) vw[i] = 0;
   gbase[threadIdx.x] = offsets[threadIdx.x * ntiles + blockIdx.x];
@@ -181,7 +187,8 @@ static __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t*
     dig[k] = d;
   }
   __syncthreads();
-  {  // per-digit offsets of each wave inside the tile
+  {  // per-digit offsets of each wave inside the digit's run, and where each
+     // digit's run starts in the tile
     uint32_t run = 0;
 #pragma unroll
     for (int ww = 0; ww < kBlock / 64; ++ww) {
@@ -189,16 +196,27 @@ static __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint64_t*
       vw[ww * kRadixBins + threadIdx.x] = run;
       run += c;
     }
+    uint32_t total;
+    tstart[threadIdx.x] = block_exclusive<OpAdd>(run, &total);
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
-    const uint32_t i = base + k * 64;
-    if (i < n) {
-      const uint32_t pos = gbase[dig[k]] + vw[w * kRadixBins + dig[k]] + rank[k];
-      kout[pos] = key[k];
-      vout[pos] = val[k];
+    if (base + k * 64 < n) {
+      const uint32_t at = tstart[dig[k]] + vw[w * kRadixBins + dig[k]] + rank[k];
+      lkey[at] = key[k];
+      lval[at] = val[k];
     }
+  }
+  __syncthreads();
+  const uint32_t t0 = blockIdx.x * kSortTile;
+  const uint32_t tn = n - t0 < static_cast<uint32_t>(kSortTile) ? n - t0 : static_cast<uint32_t>(kSortTile);
+  for (uint32_t i = threadIdx.x; i < tn; i += kBlock) {
+    const uint64_t k = lkey[i];
+    const uint32_t d = static_cast<uint32_t>(k >> shift) & mask;
+    const uint32_t pos = gbase[d] + (i - tstart[d]);
+    kout[pos] = k;
+    vout[pos] = lval[i];
   }
 }
 
