@@ -895,7 +895,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       sval = ws.get<uint32_t>("g.val", std::max<uint32_t>(ng, 1));
       // at most 2^16 workgroups, striding over the listed strands
       k_seed_write<<<std::min<uint32_t>(novf, 1u << 16), kWriteBlock, 0, st>>>(a, pk, ovf_list, novf, cellp, off2,
-                                                                               skey, sval);
+                                                                               ix.nwin, skey, sval);
       rt.check_launch("k_seed_write");
       if (!keep_stages) rerun(nullptr);
       mark(EV_WRITE1);

@@ -1134,61 +1134,94 @@ static __global__ __launch_bounds__(kBlock) void k_ovf_strand_counts(const uint8
 // on, in query-position then cell order (krnlWriteSeedList.cu:3-37 order),
 // and records the strand as a unit candidate like k_seed_segments.  The
 // strands here hold thousands of positions: kWriteBlock of them per round,
-// each seed's slot from a workgroup-wide prefix of the cell counts (wave
-// scans, then the wave totals through LDS, double-buffered so that one
-// barrier per round suffices).
+// each position's first slot from a workgroup-wide prefix of the cell
+// counts (wave scans, then the wave totals through LDS).  The round's seeds
+// are then spread evenly over the threads (seed o of the round to thread o
+// mod kWriteBlock, its position found by a binary search of the prefix in
+// LDS), so that a repeat k-mer with hundreds of entries does not hold its
+// wavefront.  With at most 64 target windows the unit candidates go as one
+// window mask per strand (one note per window, not per seed).
 constexpr int kWriteBlock = 1024;
 static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, SeedPack pk,
                                                                    const uint32_t* __restrict__ list, uint32_t nlist,
                                                                    const uint4* __restrict__ cellp,
-                                                                   const uint32_t* __restrict__ off,
+                                                                   const uint32_t* __restrict__ off, int nwin,
                                                                    uint64_t* __restrict__ skey,
                                                                    uint32_t* __restrict__ sval) {
   constexpr int kW = kWriteBlock / 64;
-  __shared__ uint32_t wsum[2][kW];
+  __shared__ uint32_t wsum[kW];
+  __shared__ uint32_t pend[kWriteBlock];   // inclusive prefix of the round's cell counts
+  __shared__ uint32_t plo[kWriteBlock];    // each position's first entry
+  __shared__ int2 pfirst[kWriteBlock];     // and that entry's (sequence, tIdx)
+  __shared__ unsigned long long wmask[kW];
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  // workgroups stride over the listed strands (the grid is capped); the
-  // partial-sum buffers alternate across strands as across chunks, so one
-  // barrier per chunk keeps a buffer from being rewritten while read
-  int buf = 0;
+  const uint64_t none = (1ull << (pk.gs_bits + pk.d_bits)) - 1ull;
+  const bool masks = nwin <= 64;
+  // workgroups stride over the listed strands (the grid is capped)
   for (uint32_t k = blockIdx.x; k < nlist; k += gridDim.x) {
-  const uint32_t s = list[k];
-  const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
-  const int qLen = a.qlen[s];
-  const int qw = a.qwin[s];
-  const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
-  uint32_t base = off[s];
-  for (uint32_t c0 = p0; c0 < p1; c0 += kWriteBlock) {
-    const uint32_t p = c0 + threadIdx.x;
-    const uint4 cp = p < p1 ? cellp[p] : make_uint4(0, 0, 0, 0);
-    const uint32_t incl = wave_sum_incl(cp.y);
-    if (lane == 63) wsum[buf][w] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
+    const uint32_t s = list[k];
+    const uint32_t p0 = a.pos_off[s], p1 = a.pos_off[s + 1];
+    const int qLen = a.qlen[s];
+    const int qw = a.qwin[s];
+    const uint64_t sk = static_cast<uint64_t>(s) << (pk.gs_bits + pk.d_bits);
+    uint32_t base = off[s];
+    uint64_t wm = 0;  // target windows of this thread's kept seeds
+    for (uint32_t c0 = p0; c0 < p1; c0 += kWriteBlock) {
+      const uint32_t p = c0 + threadIdx.x;
+      const uint4 cp = p < p1 ? cellp[p] : make_uint4(0, 0, 0, 0);
+      const uint32_t incl = wave_sum_incl(cp.y);
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      uint32_t before = 0, total = 0;
 #pragma unroll
-    for (int v = 0; v < kW; ++v) {
-      const uint32_t t = wsum[buf][v];
-      before += v < w ? t : 0u;
-      total += t;
-    }
-    const int qIdx = static_cast<int>(p - p0);
-    uint32_t o = base + before + incl - cp.y;
-    for (uint32_t j = 0; j < cp.y; ++j, ++o) {
-      const int2 e = j == 0 ? make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w)) : a.epos[cp.x + j];
-      const int d = e.y - qIdx;
-      const int2 ti = a.tinfo[e.x];
-      if (corner_ok(a, d, qLen, ti.x)) {
-        skey[o] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
-        note_unit_first(a, ti.y, qw, s);
-      } else {
-        skey[o] = sk | ((1ull << (pk.gs_bits + pk.d_bits)) - 1ull);
+      for (int v = 0; v < kW; ++v) {
+        const uint32_t t = wsum[v];
+        before += v < w ? t : 0u;
+        total += t;
       }
-      sval[o] = static_cast<uint32_t>(qIdx);
+      pend[threadIdx.x] = before + incl;
+      plo[threadIdx.x] = cp.x;
+      pfirst[threadIdx.x] = make_int2(static_cast<int>(cp.z), static_cast<int>(cp.w));
+      __syncthreads();
+      for (uint32_t o = threadIdx.x; o < total; o += kWriteBlock) {
+        // the round's position of seed o: the first whose prefix passes o
+        uint32_t lo = 0, hi = kWriteBlock - 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (pend[mid] > o) hi = mid;
+          else lo = mid + 1;
+        }
+        const uint32_t j = o - (lo ? pend[lo - 1] : 0u);
+        const int qIdx = static_cast<int>(c0 + lo - p0);
+        const int2 e = j == 0 ? pfirst[lo] : a.epos[plo[lo] + j];
+        const int d = e.y - qIdx;
+        const int2 ti = a.tinfo[e.x];
+        const uint32_t at = base + o;
+        if (corner_ok(a, d, qLen, ti.x)) {
+          skey[at] = sk | (static_cast<uint64_t>(e.x) << pk.d_bits) | static_cast<uint64_t>(d + pk.d_off);
+          if (masks) wm |= 1ull << ti.y;
+          else note_unit_first(a, ti.y, qw, s);
+        } else {
+          skey[at] = sk | none;
+        }
+        sval[at] = static_cast<uint32_t>(qIdx);
+      }
+      base += total;
+      __syncthreads();  // the round's LDS arrays are rewritten by the next
     }
-    base += total;
-    buf ^= 1;
-  }
+    if (masks) {
+      const uint64_t m = wave_or(wm);
+      if (lane == 0) wmask[w] = m;
+      __syncthreads();
+      if (threadIdx.x < 64) {
+        uint64_t all = 0;
+#pragma unroll
+        for (int v = 0; v < kW; ++v) all |= wmask[v];
+        if ((all >> threadIdx.x) & 1ull) note_unit_first(a, static_cast<int>(threadIdx.x), qw, s);
+      }
+      __syncthreads();  // wmask is rewritten by the next strand
+    }
   }
 }
 
