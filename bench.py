@@ -533,7 +533,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20, help="timed seconds of arrivals (rate requests each)")
     ap.add_argument("--warmup", type=int, default=3, help="warmup seconds of arrivals")
     ap.add_argument("--rate", type=float, default=50.0, help="requests per second")
-    ap.add_argument("--workers", type=int, default=2, help="device threads draining the batch queue")
+    ap.add_argument("--workers", type=int, default=3, help="device threads draining the batch queue")
     ap.add_argument("--part-positions", type=int, default=16_000_000,
                     help="requests are cut into parts of at most this many query k-mer positions")
     ap.add_argument("--capacity-requests", type=int, default=300)
