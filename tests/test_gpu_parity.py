@@ -266,6 +266,42 @@ def test_long_reads_mix_fast_and_global_paths():
         assert_rows_equal(got.arrays(), o["final"], what=f"long local={local}")
 
 
+@pytest.mark.parametrize("n_t", [40, 70])
+def test_global_path_many_windows(n_t):
+    # Strands over 2048 seeds take the global path, whose seed writer spreads
+    # each round's seeds over its workgroup and records unit candidates as
+    # one window mask per strand (40 target windows) or one note per seed
+    # (70: over 64 windows).  Target windows of one sequence each with 30%
+    # repeats (cells of many entries), long reads at stride 2 next to short
+    # ones, and query windows of a few reads, so that global-path strands are
+    # the first strands of some units.
+    t = gen.make_target(gen.TargetSpec(41, n_t, 12_000, n_rate=0.001, repeat_frac=0.3, n_families=6,
+                                       fam_min=300, fam_max=1500, div_min=0.0, div_max=0.05))
+    long_r = gen.make_reads(gen.ReadSpec(42, 14, len_mode=gen.LEN_LOGUNIFORM, len_a=5000, len_b=9000,
+                                         sub_rate=0.01, indel_rate=0.002, repeat_read_frac=0.3), t)
+    short_r = gen.make_reads(gen.ReadSpec(43, 60, sub_rate=0.03, indel_rate=0.01), t)
+    bases = np.concatenate([long_r.bases, short_r.bases])
+    off = np.concatenate([long_r.offsets, long_r.offsets[-1] + short_r.offsets[1:]]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t), stride=2)
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 12_100, query=False)
+    qwins = native.windows(ql.gateway, 20_000, query=True)
+    assert len(twins) == n_t and len(qwins) > 4
+    print(f"global path: {len(twins)}x{len(qwins)} units", flush=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    got = native.search_batch(p, ix, q)
+    assert got.times()[2] > 0  # the global path's seed sort ran
+    rows = got.arrays()
+    q.close()
+    ix.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert_rows_equal(rows, o["final"], what=f"global path, {n_t} target windows")
+    np.testing.assert_array_equal(rows["twin"], o["final"]["twin"])
+
+
 @pytest.mark.parametrize("gap", [2, 8, 16])
 def test_dp_16bit_lanes(gap, monkeypatch):
     # The DP runs a task in packed 16-bit lanes when its query has at most
