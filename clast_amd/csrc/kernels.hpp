@@ -1153,7 +1153,7 @@ static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, S
   __shared__ uint32_t pend[kWriteBlock];   // inclusive prefix of the round's cell counts
   __shared__ uint32_t plo[kWriteBlock];    // each position's first entry
   __shared__ int2 pfirst[kWriteBlock];     // and that entry's (sequence, tIdx)
-  __shared__ unsigned long long wmask[kW];
+  __shared__ unsigned long long wave_wins[kW];
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const uint64_t none = (1ull << (pk.gs_bits + pk.d_bits)) - 1ull;
@@ -1212,15 +1212,15 @@ static __global__ __launch_bounds__(kWriteBlock) void k_seed_write(SeedArgs a, S
     }
     if (masks) {
       const uint64_t m = wave_or(wm);
-      if (lane == 0) wmask[w] = m;
+      if (lane == 0) wave_wins[w] = m;
       __syncthreads();
       if (threadIdx.x < 64) {
         uint64_t all = 0;
 #pragma unroll
-        for (int v = 0; v < kW; ++v) all |= wmask[v];
+        for (int v = 0; v < kW; ++v) all |= wave_wins[v];
         if ((all >> threadIdx.x) & 1ull) note_unit_first(a, static_cast<int>(threadIdx.x), qw, s);
       }
-      __syncthreads();  // wmask is rewritten by the next strand
+      __syncthreads();  // wave_wins is rewritten by the next strand
     }
   }
 }
@@ -2244,125 +2244,125 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
   // capped); the loop bound is the same for every wavefront of a workgroup
   constexpr uint32_t kBlockPairs = (kBlock / 64) * kPairs;
   for (uint32_t blk = blockIdx.x; blk * kBlockPairs < (nband + 1) / 2; blk += gridDim.x) {
-  const uint32_t pair = (blk * (kBlock / 64) + wave_id()) * kPairs + static_cast<uint32_t>(grp);
-  // every lane stays to the workgroup barrier; lanes without a pair run
-  // task 0's setup and no steps
-  const bool live = grp < kPairs && 2 * pair < nband;
-  const uint32_t taskA = live ? a.order[2 * pair] : 0u;
-  const uint32_t taskB = live && 2 * pair + 1 < nband ? a.order[2 * pair + 1] : taskA;
-  const int L = a.lmer;
-  const int64_t tbytes = 16 * (static_cast<int64_t>(a.t_bmax) + 1);
-  // X0 is the task this lane works on at even steps: A for even b, B for odd
-  BandTask X0, X1;
-  auto setup = [&](BandTask& x, int k) {
-    const uint32_t task = ((b & 1) ^ k) ? taskB : taskA;
-    const uint32_t i = task >> 1;
-    const bool bwd = (task & 1u) == 0u;
-    const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
-    const int qLen = a.qlen[s];
-    x.rows = bwd ? qLen - (qIdx + L) : qIdx;
-    // task A's cell (r, b) at step 2r + b, B's at 2r + b + 1: the row at
-    // this lane's first step of the task (step k)
-    const int c = (b & 1) ^ k;  // 0: task A's timing, 1: B's
-    x.r = (k - b - c) >> 1;
-    const int tLen = a.tlen[gs];
-    const int64_t tslot = static_cast<int64_t>(a.tslot[gs]);
-    const int w = a.twin[gs];
-    const int wlo = static_cast<int>(static_cast<int64_t>(a.wslot[w]) - tslot);
-    const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - tslot);
-    x.ts = bwd ? 1 : -1;
-    const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
-    x.lo = bwd ? wlo : max(wlo, 0);
-    x.hi = bwd ? min(whi, tLen) : whi;
-    // lane W-1 reads position r + W - 1 of row r: its stream starts at
-    // position G, so stream index j = floor(step / 2) in every lane
-    x.ti = tb + x.ts * G;
-    const bool top = b == W - 1;
-    x.st.init(top ? a.tbase : a.qbase, top ? a.t_bmax : a.q_bmax,
-              top ? tslot + x.ti : static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), x.ts);
-    const int tp = tb + x.ts * (b - 1);
-    const int64_t g = min(max(tslot + tp, int64_t{0}), tbytes - 1);
-    x.t0 = t_mask(a.tbase[g], tp, x.lo, x.hi);
-    x.fS = 0; x.fM = 0; x.fR = -1;
-  };
-  setup(X0, 0);
-  setup(X1, 1);
-  // row -1 of band b + 1 (k_dp's dp_init)
-  const int bi = b + 1;
-  const int Sinit = bi < half ? -30000 : bi == half ? 0 : (bi - half) * kGapExt + kGapOpen;
-  const int Vinit = bi < half ? -30000 : bi == half ? kGapOpen : Sinit + kGapOpen;
-  const int Minit = bi < half ? -30000 : 0;
-  X0.S = X1.S = Sinit;
-  X0.M = X1.M = Minit;
-  // exports of step -1 (this lane's X1 task at row -1)
-  int eS = Sinit, eV = Vinit, eH = 0, eM = Minit;
-  uint32_t eT = X1.t0, eQ = 0;
-  // last step: task A ends at lane W-1, step 2 (rowsA - 1) + W - 1; B one later
-  const bool odd = (b & 1) != 0;
-  const int rowsA = odd ? X1.rows : X0.rows, rowsB = odd ? X0.rows : X1.rows;
-  const int tend = live ? max(2 * (rowsA - 1) + W - 1, 2 * (rowsB - 1) + W) + 1 : 0;
-#pragma unroll 1
-  for (int t = 0; t < tend; t += 2) {
-    const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
-    band_step<G, LOCAL>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-    band_step<G, LOCAL>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-  }
-  // end node per task: the group's lanes through LDS, scanned in band order
-  const int wv = wave_id();
-  const int aS = LOCAL ? (odd ? X1.fS : X0.fS) : (odd ? X1.S : X0.S);
-  const int aM = LOCAL ? (odd ? X1.fM : X0.fM) : (odd ? X1.M : X0.M);
-  const int bS = LOCAL ? (odd ? X0.fS : X1.fS) : (odd ? X0.S : X1.S);
-  const int bM = LOCAL ? (odd ? X0.fM : X1.fM) : (odd ? X0.M : X1.M);
-  red[wv][lane][0] = aS;
-  red[wv][lane][1] = aM;
-  red[wv][lane][2] = odd ? X1.fR : X0.fR;
-  red[wv][lane][3] = bS;
-  red[wv][lane][4] = bM;
-  red[wv][lane][5] = odd ? X0.fR : X1.fR;
-  __syncthreads();
-  if (live && b < 2 && !(b == 1 && taskB == taskA)) {
-    const int k = b;  // lane 0 finishes task A, lane 1 task B
-    const int base = grp * W;
-    const int rows = k ? rowsB : rowsA;
-    int move, endp, bestS, bestM;
-    if (LOCAL) {
-      // k_dp keeps the first cell, in row-major order, of the highest score
-      // above 0: highest score, then lowest row, then lowest band, as one key
-      // (0 for a band that never rose above 0)
-      uint64_t best = 0;
-      int bq = -1;
-      for (int q = 0; q < W; ++q) {
-        const int sc = red[wv][base + q][3 * k], rr = red[wv][base + q][3 * k + 2];
-        const uint64_t key = rr < 0 ? 0ull
-                                    : (static_cast<uint64_t>(static_cast<uint32_t>(sc) ^ 0x80000000u) << 32) |
-                                          (static_cast<uint64_t>(0x3FFFFFF - rr) << 6) | static_cast<uint64_t>(63 - q);
-        if (key > best) {
-          best = key;
-          bq = q;
-        }
-      }
-      bestS = bq < 0 ? 0 : red[wv][base + bq][3 * k];
-      bestM = bq < 0 ? 0 : red[wv][base + bq][3 * k + 1];
-      move = bq < 0 ? 0 : red[wv][base + bq][3 * k + 2] + 1;
-      endp = (bq < 0 ? G : bq) - G;
-    } else {
-      bestS = -30000;  // band 0
-      bestM = -30000;
-      int e = -1;
-      for (int q = 0; q < W; ++q) {
-        const int sc = red[wv][base + q][3 * k];
-        if (bestS < sc) {
-          bestS = sc;
-          bestM = red[wv][base + q][3 * k + 1];
-          e = q;
-        }
-      }
-      move = rows;
-      endp = e - G;
+    const uint32_t pair = (blk * (kBlock / 64) + wave_id()) * kPairs + static_cast<uint32_t>(grp);
+    // every lane stays to the workgroup barrier; lanes without a pair run
+    // task 0's setup and no steps
+    const bool live = grp < kPairs && 2 * pair < nband;
+    const uint32_t taskA = live ? a.order[2 * pair] : 0u;
+    const uint32_t taskB = live && 2 * pair + 1 < nband ? a.order[2 * pair + 1] : taskA;
+    const int L = a.lmer;
+    const int64_t tbytes = 16 * (static_cast<int64_t>(a.t_bmax) + 1);
+    // X0 is the task this lane works on at even steps: A for even b, B for odd
+    BandTask X0, X1;
+    auto setup = [&](BandTask& x, int k) {
+      const uint32_t task = ((b & 1) ^ k) ? taskB : taskA;
+      const uint32_t i = task >> 1;
+      const bool bwd = (task & 1u) == 0u;
+      const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
+      const int qLen = a.qlen[s];
+      x.rows = bwd ? qLen - (qIdx + L) : qIdx;
+      // task A's cell (r, b) at step 2r + b, B's at 2r + b + 1: the row at
+      // this lane's first step of the task (step k)
+      const int c = (b & 1) ^ k;  // 0: task A's timing, 1: B's
+      x.r = (k - b - c) >> 1;
+      const int tLen = a.tlen[gs];
+      const int64_t tslot = static_cast<int64_t>(a.tslot[gs]);
+      const int w = a.twin[gs];
+      const int wlo = static_cast<int>(static_cast<int64_t>(a.wslot[w]) - tslot);
+      const int whi = static_cast<int>(static_cast<int64_t>(a.wslot[w + 1]) - tslot);
+      x.ts = bwd ? 1 : -1;
+      const int tb = bwd ? tIdx + L - G : tIdx + G - 1;
+      x.lo = bwd ? wlo : max(wlo, 0);
+      x.hi = bwd ? min(whi, tLen) : whi;
+      // lane W-1 reads position r + W - 1 of row r: its stream starts at
+      // position G, so stream index j = floor(step / 2) in every lane
+      x.ti = tb + x.ts * G;
+      const bool top = b == W - 1;
+      x.st.init(top ? a.tbase : a.qbase, top ? a.t_bmax : a.q_bmax,
+                top ? tslot + x.ti : static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), x.ts);
+      const int tp = tb + x.ts * (b - 1);
+      const int64_t g = min(max(tslot + tp, int64_t{0}), tbytes - 1);
+      x.t0 = t_mask(a.tbase[g], tp, x.lo, x.hi);
+      x.fS = 0; x.fM = 0; x.fR = -1;
+    };
+    setup(X0, 0);
+    setup(X1, 1);
+    // row -1 of band b + 1 (k_dp's dp_init)
+    const int bi = b + 1;
+    const int Sinit = bi < half ? -30000 : bi == half ? 0 : (bi - half) * kGapExt + kGapOpen;
+    const int Vinit = bi < half ? -30000 : bi == half ? kGapOpen : Sinit + kGapOpen;
+    const int Minit = bi < half ? -30000 : 0;
+    X0.S = X1.S = Sinit;
+    X0.M = X1.M = Minit;
+    // exports of step -1 (this lane's X1 task at row -1)
+    int eS = Sinit, eV = Vinit, eH = 0, eM = Minit;
+    uint32_t eT = X1.t0, eQ = 0;
+    // last step: task A ends at lane W-1, step 2 (rowsA - 1) + W - 1; B one later
+    const bool odd = (b & 1) != 0;
+    const int rowsA = odd ? X1.rows : X0.rows, rowsB = odd ? X0.rows : X1.rows;
+    const int tend = live ? max(2 * (rowsA - 1) + W - 1, 2 * (rowsB - 1) + W) + 1 : 0;
+  #pragma unroll 1
+    for (int t = 0; t < tend; t += 2) {
+      const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
+      band_step<G, LOCAL>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      band_step<G, LOCAL>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
     }
-    a.dres[k ? taskB : taskA] = make_int4(move + endp, move, bestM, bestS);
-  }
-  __syncthreads();  // red is rewritten by the next round
+    // end node per task: the group's lanes through LDS, scanned in band order
+    const int wv = wave_id();
+    const int aS = LOCAL ? (odd ? X1.fS : X0.fS) : (odd ? X1.S : X0.S);
+    const int aM = LOCAL ? (odd ? X1.fM : X0.fM) : (odd ? X1.M : X0.M);
+    const int bS = LOCAL ? (odd ? X0.fS : X1.fS) : (odd ? X0.S : X1.S);
+    const int bM = LOCAL ? (odd ? X0.fM : X1.fM) : (odd ? X0.M : X1.M);
+    red[wv][lane][0] = aS;
+    red[wv][lane][1] = aM;
+    red[wv][lane][2] = odd ? X1.fR : X0.fR;
+    red[wv][lane][3] = bS;
+    red[wv][lane][4] = bM;
+    red[wv][lane][5] = odd ? X0.fR : X1.fR;
+    __syncthreads();
+    if (live && b < 2 && !(b == 1 && taskB == taskA)) {
+      const int k = b;  // lane 0 finishes task A, lane 1 task B
+      const int base = grp * W;
+      const int rows = k ? rowsB : rowsA;
+      int move, endp, bestS, bestM;
+      if (LOCAL) {
+        // k_dp keeps the first cell, in row-major order, of the highest score
+        // above 0: highest score, then lowest row, then lowest band, as one key
+        // (0 for a band that never rose above 0)
+        uint64_t best = 0;
+        int bq = -1;
+        for (int q = 0; q < W; ++q) {
+          const int sc = red[wv][base + q][3 * k], rr = red[wv][base + q][3 * k + 2];
+          const uint64_t key = rr < 0 ? 0ull
+                                      : (static_cast<uint64_t>(static_cast<uint32_t>(sc) ^ 0x80000000u) << 32) |
+                                            (static_cast<uint64_t>(0x3FFFFFF - rr) << 6) | static_cast<uint64_t>(63 - q);
+          if (key > best) {
+            best = key;
+            bq = q;
+          }
+        }
+        bestS = bq < 0 ? 0 : red[wv][base + bq][3 * k];
+        bestM = bq < 0 ? 0 : red[wv][base + bq][3 * k + 1];
+        move = bq < 0 ? 0 : red[wv][base + bq][3 * k + 2] + 1;
+        endp = (bq < 0 ? G : bq) - G;
+      } else {
+        bestS = -30000;  // band 0
+        bestM = -30000;
+        int e = -1;
+        for (int q = 0; q < W; ++q) {
+          const int sc = red[wv][base + q][3 * k];
+          if (bestS < sc) {
+            bestS = sc;
+            bestM = red[wv][base + q][3 * k + 1];
+            e = q;
+          }
+        }
+        move = rows;
+        endp = e - G;
+      }
+      a.dres[k ? taskB : taskA] = make_int4(move + endp, move, bestM, bestS);
+    }
+    __syncthreads();  // red is rewritten by the next round
   }
 }
 
