@@ -321,6 +321,39 @@ def pressure_sweep(p, ix, t, rates, n_requests: int, workers: int, part_position
 
 # ---------------------------------------------------------------- large (1 Gbp x 1 M reads)
 
+def stage_rates(kt, cnt, dpk, npos) -> dict:
+    """Each stage of one search at a time against its bound, summed over the
+    searches, with algorithmic work by the conventions of the hot-path
+    table: seed lookup 16 B per query k-mer position; seed generation
+    (expansion, sort and the isolate / duplicate compaction, fused per
+    strand) 24 B per raw seed + 32 B per seed sorted + 16 B in and 16 B out
+    per seed filtered (raw seeds stand for the seeds past the corner filter,
+    which are fewer, so the figure is an upper bound) + 16 B per hit out;
+    results (E-value, bad-hit filters, dedup, order) 40 B in and 40 B out
+    per hit; DP as cells x 14 int32-equivalent ops (9 in the score pass of a
+    two-pass stage, whose second pass's cells count 14) against the packed
+    16-bit VALU peak."""
+    raw, hits = float(cnt[:, 0].sum()), float(cnt[:, 1].sum())
+    ops = float((cnt[:, 2] * np.where(dpk[:, 11] > 0, OPS_PER_CELL_SCORE, OPS_PER_CELL)).sum()
+                + OPS_PER_CELL * dpk[:, 9].sum())
+    work = {"seed_lookup": (kt[:, 0], 16.0 * float(npos.sum()), "hbm"),
+            "seed_generation": (kt[:, 5], (24.0 + 32.0 + 32.0) * raw + 16.0 * hits, "hbm"),
+            "dp": (kt[:, 4], ops, "valu_int16"),
+            "results": (kt[:, 6], 80.0 * hits, "hbm")}
+    out = {}
+    for k, (ms, w, bound) in work.items():
+        sec = float(ms.sum()) / 1e3
+        if bound == "hbm":
+            rate = w / sec / 1e9 if sec > 0 else 0.0
+            out[k] = {"ms": round(1e3 * sec, 3), "GB": round(w / 1e9, 3), "GB/s": round(rate, 1),
+                      "frac": round(rate / HBM_PEAK_GBS, 4), "bound": bound}
+        else:
+            rate = w / sec / 1e12 if sec > 0 else 0.0
+            out[k] = {"ms": round(1e3 * sec, 3), "Tops": round(w / 1e12, 3), "Tops/s": round(rate, 2),
+                      "frac": round(rate / VALU_INT16_PEAK_TOPS, 4), "bound": bound}
+    return out
+
+
 def large_leg(steps: int, warmup: int, streams: int) -> dict:
     """Each step: every query RAM chunk's reads uploaded and encoded on the
     device (clast_query_create_reads), searched against the resident index
@@ -408,6 +441,7 @@ def large_leg(steps: int, warmup: int, streams: int) -> dict:
                                  "traffic": measured_traffic("large", "k_seed_count")},
         "dp16": {"GCUPS": round(float(dpk[:, 0].sum()) / (dp16_ms / 1e3) / 1e9, 1) if dp16_ms else 0.0,
                  "cells_16bit_frac": round(float(cnt[:, 5].sum()) / max(float(cnt[:, 2].sum()), 1.0), 4)},
+        "stages": stage_rates(kt, cnt, dpk, npos),
     }
 
 
