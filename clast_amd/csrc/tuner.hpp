@@ -10,8 +10,11 @@
 //   radix  digit bits of the task-key radix sort (4..8).
 // Every configuration computes the same rows; they differ only in speed.
 // The tuner keeps, per (gap, global / local, length bucket of the search's
-// longest extension), the configuration that was fastest in timed trials on
-// the first search of that kind, and serves it to later searches.
+// longest extension, size class of the search), the configuration that was
+// fastest in timed trials on the first search of that kind (each the least
+// of three runs), and serves it to later searches.  The size class keeps small searches, whose DP stage is
+// bound by the latency of their longest extensions, apart from large ones,
+// bound by throughput: the faster configuration differs between them.
 // Environment: CLAST_TUNE=0 turns trials off (the defaults, sched 0, band 0,
 // radix 8, are then used); CLAST_DP_SCHED=static|persist, CLAST_DP_BAND=
 // off|rest|all and CLAST_RADIX_BITS=4..8 fix a knob (the tuner varies only
@@ -48,6 +51,10 @@ struct DPConfig {
 
 // Length bucket of a search by its longest extension (rows).
 inline int dp_bucket(int maxrows) { return maxrows < 256 ? 0 : maxrows < 2048 ? 1 : maxrows < 8192 ? 2 : 3; }
+
+// Size class of a search by its DP task count: 0 below kSmallTasks.
+constexpr uint32_t kSmallTasks = 1u << 15;
+inline int dp_size_class(uint32_t ntask) { return ntask < kSmallTasks ? 0 : 1; }
 
 // Forced knobs from the environment (-1: free).
 struct DPForced {
@@ -93,7 +100,7 @@ class DPTuner {
     static DPTuner* t = new DPTuner();  // never destroyed: used from any thread until exit
     return *t;
   }
-  using Key = std::tuple<int, int, int>;  // gap, local, bucket
+  using Key = std::tuple<int, int, int, int>;  // gap, local, bucket, size class
   bool lookup(const Key& k, DPConfig* out) {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = table_.find(k);
@@ -125,16 +132,16 @@ class DPTuner {
     if (table_.count(k)) return;  // another thread tuned it first
     table_[k] = Entry{best, trials};
   }
-  // One line per tuned kind: "gap=G local=L bucket=B -> sched=S band=N radix=R | trials: cfg ms, ..."
+  // One line per tuned kind: "gap=G local=L bucket=B size=Z -> sched=S band=N radix=R | trials: cfg ms, ..."
   std::string report() {
     std::lock_guard<std::mutex> lk(mu_);
     std::string s;
     char buf[160];
     for (auto& kv : table_) {
       const auto& e = kv.second;
-      std::snprintf(buf, sizeof buf, "gap=%d local=%d bucket=%d -> sched=%d band=%d radix=%d | trials:",
-                    std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), e.best.sched, e.best.band,
-                    e.best.radix);
+      std::snprintf(buf, sizeof buf, "gap=%d local=%d bucket=%d size=%d -> sched=%d band=%d radix=%d | trials:",
+                    std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first),
+                    e.best.sched, e.best.band, e.best.radix);
       s += buf;
       for (auto& t : e.trials) {
         std::snprintf(buf, sizeof buf, " %d/%d/%d %.3f ms", t.first.sched, t.first.band, t.first.radix, t.second);
@@ -143,8 +150,9 @@ class DPTuner {
       s += "\n";
     }
     for (auto& kv : survive_) {
-      std::snprintf(buf, sizeof buf, "gap=%d local=%d bucket=%d: 16-bit tasks surviving the filters %.4f\n",
-                    std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
+      std::snprintf(buf, sizeof buf, "gap=%d local=%d bucket=%d size=%d: 16-bit tasks surviving the filters %.4f\n",
+                    std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), std::get<3>(kv.first),
+                    kv.second);
       s += buf;
     }
     return s;

@@ -246,7 +246,8 @@ int clast_hits_dp_kernels(clast_hits h, double* out12);
  * lane per band (0 none, 1 those the 16-bit kernel cannot take, 2 every one
  * over twice a SIMD's share), [2] radix digit bits of the task-key sort.
  * The configuration of each kind of search (gap, global / local, length
- * bucket of its longest extension: < 256, < 2048, < 8192 rows, longer) is
+ * bucket of its longest extension: < 256, < 2048, < 8192 rows, longer; size
+ * class: under 32768 DP tasks or not) is
  * picked by timed trials on the first such search in the process (rows are
  * the same under every configuration); CLAST_TUNE=0 turns trials off and
  * CLAST_DP_SCHED=static|persist, CLAST_DP_BAND=off|rest|all and
