@@ -779,11 +779,14 @@ enum SegMode { kSegFirst = 0, kSegBigPass = 1, kSegRerun = 2 };
 
 // Bitonic sort of one u64 per lane, lanes [0, N) ascending.  Lanes from N
 // on must hold ~0 (with N < 64 they never exchange with the lower lanes).
+// A lane takes its partner's key o when (o < k) equals its keep-the-min
+// flag (equal keys give the same result either way): one compare and one
+// select per stage.
 template <int KK, int J>
 __device__ __forceinline__ uint64_t bitonic_merge(uint64_t k, int lane) {
   const uint64_t o = xor_lane64<J>(k);
   const bool keep_min = ((lane & KK) == 0) == ((lane & J) == 0);
-  k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+  k = (o < k) == keep_min ? o : k;
   if constexpr (J > 1) return bitonic_merge<KK, J / 2>(k, lane);
   else return k;
 }
@@ -799,7 +802,7 @@ template <int J>
 __device__ __forceinline__ uint64_t wave_merge(uint64_t k, int lane, bool asc) {
   const uint64_t o = xor_lane64<J>(k);
   const bool keep_min = ((lane & J) == 0) == asc;
-  k = keep_min ? (o < k ? o : k) : (o > k ? o : k);
+  k = (o < k) == keep_min ? o : k;
   if constexpr (J > 1) return wave_merge<J / 2>(k, lane, asc);
   else return k;
 }
