@@ -1099,7 +1099,8 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       }
       DPArgs da{ntask, hs, hgs, ht, hq, q.qbase.ptr, q.qslot.ptr, q.qlen.ptr, ix.tbase.ptr, ix.seq_slot.ptr,
                 ix.seq_len.ptr, ix.seq_win.ptr, ix.win_slot.ptr, dporder, split, q.qnib.ptr, ix.tnib.ptr, dres,
-                q.bmax, ix.bmax, p.lmer, 4 * cu, chunk_next, ot, oq, thl, qhl, om, osc};
+                q.bmax, ix.bmax, p.lmer, 4 * cu, chunk_next, ot, oq, thl, qhl, om, osc, q.qclean.ptr,
+                ix.seq_clean.ptr};
       fmark(EV_DP0);
       // the long tasks (raised wave priority), the 16-bit tasks past a
       // sequence end and the int32 tasks run on the side stream beside the

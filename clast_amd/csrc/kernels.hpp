@@ -1396,6 +1396,8 @@ struct DPArgs {
   int32_t* o_qhl;
   int32_t* o_match;
   int32_t* o_score;
+  const uint8_t* qclean;   // per query strand / target sequence: no star or X class
+  const uint8_t* tclean;   // (k_dp_band's substitution shortcut)
 };
 
 constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
@@ -2151,6 +2153,7 @@ __device__ __forceinline__ int pick(int m, int a, int b) { return b ^ ((a ^ b) &
 // One task's per-lane view in k_dp_band.
 struct BandTask {
   int rows;
+  bool clean;     // query strand and target sequence without star or X letters
   int r;          // this lane's row at its next step of this task
   int ti, ts;     // target index this lane reads next (lane W-1) and its step
   int lo, hi;     // t_mask range
@@ -2173,7 +2176,12 @@ __device__ __forceinline__ void row_words_sel(uint32_t qc, uint32_t& wpm, uint32
 
 // One step of task X in lane b: computes cell (X.r, b) from the exports of
 // the neighbouring lanes' previous step and updates this lane's exports.
-template <int G, bool LOCAL>
+// CLEAN (every task of the wavefront without star or X letters, so the query
+// class is 0-4 and the target class 0-4 or 'X' outside the window): the
+// substitution is +1 / match 1 for equal classes below 4, -1000 / -1000 for
+// an 'X' target, else -3 / 0 — krnlMatrix.cuh's entries for those classes,
+// without the row words.
+template <int G, bool LOCAL, bool CLEAN>
 __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, int Vinit, int Minit, int& eS,
                                           int& eV, int& eH, int& eM, uint32_t& eT, uint32_t& eQ) {
   constexpr int W = 2 * G + 1;
@@ -2197,13 +2205,22 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   const int h = bot ? max(-30000 + (kGapOpen + kGapExt), -30000 + kGapExt) : h0;
   const int Mv = top ? 0 : rM;
   const int Mh = bot ? -30000 : lM;
-  uint32_t wpm, wmc;
-  row_words_sel(qc, wpm, wmc);
-  const int pr = __builtin_amdgcn_sbfe(static_cast<int>(wpm), T, 4);
-  const int mr = __builtin_amdgcn_sbfe(static_cast<int>(wmc), T, 4);
-  const bool bad = pr == -8;
-  const int pm = bad ? kBad : pr;
-  const int mc = bad ? kBad : mr;
+  int pm, mc;
+  if constexpr (CLEAN) {
+    const uint32_t tcl = T >> 2;
+    const bool eq = (qc == tcl) & (qc < 4u);
+    const bool bad = tcl == kClsX;
+    pm = bad ? kBad : eq ? kMatch : kMismatch;
+    mc = bad ? kBad : eq ? 1 : 0;
+  } else {
+    uint32_t wpm, wmc;
+    row_words_sel(qc, wpm, wmc);
+    const int pr = __builtin_amdgcn_sbfe(static_cast<int>(wpm), T, 4);
+    const int mr = __builtin_amdgcn_sbfe(static_cast<int>(wmc), T, 4);
+    const bool bad = pr == -8;
+    pm = bad ? kBad : pr;
+    mc = bad ? kBad : mr;
+  }
   const int pt = X.S + pm;
   const int vh = max(v, h);
   const int nm_gap = v > h ? Mv : Mh;
@@ -2264,6 +2281,7 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
       const int s = a.hs[i], gs = a.hgs[i], tIdx = a.ht[i], qIdx = a.hq[i];
       const int qLen = a.qlen[s];
       x.rows = bwd ? qLen - (qIdx + L) : qIdx;
+      x.clean = a.qclean[s] && a.tclean[gs];
       // task A's cell (r, b) at step 2r + b, B's at 2r + b + 1: the row at
       // this lane's first step of the task (step k)
       const int c = (b & 1) ^ k;  // 0: task A's timing, 1: B's
@@ -2304,11 +2322,21 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
     const bool odd = (b & 1) != 0;
     const int rowsA = odd ? X1.rows : X0.rows, rowsB = odd ? X0.rows : X1.rows;
     const int tend = live ? max(2 * (rowsA - 1) + W - 1, 2 * (rowsB - 1) + W) + 1 : 0;
-  #pragma unroll 1
-    for (int t = 0; t < tend; t += 2) {
-      const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
-      band_step<G, LOCAL>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-      band_step<G, LOCAL>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      // lanes without a pair ran task 0's setup: they count as clean
+    if (__all(!live || (X0.clean && X1.clean))) {
+#pragma unroll 1
+      for (int t = 0; t < tend; t += 2) {
+        const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
+        band_step<G, LOCAL, true>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+        band_step<G, LOCAL, true>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      }
+    } else {
+#pragma unroll 1
+      for (int t = 0; t < tend; t += 2) {
+        const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
+        band_step<G, LOCAL, false>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+        band_step<G, LOCAL, false>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      }
     }
     // end node per task: the group's lanes through LDS, scanned in band order
     const int wv = wave_id();
