@@ -26,6 +26,9 @@ HOST = os.path.join(PKG, "host")
 GEN = os.path.join(PKG, "gen")
 INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# outputs are written under a per-process name, then renamed into place
+# (builds in parallel test workers do not clobber each other)
+_TMP = f".tmp{os.getpid()}"
 ARCH = "gfx950"
 
 # Packed-FP32 instructions are disabled for every HIP translation unit.
@@ -58,8 +61,8 @@ def build_hip() -> str:
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         list(ex.map(_run, cmds))
     out = os.path.join(LIB, "libclast_hip.so")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out + _TMP])
+    os.replace(out + _TMP, out)
     for o in objs:
         os.remove(o)
     return out
@@ -68,8 +71,8 @@ def build_hip() -> str:
 def build_gen() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libclast_gen.so")
-    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", os.path.join(GEN, "clast_gen.cpp"), "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", os.path.join(GEN, "clast_gen.cpp"), "-o", out + _TMP])
+    os.replace(out + _TMP, out)
     return out
 
 
@@ -80,16 +83,16 @@ def build_host() -> str:
     srcs = [os.path.join(HOST, u) for u in HOST_UNITS]
     lib = os.path.join(LIB, "libclast_host.so")
     common = ["g++", "-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}"]
-    _run([*common, "-shared", *srcs, f"-L{LIB}", "-lclast_hip", "-Wl,-rpath,$ORIGIN", "-o", lib + ".tmp"])
-    os.replace(lib + ".tmp", lib)
+    _run([*common, "-shared", *srcs, f"-L{LIB}", "-lclast_hip", "-Wl,-rpath,$ORIGIN", "-o", lib + _TMP])
+    os.replace(lib + _TMP, lib)
     exe = os.path.join(BIN, "clast")
     _run([*common, os.path.join(HOST, "main.cpp"), f"-L{LIB}", "-lclast_host", "-lclast_hip",
-          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
-    os.replace(exe + ".tmp", exe)
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + _TMP])
+    os.replace(exe + _TMP, exe)
     for src, name in (("preprocess_db.cpp", "preprocessDB"), ("divide_query.cpp", "devideQuery")):
         tool = os.path.join(BIN, name)
-        _run(["g++", "-O3", "-std=c++17", os.path.join(HOST, src), "-o", tool + ".tmp"])
-        os.replace(tool + ".tmp", tool)
+        _run(["g++", "-O3", "-std=c++17", os.path.join(HOST, src), "-o", tool + _TMP])
+        os.replace(tool + _TMP, tool)
     return exe
 
 
@@ -105,8 +108,8 @@ def build_host_selftest(sanitize: bool = True) -> str:
     exe = os.path.join(BIN, "host_selftest_san" if sanitize else "host_selftest")
     flags = SANITIZE if sanitize else ["-O2"]
     _run(["g++", *flags, "-std=c++17", f"-I{INCLUDE}", *srcs, f"-L{LIB}", "-lclast_hip",
-          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"])
-    os.replace(exe + ".tmp", exe)
+          "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + _TMP])
+    os.replace(exe + _TMP, exe)
     return exe
 
 

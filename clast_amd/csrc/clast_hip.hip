@@ -183,7 +183,7 @@ struct clast_index_s {
   DevArray<uint64_t> seq_slot;
   DevArray<int32_t> seq_len, seq_win, seq_ref;
   DevArray<int2> seq_info;  // (length, window) per target sequence
-  DevArray<uint8_t> seq_clean;  // no star / X class in the sequence (16-bit DP)
+  DevArray<uint8_t> seq_clean;  // 1: no star / X class in the sequence (16-bit DP), 3: no class 4 either
   DevArray<uint32_t> tnib;      // 4-bit copy of tbase (16-bit DP)
   uint32_t ncell = 0;
   DevArray<uint32_t> ekey32;
@@ -388,7 +388,7 @@ static void target_tables(Runtime& rt, clast_index_s& ix, const std::vector<uint
   ix.seq_clean.alloc(std::max(ix.nseq, 1));
   if (ix.nseq)
     k_clean_flags<uint64_t><<<grid_for(ix.nseq, kBlock / 64), kBlock, 0, st>>>(ix.tbase.ptr, ix.seq_slot.ptr,
-                                                                               ix.seq_len.ptr, ix.nseq,
+                                                                               ix.seq_len.ptr, ix.nseq, 1,
                                                                                ix.seq_clean.ptr);
 }
 
@@ -539,7 +539,7 @@ static void query_finish(Runtime& rt, clast_query_s& q, const QueryLayout& L) {
   q.qclean.alloc(std::max<uint64_t>(q.nseq, 1));
   if (q.nseq)
     k_clean_flags<uint32_t><<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.qbase.ptr, q.qslot.ptr, q.qlen.ptr,
-                                                                              q.nseq, q.qclean.ptr);
+                                                                              q.nseq, 0, q.qclean.ptr);
   q.smin.alloc(std::max<uint64_t>(q.nseq, 1));
   rt.check_launch("query tables");
   rt.sync();

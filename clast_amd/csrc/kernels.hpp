@@ -178,21 +178,22 @@ static __global__ __launch_bounds__(kBlock) void k_encode_reads(const uint8_t* _
 
 // clean[k] = 1 when the bytes [slot[k], slot[k] + len[k]) of a class array
 // hold no star or X class (scores and match counts of such a sequence's
-// cells stay within 16 bits; see k_dp16).  One wavefront per sequence, four
-// aligned 16-byte blocks per lane and step; bytes outside the range are
-// masked off.
+// cells stay within 16 bits; see k_dp16), 3 when with other_bit they also
+// hold no class 4 (k_dp16's query profiles), else 0.  One wavefront per
+// sequence, four aligned 16-byte blocks per lane and step; bytes outside the
+// range are masked off.
 template <class SlotT>
 static __global__ __launch_bounds__(kBlock) void k_clean_flags(const uint8_t* __restrict__ base,
                                                                const SlotT* __restrict__ slot,
                                                                const int32_t* __restrict__ len, uint32_t n,
-                                                               uint8_t* __restrict__ clean) {
+                                                               int other_bit, uint8_t* __restrict__ clean) {
   const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (k >= n) return;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t a = static_cast<uint64_t>(slot[k]);
   const uint64_t e = a + static_cast<uint64_t>(max(len[k], 0));
   const uint4* blk = reinterpret_cast<const uint4*>(base);
-  uint32_t bad = 0;
+  uint32_t bad = 0, other = 0;
   for (uint64_t b0 = (a >> 4) + lane; b0 * 16 < e; b0 += 256) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -202,18 +203,21 @@ static __global__ __launch_bounds__(kBlock) void k_clean_flags(const uint8_t* __
       const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        // bytes of classes 5 and 6 (the only ones above 4) get bit 3 set by +3
-        uint32_t hit = (w[d] + 0x03030303u) & 0x08080808u;
         const uint64_t p = b * 16 + 4u * d;  // first byte of the dword
+        uint32_t in = 0xFFFFFFFFu;
 #pragma unroll
         for (int y = 0; y < 4; ++y)
-          if (p + y < a || p + y >= e) hit &= ~(0xFFu << (8 * y));
-        bad |= hit;
+          if (p + y < a || p + y >= e) in &= ~(0xFFu << (8 * y));
+        // bytes of classes 5 and 6 (the only ones above 4) get bit 3 set by
+        // +3; bit 2 is set in classes 4, 5 and 6
+        bad |= (w[d] + 0x03030303u) & 0x08080808u & in;
+        other |= w[d] & 0x04040404u & in;
       }
     }
   }
   const bool any = __builtin_amdgcn_ballot_w64(bad != 0u) != 0ull;
-  if (lane == 0) clean[k] = any ? 0 : 1;
+  const bool any4 = __builtin_amdgcn_ballot_w64(other != 0u) != 0ull;
+  if (lane == 0) clean[k] = any ? 0 : other_bit && !any4 ? 3 : 1;
 }
 
 template <class T>
@@ -1397,7 +1401,7 @@ struct DPArgs {
   int32_t* o_match;
   int32_t* o_score;
   const uint8_t* qclean;   // per query strand / target sequence: no star or X class
-  const uint8_t* tclean;   // (k_dp_band's substitution shortcut)
+  const uint8_t* tclean;   // (k_dp_band's substitution shortcut; bit 1: no class 4, k_dp16's query profiles)
 };
 
 constexpr int kGapOpen = -5, kGapExt = -2, kMatch = 1, kMismatch = -3, kBad = -1000;
@@ -1875,6 +1879,36 @@ __device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W +
   }
 }
 
+// dp_row16_drift with the diagonal term from one byte permute per cell pair:
+// the pool is the row's query profiles (px / py: byte c = 10 when the half's
+// query class is c, else 6), the selector the ring's target word (half 0's
+// class in byte 0, half 1's as 4 + class in byte 2, 12 in bytes 1 and 3 for
+// zero high bytes).  Target classes 0..3 only (class 4 would select py's
+// byte 0): the task's target sequence holds no class 4.
+template <int W, int O>
+__device__ __forceinline__ void dp_row16_prof(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
+                                              uint32_t px, uint32_t py, s16x2 h1, s16x2 vw) {
+  s16x2 hp = h1;
+#pragma unroll
+  for (int b = 1; b <= W; ++b) {
+    const s16x2 v = b == W ? vw : __builtin_elementwise_max(Z[b + 1], V[b + 1]);
+    const s16x2 h = b == 1 ? h1 : __builtin_elementwise_max(Z[b - 1], hp);
+    const s16x2 pt = Z[b] + pk_val(__builtin_amdgcn_perm(py, px, T[O + b - 1]));
+    Z[b] = __builtin_elementwise_max(pt, __builtin_elementwise_max(v, h)) + pk_splat(-5);
+    V[b] = v;
+    hp = h;
+  }
+}
+
+// The query profile of dp_row16_prof for the class in byte K of Q (0 or 2):
+// byte c of the result is 10 when c equals the class, else 6 (class 7, the
+// query's other letters, equals none).
+template <int K>
+__device__ __forceinline__ uint32_t query_profile(uint32_t Q) {
+  const uint32_t sel = __builtin_amdgcn_perm(Q, Q, K * 0x01010101u) ^ 0x03020100u;
+  return __builtin_amdgcn_perm(0x06060606u, 0x0606060Au, sel);
+}
+
 // End node of the last row (k_dp's global selection: strict '<' from the
 // constant band 0) of half `half`, written to dres[task].  SCORE: the match
 // count is written as 0.
@@ -1924,11 +1958,13 @@ __device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], int4
 // query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
 // One chunk of 64 task pairs: lane l runs pair 64 chunk + l of the
 // kernel's task range [first, first + n).
-template <int G, bool EDGE, bool SCORE>
+template <int G, bool EDGE, bool SCORE, bool PROF>
 __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint32_t n, uint32_t chunk) {
   constexpr int W = 2 * G + 1;
-  // the score pass outside EDGE runs in drifted units (dp_row16_drift)
+  // the score pass outside EDGE runs in drifted units (dp_row16_drift; with
+  // PROF, dp_row16_prof)
   constexpr bool DRIFT = SCORE && !EDGE;
+  static_assert(!PROF || DRIFT, "query profiles in the drifted score pass only");
   // the target streams start kLead positions early, so that both streams
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
@@ -1999,7 +2035,9 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     for (int j = 0; j < 16; ++j) {
       const uint32_t v = tp.next(j);
       const int k = 16 * g + j - kLead;
-      if (k >= 0 && k < W) T[k] = EDGE ? v | (k >= xk[0] ? 0x8u : 0u) | (k >= xk[1] ? 0x80000u : 0u) : v;
+      if (k >= 0 && k < W) {
+        T[k] = EDGE ? v | (k >= xk[0] ? 0x8u : 0u) | (k >= xk[1] ? 0x80000u : 0u) : PROF ? v | 0x0C040C00u : v;
+      }
     }
   }
   const int rmax = max(rows[0], rows[1]);
@@ -2026,7 +2064,12 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
     T[W + i] = tp.next(j);          // target position r + W + kLead = r (mod 16)
     if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
-    if constexpr (DRIFT) {
+    if constexpr (PROF) {
+      T[W + i] |= 0x0C040C00u;
+      const int d4 = 4 * (r & 15);
+      dp_row16_prof<W, i>(Z, V, T, query_profile<0>(Q), query_profile<2>(Q), pk_splat(kH1 + 2 + d4),
+                          pk_splat(kVW + 2 * W + d4));
+    } else if constexpr (DRIFT) {
       const int d4 = 4 * (r & 15);
       dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
     } else {
@@ -2110,18 +2153,37 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   const uint32_t first = a.split[EDGE ? 1 : 0];
   const uint32_t n = a.split[EDGE ? 2 : 1] - first;
   const uint32_t nchunk = ((n + 1) / 2 + 63) / 64;
+  // the drifted score pass with 9 or more bands takes query profiles
+  // (dp_row16_prof) when every task of the chunk lies on a target sequence
+  // without class 4 (clean flag 3)
+  auto run = [&](uint32_t c) {
+    if constexpr (SCORE && !EDGE && 2 * G + 1 >= 9) {
+      const uint32_t t = c * 64 + (threadIdx.x & 63u);
+      bool prof = true;
+      if (2 * t < n) {
+        const uint32_t t0 = a.order[first + 2 * t];
+        const uint32_t t1 = 2 * t + 1 < n ? a.order[first + 2 * t + 1] : t0;
+        prof = (a.tclean[a.hgs[t0 >> 1]] & a.tclean[a.hgs[t1 >> 1]] & 2u) != 0u;
+      }
+      if (__all(prof)) {
+        dp16_chunk<G, EDGE, SCORE, true>(a, first, n, c);
+        return;
+      }
+    }
+    dp16_chunk<G, EDGE, SCORE, false>(a, first, n, c);
+  };
   if constexpr (PERSIST) {
     for (;;) {
       uint32_t c = 0;
       if ((threadIdx.x & 63u) == 0) c = atomicAdd(a.chunk_next, 1u);
       c = __builtin_amdgcn_readfirstlane(c);
       if (c >= nchunk) return;
-      dp16_chunk<G, EDGE, SCORE>(a, first, n, c);
+      run(c);
     }
   } else {
     const uint32_t wv = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (wv >= nchunk) return;
-    dp16_chunk<G, EDGE, SCORE>(a, first, n, snake_chunk(wv, nchunk, a.round));
+    run(snake_chunk(wv, nchunk, a.round));
   }
 }
 

@@ -353,6 +353,53 @@ def test_dp_16bit_lanes(gap, monkeypatch):
     assert_rows_equal(got0, o["final"], what=f"int32 dp gap {gap}")
 
 
+@pytest.mark.parametrize("with_n", [False, True], ids=["no_n", "n_in_two"])
+@pytest.mark.parametrize("gap", [4, 8, 16])
+def test_dp16_query_profiles(gap, with_n, monkeypatch):
+    # The 16-bit score pass with 9 or more bands takes its diagonal terms
+    # from per-row query profiles when every task of a 64-pair chunk lies on
+    # a target sequence without N; other chunks use the class-code xor.
+    # Every 5th read carries an N (the query's other class, which matches no
+    # target letter).  Targets: no N at all (every chunk on profiles), or N
+    # in two of six sequences (both kinds of chunk in one launch).  Rows equal
+    # the oracle's and the int32 kernel's.
+    monkeypatch.setenv("CLAST_DP_BAND", "off")
+    t, r = workload(n_t=6, t_len=20_000, n_r=300, n_rate=0.0, len_mode=gen.LEN_LOGUNIFORM, read_len=60,
+                    len_b=1200, indel=0.02, seed=21)
+    if with_n:
+        tb = t.bases.copy()
+        for k in (1, 4):
+            tb[int(t.offsets[k]) + 500:int(t.offsets[k + 1]) - 500:331] = ord("N")
+        t = gen.Target(tb, t.offsets)
+    seqs = [r.seq(i) for i in range(len(r))]
+    for i in range(0, len(seqs), 5):
+        b = bytearray(seqs[i])
+        b[len(b) // 3] = ord("N")
+        seqs[i] = bytes(b)
+    bases = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.int64)
+    r = gen.Reads(bases, off)
+    p = native.Params(total_db_size=db_size(t), gap=gap)
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 45_000, query=False)
+    qwins = native.windows(ql.gateway, 40_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    print(f"profiles gap={gap} with_n={with_n}: {len(twins)}x{len(qwins)} windows", flush=True)
+    h = native.search_batch(p, ix, q)
+    got, cnt = h.arrays(), h.counts()
+    monkeypatch.setenv("CLAST_DP16", "0")
+    h0 = native.search_batch(p, ix, q)
+    got0 = h0.arrays()
+    ix.close()
+    q.close()
+    o = oracle.search(p, tl, twins, ql, qwins)
+    assert len(o["final"]["tID"]) > 0
+    assert cnt[4] > 0, "no task in 16-bit lanes"
+    assert_rows_equal(got, o["final"], what=f"profiles gap {gap}")
+    assert_rows_equal(got0, o["final"], what=f"int32 dp gap {gap}")
+
+
 @pytest.mark.parametrize("local,gap", [(0, 2), (0, 8), (0, 16), (1, 8), (1, 16)])
 def test_dp_lane_per_band(local, gap, monkeypatch):
     # Extensions of at least 1024 rows run one lane per band (two tasks per
