@@ -1849,8 +1849,12 @@ __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s
   }
 }
 
+// Rows per drift period of the score pass (dp_row16_drift).
+constexpr int kDriftRows = 256;
+
 // One query row of the score pass (SCORE, not EDGE) in drifted units: every
-// value of band b at row r is held as value + d(r, b), d = 2 b + 4 (r mod 16)
+// value of band b at row r is held as value + d(r, b), d = 2 b + 4 (r mod
+// kDriftRows)
 // (Z as Z + 2 + d, i.e. S - 5 + d), which the gap terms' -2 steps cancel:
 // v(r, b) reads band b + 1 of row r - 1 (d two lower, Z two higher) and
 // h(r, b) band b - 1 of row r (the same), so
@@ -1862,7 +1866,9 @@ __device__ __forceinline__ void dp_row16(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s
 // select 0 for the high bytes).  Band 1's horizontal term and band W's
 // vertical term are constants of k_dp's edges (-30002 and the band W + 1
 // constants): h1 and vw, drifted by the caller.  The values change only by
-// d, and |d| <= 2 W + 60, inside the 16-bit margins of k_dp16's range proof.
+// d, and 0 <= d <= 2 W + 4 (kDriftRows - 1) <= 1086, inside the 16-bit
+// margin above k_dp16's range proof's scores (at most 30001, and the
+// diagonal adds at most 12).
 template <int W, int O>
 __device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
                                                uint32_t Qs, s16x2 h1, s16x2 vw) {
@@ -1954,10 +1960,14 @@ __device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], int4
 
 // Tasks order[2t] and order[2t + 1] (neighbours in the longest-first order,
 // so near-equal rows) in the low and high halves; a lone last task is run
-// in both.  Each half's end node is taken at its own last row.  Target and
-// query classes come from the 4-bit arrays (a.tnib, a.qnib) through NibWin.
-// One chunk of 64 task pairs: lane l runs pair 64 chunk + l of the
-// kernel's task range [first, first + n).
+// in both.  Target and query classes come from the 4-bit arrays (a.tnib,
+// a.qnib) through NibWin.  One chunk of 64 task pairs: lane l runs pair
+// 64 chunk + l of the kernel's task range [first, first + n).  The rows go
+// in step over the wavefront up to its longest task: a lane past its own
+// rows computes on (clamped reads, no stores; values stay inside the range
+// proof's bounds, which hold for any 30000 rows), and each half's end node
+// is taken at its own last row, when the wavefront reaches the next such
+// row of any lane (ev, uniform), so the row loop has no per-lane branch.
 template <int G, bool EDGE, bool SCORE, bool PROF>
 __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint32_t n, uint32_t chunk) {
   constexpr int W = 2 * G + 1;
@@ -1969,8 +1979,10 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
   constexpr int kInitGroups = (W + kLead) / 16;  // even
-  const uint32_t t = chunk * 64 + (threadIdx.x & 63u);
-  if (2 * t >= n) return;
+  const uint32_t tl = chunk * 64 + (threadIdx.x & 63u);
+  const bool valid = 2 * tl < n;
+  // a lane past the range runs pair 0 with no rows (no end node)
+  const uint32_t t = valid ? tl : 0u;
   const uint32_t task0 = a.order[first + 2 * t];
   const uint32_t task1 = 2 * t + 1 < n ? a.order[first + 2 * t + 1] : task0;
   const int L = a.lmer;
@@ -1993,6 +2005,7 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     if constexpr (EDGE) xk[h] = bwd ? a.tlen[a.hgs[i]] - tb : tb + 1;
     tw[h].init(tn2, tumax, static_cast<int64_t>(a.tslot[a.hgs[i]]) + tb - ts * kLead, ts);
     qw[h].init(qn2, qumax, static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), ts);
+    if (!valid) rows[h] = 0;
   }
   s16x2 Z[W + 2], V[W + 2], M[W + 2];
   constexpr int half = G + 1;
@@ -2006,9 +2019,9 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     } else {
       sc = (b - half) * kGapExt + kGapOpen; vv = sc + kGapOpen; mm = 0;
     }
-    // drifted (dp_row16_drift): as the last row of a group, d = 2 b + 60
-    const int dz = DRIFT && b >= 1 && b <= W ? 2 + 2 * b + 60 : 0;
-    const int dv = DRIFT && b >= 1 && b <= W ? 2 * b + 60 : 0;
+    // drifted (dp_row16_drift): as the last row of a period, d = 2 b + 4 (kDriftRows - 1)
+    const int dz = DRIFT && b >= 1 && b <= W ? 2 + 2 * b + 4 * (kDriftRows - 1) : 0;
+    const int dv = DRIFT && b >= 1 && b <= W ? 2 * b + 4 * (kDriftRows - 1) : 0;
     Z[b] = pk_splat(sc + kGapOpen + kGapExt + dz);
     V[b] = pk_splat(vv + dv);
     M[b] = pk_splat(mm);
@@ -2040,8 +2053,14 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
       }
     }
   }
-  const int rmax = max(rows[0], rows[1]);
-  const int rsplit = rows[0] != rows[1] ? min(rows[0], rows[1]) : -1;
+  // the wavefront's longest task, and its least end row after `after`
+  // (INT32_MAX: none)
+  const int rmax = __builtin_amdgcn_readlane(wave_max_scan_incl(max(rows[0], rows[1])), 63);
+  auto next_end = [&](int after) {
+    const int e = min(rows[0] > after ? rows[0] : INT32_MAX, rows[1] > after ? rows[1] : INT32_MAX);
+    return -__builtin_amdgcn_readlane(wave_max_scan_incl(-e), 63);
+  };
+  int ev = next_end(0);
   // 32 rows per iteration: even group, 16 rows, odd group, 16 rows (each
   // stream register then has one definition in the loop, so the loads land
   // in place); every active lane is at the same row
@@ -2054,7 +2073,7 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
   // k_dp's first)
   auto end_node = [&](uint32_t task, int h, int rws) {
     if constexpr (DRIFT) {
-      dp16_end_drift<G>(Z, a.dres, task, h, rws, 4 * ((rws - 1) & 15));
+      dp16_end_drift<G>(Z, a.dres, task, h, rws, 4 * ((rws - 1) & (kDriftRows - 1)));
     } else {
       dp16_end<G, SCORE>(Z, M, a.dres, task, h, rws);
     }
@@ -2066,27 +2085,30 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
     if constexpr (PROF) {
       T[W + i] |= 0x0C040C00u;
-      const int d4 = 4 * (r & 15);
+      const int d4 = 4 * (r & (kDriftRows - 1));
       dp_row16_prof<W, i>(Z, V, T, query_profile<0>(Q), query_profile<2>(Q), pk_splat(kH1 + 2 + d4),
                           pk_splat(kVW + 2 * W + d4));
     } else if constexpr (DRIFT) {
-      const int d4 = 4 * (r & 15);
+      const int d4 = 4 * (r & (kDriftRows - 1));
       dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
     } else {
       dp_row16<W, i, EDGE, SCORE>(Z, V, M, T, Q);
     }
-    if (r + 1 == rsplit) {
-      const int h = rows[0] == rsplit ? 0 : 1;
-      end_node(h ? task1 : task0, h, rsplit);
+    if (r + 1 == ev) {
+      if (rows[0] == ev) end_node(task0, 0, ev);
+      if (rows[1] == ev) end_node(task1, 1, ev);
+      ev = next_end(ev);
     }
   };
   auto rows16 = [&](int r0) {
     if constexpr (DRIFT) {
-      // a new group: d drops by 64 (row r0 - 1 becomes row -1 of the group)
+      if ((r0 & (kDriftRows - 1)) == 0) {
+        // a new period: d drops by 4 kDriftRows (row r0 - 1 becomes row -1)
 #pragma unroll
-      for (int b = 1; b <= W; ++b) {
-        Z[b] = Z[b] - pk_splat(64);
-        V[b] = V[b] - pk_splat(64);
+        for (int b = 1; b <= W; ++b) {
+          Z[b] = Z[b] - pk_splat(4 * kDriftRows);
+          V[b] = V[b] - pk_splat(4 * kDriftRows);
+        }
       }
     }
 #pragma unroll 1
@@ -2133,9 +2155,6 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     }
     rows16(rg + 16);
   }
-  // the longer half (both when equal; a lone task's copy writes the same)
-  if (rows[0] >= rows[1] && rows[0] > 0) end_node(task0, 0, rows[0]);
-  if (rows[1] >= rows[0] && rows[1] > 0) end_node(task1, 1, rows[1]);
 }
 
 // The chunks of the range go out longest first.  Static (PERSIST false):
