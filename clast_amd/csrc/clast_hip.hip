@@ -147,8 +147,11 @@ static void check_window(const clast_window& w) {
 
 // Score threshold of one query length: the smallest integer score s with
 // !(cutOff < K * D * qLen * exp(-lambda * s)) (…deleteBadHits.cuh:32-39
-// with krnlCalculateEvalue.cu:21), found by bisection on that exact
-// expression; INT_MAX when no score passes, INT_MIN when every score does.
+// with krnlCalculateEvalue.cu:21), decided by that exact expression;
+// INT_MAX when no score passes, INT_MIN when every score does.  The search
+// starts from the closed form ceil(ln(K D qLen / cutOff) / lambda) and
+// steps to the boundary (keep(s) and not keep(s - 1)); a guess that is not
+// finite or misses by more than a few steps falls back to bisection.
 static int32_t score_threshold(const clast_params& p, int qlen) {
   auto keep = [&](int64_t s) {
     const double e = p.K * p.total_db_size * qlen * std::exp(-p.lambda * static_cast<double>(s));
@@ -157,6 +160,13 @@ static int32_t score_threshold(const clast_params& p, int qlen) {
   int64_t lo = -(1ll << 30), hi = (1ll << 30);
   if (!keep(hi)) return INT32_MAX;
   if (keep(lo)) return INT32_MIN;
+  const double g = std::ceil(std::log(p.K * p.total_db_size * qlen / p.cut_off) / p.lambda);
+  if (std::isfinite(g) && g > static_cast<double>(lo + 8) && g < static_cast<double>(hi - 8)) {
+    int64_t s = static_cast<int64_t>(g);
+    for (int i = 0; i < 4 && !keep(s); ++i) ++s;
+    for (int i = 0; i < 4 && keep(s - 1); ++i) --s;
+    if (keep(s) && !keep(s - 1)) return static_cast<int32_t>(s);
+  }
   while (hi - lo > 1) {
     const int64_t mid = lo + (hi - lo) / 2;
     if (keep(mid)) hi = mid;
@@ -524,7 +534,7 @@ static void query_finish(Runtime& rt, clast_query_s& q, const QueryLayout& L) {
   q.nseq = static_cast<uint32_t>(L.qslot.size());
   q.npos = static_cast<uint32_t>(L.npos);
   q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
-  q.qslot.upload(L.qslot.data(), L.qslot.size(), st);
+  if (!q.qslot.ptr) q.qslot.upload(L.qslot.data(), L.qslot.size(), st);  // (build_query_reads uploads it first)
   q.pos_off.upload(L.pos_off.data(), L.pos_off.size(), st);
   q.pos_strand.alloc(std::max<uint64_t>(L.npos, 1));
   if (q.nseq) k_pos_strand<<<grid_for(q.nseq, kBlock / 64), kBlock, 0, st>>>(q.pos_off.ptr, q.nseq, q.pos_strand.ptr);
@@ -594,40 +604,56 @@ void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_
       throw ClastError(CLAST_ERR_ARG, "read windows must be consecutive ranges in order");
   }
   const int64_t joint = std::max(p.lmer - 1, 0);
+  const uint32_t nreads = static_cast<uint32_t>(r1 - r0);
+  const size_t nstr = 2 * static_cast<size_t>(nreads);
+  // one pass over the reads into arrays sized up front: read k's '+' strand
+  // is strand 2 (k - r0), its '-' strand the next
   QueryLayout L;
-  std::vector<uint64_t> rslot;  // '+' strand slot of each read
-  std::vector<int64_t> roff;    // byte offsets of the reads in the upload
-  rslot.reserve(static_cast<size_t>(r1 - r0));
-  roff.reserve(static_cast<size_t>(r1 - r0) + 1);
-  const int64_t b0 = offsets[r0];
-  uint64_t slot = 0;
+  L.qslot.resize(nstr);
+  L.pos_off.resize(nstr + 1);
+  L.win_slot.reserve(static_cast<size_t>(nwin) + 1);
+  q.h_qlen.resize(nstr);
+  q.h_qref.resize(nstr);
+  q.h_qwin.resize(nstr);
+  q.h_id_begin.assign(id_begin, id_begin + nwin);
+  uint64_t slot = 0, npos = 0;
+  int max_len = 0;
+  size_t si = 0;
   for (int w = 0; w < nwin; ++w) {
     for (int32_t k = win[2 * w]; k < win[2 * w + 1]; ++k) {
       const int64_t len = offsets[k + 1] - offsets[k];
       if (len < 0 || len > INT32_MAX - joint) throw ClastError(CLAST_ERR_ARG, "bad read offsets");
-      roff.push_back(offsets[k] - b0);
-      rslot.push_back(slot);
-      const int local = 2 * (k - win[2 * w]);
-      query_add_strand(p, q, L, slot, static_cast<int>(len), id_begin[w] + local, w);
-      query_add_strand(p, q, L, slot + static_cast<uint64_t>(len + joint), static_cast<int>(len),
-                       id_begin[w] + local + 1, w);
-      slot += 2 * static_cast<uint64_t>(len + joint);
+      const uint64_t span = static_cast<uint64_t>(len + joint);
+      if (slot + span >= (1ull << 32)) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 slots");
+      const uint64_t kpos = len >= p.lmer ? static_cast<uint64_t>(len - p.lmer + 1) : 0ull;
+      if (npos + 2 * kpos >= (1ull << 32) - 1) throw ClastError(CLAST_ERR_LIMIT, "query set over 2^32 k-mer positions");
+      const int32_t ref = id_begin[w] + 2 * (k - win[2 * w]);
+      for (int h = 0; h < 2; ++h, ++si) {
+        L.qslot[si] = static_cast<uint32_t>(slot + h * span);
+        q.h_qlen[si] = static_cast<int32_t>(len);
+        q.h_qref[si] = ref + h;
+        q.h_qwin[si] = w;
+        npos += kpos;
+        L.pos_off[si + 1] = static_cast<uint32_t>(npos);
+      }
+      max_len = std::max(max_len, static_cast<int>(len));
+      slot += 2 * span;
     }
     L.win_slot.push_back(slot);
-    q.h_id_begin.push_back(id_begin[w]);
   }
-  roff.push_back(offsets[r1] - b0);
+  L.npos = npos;
+  q.max_qlen = max_len;
   query_alloc_bases(q, L);
-  const uint32_t nreads = static_cast<uint32_t>(r1 - r0);
   DevArray<uint8_t> dreads;
-  DevArray<int64_t> droff;
-  DevArray<uint64_t> dslot;
+  DevArray<int64_t> doff;
   if (nreads) {
-    dreads.upload(bases + b0, static_cast<uint64_t>(roff.back()), st);
-    droff.upload(roff.data(), roff.size(), st);
-    dslot.upload(rslot.data(), rslot.size(), st);
-    k_encode_reads<<<grid_for(nreads, kBlock / 64), kBlock, 0, st>>>(dreads.ptr, droff.ptr, nreads, dslot.ptr, p.lmer,
-                                                                     q.qbase.ptr);
+    const int64_t b0 = offsets[r0];
+    dreads.upload(bases + b0, static_cast<uint64_t>(offsets[r1] - b0), st);
+    doff.upload(offsets + r0, static_cast<uint64_t>(nreads) + 1, st);
+    // the '+' strand slots (every other entry of the strand slots)
+    q.qslot.upload(L.qslot.data(), L.qslot.size(), st);
+    k_encode_reads<<<grid_for(nreads, kBlock / 64), kBlock, 0, st>>>(dreads.ptr, doff.ptr, b0, nreads, q.qslot.ptr,
+                                                                     p.lmer, q.qbase.ptr);
   }
   query_finish(rt, q, L);  // ends with a sync: the uploads are done before they are freed
 }
@@ -636,13 +662,27 @@ void ensure_smin(Runtime& rt, const clast_params& p, clast_query_s& q) {
   const double key[4] = {p.cut_off, p.total_db_size, p.K, p.lambda};
   std::lock_guard<std::mutex> lk(q.smin_mu);
   if (q.smin_valid && std::memcmp(key, q.smin_key, sizeof key) == 0) return;
-  std::unordered_map<int, int32_t> memo;
   std::vector<int32_t> sm(q.nseq);
-  for (uint32_t s = 0; s < q.nseq; ++s) {
-    const int len = q.h_qlen[s];
-    auto it = memo.find(len);
-    if (it == memo.end()) it = memo.emplace(len, score_threshold(p, len)).first;
-    sm[s] = it->second;
+  if (q.max_qlen <= (1 << 20)) {
+    // one threshold per length up to the longest strand, each found once
+    std::vector<int32_t> table(static_cast<size_t>(q.max_qlen) + 1);
+    std::vector<uint8_t> have(table.size(), 0);
+    for (uint32_t s = 0; s < q.nseq; ++s) {
+      const int len = std::max(q.h_qlen[s], 0);
+      if (!have[len]) {
+        table[len] = score_threshold(p, len);
+        have[len] = 1;
+      }
+      sm[s] = table[len];
+    }
+  } else {
+    std::unordered_map<int, int32_t> memo;
+    for (uint32_t s = 0; s < q.nseq; ++s) {
+      const int len = q.h_qlen[s];
+      auto it = memo.find(len);
+      if (it == memo.end()) it = memo.emplace(len, score_threshold(p, len)).first;
+      sm[s] = it->second;
+    }
   }
   if (q.nseq)
     hip_check(hipMemcpyAsync(q.smin.ptr, sm.data(), sizeof(int32_t) * q.nseq, hipMemcpyHostToDevice, rt.stream()),

@@ -148,18 +148,20 @@ static __global__ __launch_bounds__(kBlock) void k_classify(uint8_t* __restrict_
 // slot[k] and its reverse complement ('-': A<->T, C<->G, any other byte N)
 // right after, each followed by lMer-1 joint bases (the strand's own first
 // bases, N past a short strand's end); written as class codes.  One
-// wavefront per read; roff holds the reads' byte offsets in `reads`.
+// wavefront per read; `reads` holds the bytes from offset b0 on, off the
+// reads' offsets (nreads + 1), qslot the strand slots (read k's '+' strand
+// at qslot[2 k]).
 static __global__ __launch_bounds__(kBlock) void k_encode_reads(const uint8_t* __restrict__ reads,
-                                                                const int64_t* __restrict__ roff, uint32_t nreads,
-                                                                const uint64_t* __restrict__ slot, int lmer,
-                                                                uint8_t* __restrict__ qbase) {
+                                                                const int64_t* __restrict__ off, int64_t b0,
+                                                                uint32_t nreads, const uint32_t* __restrict__ qslot,
+                                                                int lmer, uint8_t* __restrict__ qbase) {
   const uint32_t k = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (k >= nreads) return;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint8_t* r = reads + roff[k];
-  const int64_t len = roff[k + 1] - roff[k];
+  const uint8_t* r = reads + (off[k] - b0);
+  const int64_t len = off[k + 1] - off[k];
   const int64_t span = len + max(lmer - 1, 0);  // slots per strand
-  uint8_t* plus = qbase + slot[k];
+  uint8_t* plus = qbase + qslot[2 * static_cast<uint64_t>(k)];
   uint8_t* minus = plus + span;
   for (int64_t i = lane; i < span; i += 64) {
     // the strand position whose base slot i holds (joint slots repeat the
