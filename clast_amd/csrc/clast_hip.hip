@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -533,7 +534,8 @@ static void query_finish(Runtime& rt, clast_query_s& q, const QueryLayout& L) {
   hipStream_t st = rt.stream();
   q.nseq = static_cast<uint32_t>(L.qslot.size());
   q.npos = static_cast<uint32_t>(L.npos);
-  q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
+  if (!q.qslot.ptr)  // (build_query_reads sets it and min_qlen first)
+    q.min_qlen = q.h_qlen.empty() ? 0 : *std::min_element(q.h_qlen.begin(), q.h_qlen.end());
   if (!q.qslot.ptr) q.qslot.upload(L.qslot.data(), L.qslot.size(), st);  // (build_query_reads uploads it first)
   q.pos_off.upload(L.pos_off.data(), L.pos_off.size(), st);
   q.pos_strand.alloc(std::max<uint64_t>(L.npos, 1));
@@ -606,6 +608,18 @@ void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_
   const int64_t joint = std::max(p.lmer - 1, 0);
   const uint32_t nreads = static_cast<uint32_t>(r1 - r0);
   const size_t nstr = 2 * static_cast<size_t>(nreads);
+  // the reads' bytes go up (a copy from pageable memory blocks its thread)
+  // while this thread lays out the strand tables; `up` is joined before
+  // dreads goes (declared after it), also when the layout throws
+  DevArray<uint8_t> dreads;
+  std::future<void> up;
+  if (nreads) {
+    const int dev = rt.device();
+    up = std::async(std::launch::async, [&dreads, bases, offsets, r0, r1, dev, st] {
+      hip_check(hipSetDevice(dev), "hipSetDevice");
+      dreads.upload(bases + offsets[r0], static_cast<uint64_t>(offsets[r1] - offsets[r0]), st);
+    });
+  }
   // one pass over the reads into arrays sized up front: read k's '+' strand
   // is strand 2 (k - r0), its '-' strand the next
   QueryLayout L;
@@ -617,7 +631,7 @@ void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_
   q.h_qwin.resize(nstr);
   q.h_id_begin.assign(id_begin, id_begin + nwin);
   uint64_t slot = 0, npos = 0;
-  int max_len = 0;
+  int max_len = 0, min_len = INT32_MAX;
   size_t si = 0;
   for (int w = 0; w < nwin; ++w) {
     for (int32_t k = win[2 * w]; k < win[2 * w + 1]; ++k) {
@@ -637,18 +651,19 @@ void build_query_reads(const clast_params& p, const uint8_t* bases, const int64_
         L.pos_off[si + 1] = static_cast<uint32_t>(npos);
       }
       max_len = std::max(max_len, static_cast<int>(len));
+      min_len = std::min(min_len, static_cast<int>(len));
       slot += 2 * span;
     }
     L.win_slot.push_back(slot);
   }
   L.npos = npos;
   q.max_qlen = max_len;
+  q.min_qlen = nreads ? min_len : 0;
   query_alloc_bases(q, L);
-  DevArray<uint8_t> dreads;
   DevArray<int64_t> doff;
   if (nreads) {
     const int64_t b0 = offsets[r0];
-    dreads.upload(bases + b0, static_cast<uint64_t>(offsets[r1] - b0), st);
+    up.get();  // the bytes are on the stream before the kernel
     doff.upload(offsets + r0, static_cast<uint64_t>(nreads) + 1, st);
     // the '+' strand slots (every other entry of the strand slots)
     q.qslot.upload(L.qslot.data(), L.qslot.size(), st);
