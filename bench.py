@@ -54,6 +54,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# hardware queues per process (HIP reads this at its first call): each device
+# thread of the server issues on three streams (compute, side, copy), so three
+# threads want nine queues; with HIP's default of four, streams share queues
+# and their kernels wait behind each other (serving p50 about 20% higher).
+# Raised to 16 when the environment asks for fewer.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or "4") < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 CLOCK_GHZ = 2.4

@@ -1200,6 +1200,9 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     DPConfig cfg;
     const bool tuned = DPTuner::get().lookup(tkey, &cfg);
     cfg = forced.apply(tuned ? cfg : DPConfig{});
+    // the persistent schedule runs gaps up to kPersistMaxGap only
+    if (cfg.sched == 1 && p.gap > kPersistMaxGap)
+      throw ClastError(CLAST_ERR_ARG, "CLAST_DP_SCHED=persist takes gaps up to " + std::to_string(kPersistMaxGap));
     // (the allocation-free entry point runs no trials: they create events)
     if (!tuned && forced.tune && !rows_out) {
       hipEvent_t e0, e1;
@@ -1236,7 +1239,7 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
           cfg = c;
         }
       };
-      if (packed && forced.sched < 0) consider(DPConfig{1 - cfg.sched, cfg.band, cfg.radix});
+      if (packed && forced.sched < 0 && p.gap <= kPersistMaxGap) consider(DPConfig{1 - cfg.sched, cfg.band, cfg.radix});
       if (maxrows >= kBandRows && forced.band < 0)
         for (int b = 0; b < 3; ++b) consider(DPConfig{cfg.sched, b, cfg.radix});
       if (forced.radix < 0)

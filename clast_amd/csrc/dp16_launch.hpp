@@ -17,8 +17,12 @@ static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, const DP
   if (edge) {
     k_dp16<G, true, false, SCORE><<<grid, kBlock, 0, st>>>(a);
   } else if (persist) {
-    grid = std::max<uint32_t>(1, std::min(grid, grid_cap));
-    k_dp16<G, false, true, SCORE><<<grid, kBlock, 0, st>>>(a);
+    if constexpr (G <= kPersistMaxGap) {
+      grid = std::max<uint32_t>(1, std::min(grid, grid_cap));
+      k_dp16<G, false, true, SCORE><<<grid, kBlock, 0, st>>>(a);
+    } else {
+      throw ClastError(1, "persistent 16-bit DP schedule over its gap limit");
+    }
   } else {
     k_dp16<G, false, false, SCORE><<<grid, kBlock, 0, st>>>(a);
   }

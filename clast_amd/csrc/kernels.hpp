@@ -2169,6 +2169,12 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
 // passes the last chunk.  SCORE: scores and end nodes only, match counts 0
 // (the score pass of a two-pass DP stage: the tasks of the hits that pass
 // the bad-hit filters run again without SCORE).
+// Widest band of the persistent schedule: wider bands' kernels spill many
+// registers to scratch, and with gap 16 the persistent kernel on long
+// extensions gave a wrong row and then a memory fault whose cause is not
+// found, so those gaps run the static schedule only.
+constexpr int kPersistMaxGap = 8;
+
 template <int G, bool EDGE, bool PERSIST, bool SCORE>
 static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_dp16(DPArgs a) {
   const uint32_t first = a.split[EDGE ? 1 : 0];

@@ -748,6 +748,29 @@ def test_dp_order_graph_replays_equal_eager(monkeypatch):
         h.close()
 
 
+def test_dp_persistent_schedule_gap_limit(monkeypatch):
+    # The persistent 16-bit schedule takes gaps up to 8: the tuner does not
+    # try it for wider bands, and forcing it there is refused before any DP
+    # launch; the static schedule gives the oracle's rows.
+    t, r = workload(n_t=2, t_len=20_000, n_r=60, seed=17)
+    p = native.Params(total_db_size=db_size(t), gap=16)
+    tl, ql = lists(t, r, p.lmer)
+    twins = native.windows(tl.gateway, 45_000, query=False)
+    qwins = native.windows(ql.gateway, 40_000, query=True)
+    ix = native.Index(p, tl, twins)
+    q = native.Query(p, ql, qwins)
+    monkeypatch.setenv("CLAST_DP_SCHED", "persist")
+    with pytest.raises(native.ClastError, match="persist takes gaps up to 8"):
+        native.search_batch(p, ix, q)
+    monkeypatch.delenv("CLAST_DP_SCHED")
+    h = native.search_batch(p, ix, q)
+    assert h.dp_config()[0] == 0
+    got = h.arrays()
+    ix.close()
+    q.close()
+    assert_rows_equal(got, oracle.search(p, tl, twins, ql, qwins)["final"], what="gap 16 static")
+
+
 @pytest.mark.parametrize("sched", ["static", "persist"])
 def test_dp_two_pass_equals_one_pass(sched, monkeypatch):
     # Two-pass DP stage: the 16-bit tasks first as scores only, then the
