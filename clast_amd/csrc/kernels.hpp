@@ -163,18 +163,33 @@ static __global__ __launch_bounds__(kBlock) void k_encode_reads(const uint8_t* _
   const int64_t span = len + max(lmer - 1, 0);  // slots per strand
   uint8_t* plus = qbase + qslot[2 * static_cast<uint64_t>(k)];
   uint8_t* minus = plus + span;
-  for (int64_t i = lane; i < span; i += 64) {
-    // the strand position whose base slot i holds (joint slots repeat the
-    // first bases); none past a short strand's end
-    const int64_t src = i < len ? i : i - len;
-    const bool has = src < len;
-    const uint32_t cp = has ? dp_class(r[src]) : kClsOther;
-    // '-' base at src is the complement of read base len - 1 - src; the
-    // complement keeps A C G T (as T G C A) and turns every other byte into N
-    const uint32_t cr = has ? dp_class(r[len - 1 - src]) : kClsOther;
-    const uint32_t cm = cr < 4u ? 3u - cr : kClsOther;
-    plus[i] = static_cast<uint8_t>(cp);
-    minus[i] = static_cast<uint8_t>(cm);
+  // kU slots per lane and round, their loads issued together (a long read's
+  // wavefront waits for one load latency per round, not per slot)
+  constexpr int kU = 8;
+  for (int64_t i0 = lane; i0 < span; i0 += 64 * kU) {
+    uint8_t bp[kU], bm[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + 64 * u;
+      // the strand position whose base slot i holds (joint slots repeat the
+      // first bases); none past a short strand's end or past the span
+      const int64_t src = i < len ? i : i - len;
+      const bool has = i < span && src < len;
+      bp[u] = has ? r[src] : 'N';
+      // '-' base at src is the complement of read base len - 1 - src
+      bm[u] = has ? r[len - 1 - src] : 'N';
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + 64 * u;
+      if (i >= span) break;
+      const uint32_t cp = dp_class(bp[u]);
+      // the complement keeps A C G T (as T G C A) and turns every other byte into N
+      const uint32_t cr = dp_class(bm[u]);
+      const uint32_t cm = cr < 4u ? 3u - cr : kClsOther;
+      plus[i] = static_cast<uint8_t>(cp);
+      minus[i] = static_cast<uint8_t>(cm);
+    }
   }
 }
 
