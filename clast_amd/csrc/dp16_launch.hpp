@@ -8,23 +8,32 @@
 
 namespace clast {
 
+// LDS the score pass's workgroups reserve (they use none): with 160 KiB per
+// CU at most three fit, so three of a SIMD's four register-file quarters go
+// to the score pass and the fourth stays open for the kernels beside it
+// (the second pass, past-the-end tasks, one lane per band, int32), which
+// otherwise wait for score-pass wavefronts of other searches to finish.
+constexpr size_t kScorePassLds = 41 * 1024;
+static_assert(kScorePassLds * 4 > 160 * 1024 && kScorePassLds * 3 <= 160 * 1024, "three workgroups per CU");
+
 // grid: one wavefront per chunk of 64 task pairs of all a.n tasks (the
 // kernel's range is a part of them); persistent: at most grid_cap
 // workgroups (about one occupancy round), which loop over the chunks.
 template <int G, bool SCORE>
 static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
   uint32_t grid = grid_for((static_cast<uint64_t>(a.n) + 1) / 2, kBlock);
+  const size_t lds = SCORE ? kScorePassLds : 0;
   if (edge) {
     k_dp16<G, true, false, SCORE><<<grid, kBlock, 0, st>>>(a);
   } else if (persist) {
     if constexpr (G <= kPersistMaxGap) {
       grid = std::max<uint32_t>(1, std::min(grid, grid_cap));
-      k_dp16<G, false, true, SCORE><<<grid, kBlock, 0, st>>>(a);
+      k_dp16<G, false, true, SCORE><<<grid, kBlock, lds, st>>>(a);
     } else {
       throw ClastError(1, "persistent 16-bit DP schedule over its gap limit");
     }
   } else {
-    k_dp16<G, false, false, SCORE><<<grid, kBlock, 0, st>>>(a);
+    k_dp16<G, false, false, SCORE><<<grid, kBlock, lds, st>>>(a);
   }
 }
 
