@@ -2258,6 +2258,7 @@ __device__ __forceinline__ int pick(int m, int a, int b) { return b ^ ((a ^ b) &
 struct BandTask {
   int rows;
   bool clean;     // query strand and target sequence without star or X letters
+  bool inwin;     // every target position the task's lane W-1 reads lies in [lo, hi)
   int r;          // this lane's row at its next step of this task
   int ti, ts;     // target index this lane reads next (lane W-1) and its step
   int lo, hi;     // t_mask range
@@ -2284,8 +2285,10 @@ __device__ __forceinline__ void row_words_sel(uint32_t qc, uint32_t& wpm, uint32
 // class is 0-4 and the target class 0-4 or 'X' outside the window): the
 // substitution is +1 / match 1 for equal classes below 4, -1000 / -1000 for
 // an 'X' target, else -3 / 0 — krnlMatrix.cuh's entries for those classes,
-// without the row words.
-template <int G, bool LOCAL, bool CLEAN>
+// without the row words.  AM false: every lane is inside its task's rows
+// (the wavefront's steady steps), so no lane shows its row -1 values; WIN
+// false: every target position read lies in the window (no 'X' mask).
+template <int G, bool LOCAL, bool CLEAN, bool AM = true, bool WIN = true>
 __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, int Vinit, int Minit, int& eS,
                                           int& eV, int& eH, int& eM, uint32_t& eT, uint32_t& eQ) {
   constexpr int W = 2 * G + 1;
@@ -2300,7 +2303,7 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   const uint32_t sb = X.st.next(j);
   const bool bot = b == 0, top = b == W - 1;
   const uint32_t qc = bot ? sb : lQ;
-  const uint32_t tm = 4u * (((X.ti >= X.lo) & (X.ti < X.hi)) ? sb : kClsX);  // t_mask
+  const uint32_t tm = WIN ? 4u * (((X.ti >= X.lo) & (X.ti < X.hi)) ? sb : kClsX) : 4u * sb;  // t_mask
   const uint32_t T = top ? tm : rT;
   X.ti += X.ts;
   const int v0 = max(rS + (kGapOpen + kGapExt), rV + kGapExt);
@@ -2332,20 +2335,35 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   const int dmask = -static_cast<int>(pt > vh);
   const int nm = nm_gap ^ ((nm_gap ^ nm_diag) & dmask);
   const int ns = max(pt, vh);
-  const int am = -static_cast<int>(static_cast<uint32_t>(X.r) < static_cast<uint32_t>(X.rows));
-  if (LOCAL) {
-    const int bm = am & -static_cast<int>(X.fS < ns);
-    X.fS = pick(bm, ns, X.fS);
-    X.fM = pick(bm, nm, X.fM);
-    X.fR = pick(bm, X.r, X.fR);
+  if constexpr (AM) {
+    const int am = -static_cast<int>(static_cast<uint32_t>(X.r) < static_cast<uint32_t>(X.rows));
+    if (LOCAL) {
+      const int bm = am & -static_cast<int>(X.fS < ns);
+      X.fS = pick(bm, ns, X.fS);
+      X.fM = pick(bm, nm, X.fM);
+      X.fR = pick(bm, X.r, X.fR);
+    }
+    X.S = pick(am, ns, X.S);
+    X.M = pick(am, nm, X.M);
+    // before its first row a lane shows its row -1 (k_dp's initial band)
+    eS = pick(am, ns, Sinit);
+    eV = pick(am, v, Vinit);
+    eM = pick(am, nm, Minit);
+    eT = static_cast<uint32_t>(pick(am, static_cast<int>(T), static_cast<int>(X.t0)));
+  } else {
+    if (LOCAL) {
+      const int bm = -static_cast<int>(X.fS < ns);
+      X.fS = pick(bm, ns, X.fS);
+      X.fM = pick(bm, nm, X.fM);
+      X.fR = pick(bm, X.r, X.fR);
+    }
+    X.S = ns;
+    X.M = nm;
+    eS = ns;
+    eV = v;
+    eM = nm;
+    eT = T;
   }
-  X.S = pick(am, ns, X.S);
-  X.M = pick(am, nm, X.M);
-  // before its first row a lane shows its row -1 (k_dp's initial band)
-  eS = pick(am, ns, Sinit);
-  eV = pick(am, v, Vinit);
-  eM = pick(am, nm, Minit);
-  eT = static_cast<uint32_t>(pick(am, static_cast<int>(T), static_cast<int>(X.t0)));
   eH = h;
   eQ = qc;
   X.r += 1;
@@ -2402,6 +2420,11 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
       // lane W-1 reads position r + W - 1 of row r: its stream starts at
       // position G, so stream index j = floor(step / 2) in every lane
       x.ti = tb + x.ts * G;
+      // lane W-1 reads positions ti, ti + ts, ... for at most rows + W + 1 calls
+      {
+        const int64_t a0 = static_cast<int64_t>(x.ti), a1 = a0 + static_cast<int64_t>(x.ts) * (x.rows + W + 1);
+        x.inwin = min(a0, a1) >= x.lo && max(a0, a1) < x.hi;
+      }
       const bool top = b == W - 1;
       x.st.init(top ? a.tbase : a.qbase, top ? a.t_bmax : a.q_bmax,
                 top ? tslot + x.ti : static_cast<int64_t>(a.qslot[s]) + (bwd ? qIdx + L : qIdx - 1), x.ts);
@@ -2426,21 +2449,61 @@ static __global__ __launch_bounds__(kBlock) void k_dp_band(DPArgs a) {
     const bool odd = (b & 1) != 0;
     const int rowsA = odd ? X1.rows : X0.rows, rowsB = odd ? X0.rows : X1.rows;
     const int tend = live ? max(2 * (rowsA - 1) + W - 1, 2 * (rowsB - 1) + W) + 1 : 0;
-      // lanes without a pair ran task 0's setup: they count as clean
-    if (__all(!live || (X0.clean && X1.clean))) {
-#pragma unroll 1
-      for (int t = 0; t < tend; t += 2) {
-        const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
-        band_step<G, LOCAL, true>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-        band_step<G, LOCAL, true>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+    // the wavefront's steady steps: calls [i0, i1) find every lane inside
+    // both its tasks' rows (call i of task X is its row X.r + i); they run
+    // four calls at a time without the row -1 picks (and without the window
+    // mask when every task's reads stay in its window), the calls before and
+    // after them one at a time
+    // (a wavefront with no pair has none; lanes without a pair count only
+    // through the wavefront's largest call count, which bounds i1)
+    int i0 = 0, i1 = 0;
+    {
+      const int lo = live ? max(-X0.r, -X1.r) : 0;
+      const int hi = live ? min(X0.rows - X0.r, X1.rows - X1.r) : INT32_MAX;
+      const int mlo = __builtin_amdgcn_readlane(wave_max_scan_incl(lo), 63);
+      const int mhi = -__builtin_amdgcn_readlane(wave_max_scan_incl(-hi), 63);
+      const int mcalls = __builtin_amdgcn_readlane(wave_max_scan_incl((tend + 1) / 2), 63);
+      const int cap = min(mhi, mcalls);
+      if (__any(live)) {
+        i0 = (mlo + 3) & ~3;
+        i1 = cap > i0 ? i0 + ((cap - i0) & ~3) : i0;
       }
+    }
+    // lanes without a pair ran task 0's setup: they count as clean and in the window
+    const bool clean = __all(!live || (X0.clean && X1.clean));
+    const bool inwin = __all(!live || (X0.inwin && X1.inwin));
+    auto run = [&](auto cleanc, auto winc) {
+      constexpr bool C = decltype(cleanc)::value, WN = decltype(winc)::value;
+      // every lane runs the calls before i0 (lanes without a pair compute
+      // on task 0 and store nothing); then the steady calls; then each lane
+      // to its own end
+#pragma unroll 1
+      for (int i = 0; i < i0; ++i) {
+        const int j = __builtin_amdgcn_readfirstlane(i);
+        band_step<G, LOCAL, C>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+        band_step<G, LOCAL, C>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      }
+#pragma unroll 1
+      for (int i = i0; i < i1; i += 4) {
+        const int j4 = (__builtin_amdgcn_readfirstlane(i) >> 2) << 2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          band_step<G, LOCAL, C, false, WN>(X0, b, j4 + u, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+          band_step<G, LOCAL, C, false, WN>(X1, b, j4 + u, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+        }
+      }
+#pragma unroll 1
+      for (int t = 2 * i1; t < tend; t += 2) {
+        const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
+        band_step<G, LOCAL, C>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+        band_step<G, LOCAL, C>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
+      }
+    };
+    if (clean) {
+      if (inwin) run(std::true_type{}, std::false_type{});
+      else run(std::true_type{}, std::true_type{});
     } else {
-#pragma unroll 1
-      for (int t = 0; t < tend; t += 2) {
-        const int j = __builtin_amdgcn_readfirstlane(t) >> 1;
-        band_step<G, LOCAL, false>(X0, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-        band_step<G, LOCAL, false>(X1, b, j, Sinit, Vinit, Minit, eS, eV, eH, eM, eT, eQ);
-      }
+      run(std::false_type{}, std::true_type{});
     }
     // end node per task: the group's lanes through LDS, scanned in band order
     const int wv = wave_id();
