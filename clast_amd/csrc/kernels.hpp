@@ -2260,7 +2260,7 @@ struct BandTask {
   bool clean;     // query strand and target sequence without star or X letters
   bool inwin;     // every target position the task's lane W-1 reads lies in [lo, hi)
   int r;          // this lane's row at its next step of this task
-  int ti, ts;     // target index this lane reads next (lane W-1) and its step
+  int ti, ts;     // target index lane W-1 reads at stream position 0, and its step
   int lo, hi;     // t_mask range
   int S, M;       // this lane's cell of the previous row
   int fS, fM, fR; // local: best cell of this band (score, match, row); global: unused
@@ -2303,9 +2303,9 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   const uint32_t sb = X.st.next(j);
   const bool bot = b == 0, top = b == W - 1;
   const uint32_t qc = bot ? sb : lQ;
-  const uint32_t tm = WIN ? 4u * (((X.ti >= X.lo) & (X.ti < X.hi)) ? sb : kClsX) : 4u * sb;  // t_mask
+  const int ti = X.ti + X.ts * j;  // the target index of stream position j
+  const uint32_t tm = WIN ? 4u * (((ti >= X.lo) & (ti < X.hi)) ? sb : kClsX) : 4u * sb;  // t_mask
   const uint32_t T = top ? tm : rT;
-  X.ti += X.ts;
   const int v0 = max(rS + (kGapOpen + kGapExt), rV + kGapExt);
   const int h0 = max(lS + (kGapOpen + kGapExt), lH + kGapExt);
   const int v = top ? max(kTopS + (kGapOpen + kGapExt), kTopV + kGapExt) : v0;
@@ -2316,7 +2316,7 @@ __device__ __forceinline__ void band_step(BandTask& X, int b, int j, int Sinit, 
   if constexpr (CLEAN) {
     const uint32_t tcl = T >> 2;
     const bool eq = (qc == tcl) & (qc < 4u);
-    const bool bad = tcl == kClsX;
+    const bool bad = WIN && tcl == kClsX;  // (no 'X' when every read is in the window)
     pm = bad ? kBad : eq ? kMatch : kMismatch;
     mc = bad ? kBad : eq ? 1 : 0;
   } else {
