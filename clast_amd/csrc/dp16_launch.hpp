@@ -5,14 +5,17 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "tuner.hpp"
 
 namespace clast {
 
-// LDS the score pass's workgroups reserve (they use none): with 160 KiB per
-// CU at most three fit, so three of a SIMD's four register-file quarters go
-// to the score pass and the fourth stays open for the kernels beside it
-// (the second pass, past-the-end tasks, one lane per band, int32), which
-// otherwise wait for score-pass wavefronts of other searches to finish.
+// LDS a large search's score-pass workgroups reserve (they use none): with
+// 160 KiB per CU at most three fit, so three of a SIMD's four register-file
+// quarters go to large score passes and the fourth stays open for the
+// kernels beside them and for small searches (the second pass, past-the-end
+// tasks, one lane per band, int32, the score passes of searches under
+// kSmallTasks tasks), which otherwise wait for the large score passes'
+// wavefronts to finish.
 constexpr size_t kScorePassLds = 41 * 1024;
 static_assert(kScorePassLds * 4 > 160 * 1024 && kScorePassLds * 3 <= 160 * 1024, "three workgroups per CU");
 
@@ -22,7 +25,7 @@ static_assert(kScorePassLds * 4 > 160 * 1024 && kScorePassLds * 3 <= 160 * 1024,
 template <int G, bool SCORE>
 static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
   uint32_t grid = grid_for((static_cast<uint64_t>(a.n) + 1) / 2, kBlock);
-  const size_t lds = SCORE ? kScorePassLds : 0;
+  const size_t lds = SCORE && dp_size_class(a.n) == 1 ? kScorePassLds : 0;
   if (edge) {
     k_dp16<G, true, false, SCORE><<<grid, kBlock, 0, st>>>(a);
   } else if (persist) {
