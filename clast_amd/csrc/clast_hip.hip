@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "score_threshold.hpp"
 #include "prims.hpp"
 #include "runtime.hpp"
 #include "tuner.hpp"
@@ -146,34 +147,9 @@ static void check_window(const clast_window& w) {
   if (w.n_seq == 0 && w.n_slots != 0) throw ClastError(CLAST_ERR_ARG, "window slots without sequences");
 }
 
-// Score threshold of one query length: the smallest integer score s with
-// !(cutOff < K * D * qLen * exp(-lambda * s)) (…deleteBadHits.cuh:32-39
-// with krnlCalculateEvalue.cu:21), decided by that exact expression;
-// INT_MAX when no score passes, INT_MIN when every score does.  The search
-// starts from the closed form ceil(ln(K D qLen / cutOff) / lambda) and
-// steps to the boundary (keep(s) and not keep(s - 1)); a guess that is not
-// finite or misses by more than a few steps falls back to bisection.
+// Score threshold of one query length (score_threshold.hpp).
 static int32_t score_threshold(const clast_params& p, int qlen) {
-  auto keep = [&](int64_t s) {
-    const double e = p.K * p.total_db_size * qlen * std::exp(-p.lambda * static_cast<double>(s));
-    return !(p.cut_off < e);
-  };
-  int64_t lo = -(1ll << 30), hi = (1ll << 30);
-  if (!keep(hi)) return INT32_MAX;
-  if (keep(lo)) return INT32_MIN;
-  const double g = std::ceil(std::log(p.K * p.total_db_size * qlen / p.cut_off) / p.lambda);
-  if (std::isfinite(g) && g > static_cast<double>(lo + 8) && g < static_cast<double>(hi - 8)) {
-    int64_t s = static_cast<int64_t>(g);
-    for (int i = 0; i < 4 && !keep(s); ++i) ++s;
-    for (int i = 0; i < 4 && keep(s - 1); ++i) --s;
-    if (keep(s) && !keep(s - 1)) return static_cast<int32_t>(s);
-  }
-  while (hi - lo > 1) {
-    const int64_t mid = lo + (hi - lo) / 2;
-    if (keep(mid)) hi = mid;
-    else lo = mid;
-  }
-  return static_cast<int32_t>(hi);
+  return score_threshold(p.K, p.total_db_size, p.cut_off, p.lambda, qlen);
 }
 
 }  // namespace clast

@@ -361,3 +361,18 @@ def test_poisson_requests_partition_reads():
     assert np.all(np.diff(arr) > 0)
     # mean inter-arrival close to 1/rate
     assert 0.01 < float(np.mean(np.diff(arr))) < 0.04
+
+
+def test_score_threshold_equals_bisection(tmp_path):
+    """The per-length score threshold of the E-value cut (closed-form start,
+    stepped to the exact boundary) against plain bisection on the same
+    expression, over a grid of K, lambda, database size, cut-off and query
+    length, with cut-offs exactly on a score's E-value, and every length up
+    to 20000 at the reference defaults (tests/threshold_check.cpp)."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = str(tmp_path / "threshold_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(here, "threshold_check.cpp"), "-o", exe], check=True)
+    res = subprocess.run([exe], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout
+    m = re.search(r"cases (\d+) mismatches 0", res.stdout)
+    assert m and int(m.group(1)) > 20000, res.stdout
