@@ -1042,6 +1042,10 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
     // (the allocation-free entry point and the stage read-back run one pass)
     sp.on = packed && use_cut && !keep_stages && !rows_out && DPTuner::get().two_pass(tkey, forced.two_pass);
     auto* const launch16 = sp.on ? launch_dp16_score : launch_dp16;
+    // a large score pass with long extensions holds its slots for long:
+    // it leaves a quarter of them to the kernels beside it and to other
+    // searches (searches of short reads keep the full occupancy)
+    const bool reserve16 = dp_size_class(ntask) == 1 && maxrows >= 256;
     uint32_t* split = ws.get<uint32_t>("d.split", 3);
     int* band_rows = ws.get<int>("d.bandrows", 1);
     uint32_t* chunk_next = ws.get<uint32_t>("d.chunk", 1);
@@ -1144,13 +1148,13 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
           fmark_on(EV_DBAND1, ss);
         }
         fmark_on(EV_E16_0, ss);
-        launch16(p.gap, true, false, 0, da, ss);
+        launch16(p.gap, true, false, 0, false, da, ss);
         fmark_on(EV_E16_1, ss);
         fmark_on(EV_D32_0, ss);
         launch_dp_global(p.gap, da, ss);
         fmark_on(EV_D32_1, ss);
         fmark(EV_D16_0);
-        launch16(p.gap, false, c.sched == 1, 4 * cu, da, st);
+        launch16(p.gap, false, c.sched == 1, 4 * cu, reserve16, da, st);
         fmark(EV_D16_1);
         rt.join(0);
       } else {
@@ -1329,11 +1333,11 @@ void run_search(const clast_params& p, clast_index_s& ix, clast_query_s& q, int 
       }
       if (p2n[1]) {
         d2.n = p2n[1];
-        launch_dp16(p.gap, false, false, 0, d2, ss);
+        launch_dp16(p.gap, false, false, 0, false, d2, ss);
       }
       if (p2n[2]) {
         d2.n = p2n[2];
-        launch_dp16(p.gap, true, false, 0, d2, ss);
+        launch_dp16(p.gap, true, false, 0, false, d2, ss);
       }
       k_match_patch<<<g, kBlock, 0, ss>>>(nhit, kflag, dres, hs, hq, q.qlen.ptr, p.lmer, om);
       mark_on(EV_P2_1, ss);

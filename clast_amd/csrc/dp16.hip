@@ -5,8 +5,8 @@
 
 namespace clast {
 
-void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
-  launch_dp16_any<false>(gap, edge, persist, grid_cap, a, st);
+void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, bool, const DPArgs& a, hipStream_t st) {
+  launch_dp16_any<false>(gap, edge, persist, grid_cap, false, a, st);
 }
 
 }  // namespace clast

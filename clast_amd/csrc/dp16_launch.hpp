@@ -5,17 +5,14 @@
 #include <algorithm>
 
 #include "kernels.hpp"
-#include "tuner.hpp"
 
 namespace clast {
 
-// LDS a large search's score-pass workgroups reserve (they use none): with
+// LDS a score pass's workgroups reserve when asked (they use none): with
 // 160 KiB per CU at most three fit, so three of a SIMD's four register-file
-// quarters go to large score passes and the fourth stays open for the
-// kernels beside them and for small searches (the second pass, past-the-end
-// tasks, one lane per band, int32, the score passes of searches under
-// kSmallTasks tasks), which otherwise wait for the large score passes'
-// wavefronts to finish.
+// quarters go to that score pass and the fourth stays open for the kernels
+// beside it and for other searches' small launches, which otherwise wait
+// for its long-lived wavefronts to finish.
 constexpr size_t kScorePassLds = 41 * 1024;
 static_assert(kScorePassLds * 4 > 160 * 1024 && kScorePassLds * 3 <= 160 * 1024, "three workgroups per CU");
 
@@ -23,9 +20,10 @@ static_assert(kScorePassLds * 4 > 160 * 1024 && kScorePassLds * 3 <= 160 * 1024,
 // kernel's range is a part of them); persistent: at most grid_cap
 // workgroups (about one occupancy round), which loop over the chunks.
 template <int G, bool SCORE>
-static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
+static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, bool reserve, const DPArgs& a,
+                            hipStream_t st) {
   uint32_t grid = grid_for((static_cast<uint64_t>(a.n) + 1) / 2, kBlock);
-  const size_t lds = SCORE && dp_size_class(a.n) == 1 ? kScorePassLds : 0;
+  const size_t lds = SCORE && reserve ? kScorePassLds : 0;
   if (edge) {
     k_dp16<G, true, false, SCORE><<<grid, kBlock, 0, st>>>(a);
   } else if (persist) {
@@ -41,23 +39,24 @@ static void launch_dp16_gap(bool edge, bool persist, uint32_t grid_cap, const DP
 }
 
 template <bool SCORE>
-static void launch_dp16_any(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
+static void launch_dp16_any(int gap, bool edge, bool persist, uint32_t grid_cap, bool reserve, const DPArgs& a,
+                            hipStream_t st) {
   switch (gap) {
-    case 2: launch_dp16_gap<2, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 3: launch_dp16_gap<3, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 4: launch_dp16_gap<4, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 5: launch_dp16_gap<5, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 6: launch_dp16_gap<6, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 7: launch_dp16_gap<7, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 8: launch_dp16_gap<8, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 9: launch_dp16_gap<9, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 10: launch_dp16_gap<10, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 11: launch_dp16_gap<11, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 12: launch_dp16_gap<12, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 13: launch_dp16_gap<13, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 14: launch_dp16_gap<14, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 15: launch_dp16_gap<15, SCORE>(edge, persist, grid_cap, a, st); break;
-    case 16: launch_dp16_gap<16, SCORE>(edge, persist, grid_cap, a, st); break;
+    case 2: launch_dp16_gap<2, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 3: launch_dp16_gap<3, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 4: launch_dp16_gap<4, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 5: launch_dp16_gap<5, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 6: launch_dp16_gap<6, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 7: launch_dp16_gap<7, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 8: launch_dp16_gap<8, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 9: launch_dp16_gap<9, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 10: launch_dp16_gap<10, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 11: launch_dp16_gap<11, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 12: launch_dp16_gap<12, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 13: launch_dp16_gap<13, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 14: launch_dp16_gap<14, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 15: launch_dp16_gap<15, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
+    case 16: launch_dp16_gap<16, SCORE>(edge, persist, grid_cap, reserve, a, st); break;
     default: throw ClastError(1, "gap out of range");
   }
 }

@@ -5,8 +5,9 @@
 
 namespace clast {
 
-void launch_dp16_score(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st) {
-  launch_dp16_any<true>(gap, edge, persist, grid_cap, a, st);
+void launch_dp16_score(int gap, bool edge, bool persist, uint32_t grid_cap, bool reserve, const DPArgs& a,
+                       hipStream_t st) {
+  launch_dp16_any<true>(gap, edge, persist, grid_cap, reserve, a, st);
 }
 
 }  // namespace clast

@@ -2571,8 +2571,11 @@ void launch_dp_local(int gap, const DPArgs& a, hipStream_t st);
 // k_dp16 over order[split[0], split[1]) and, with edge, over
 // order[split[1], split[2]) (dp16.hip; its score-only form, dp16_score.hip),
 // k_dp_band over order[0, split[0]) (dp_band.hip); a.n bounds the task count.
-void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st);
-void launch_dp16_score(int gap, bool edge, bool persist, uint32_t grid_cap, const DPArgs& a, hipStream_t st);
+// reserve: the score pass's workgroups reserve LDS so that at most three fit
+// per CU (dp16_launch.hpp); launch_dp16 ignores it.
+void launch_dp16(int gap, bool edge, bool persist, uint32_t grid_cap, bool reserve, const DPArgs& a, hipStream_t st);
+void launch_dp16_score(int gap, bool edge, bool persist, uint32_t grid_cap, bool reserve, const DPArgs& a,
+                       hipStream_t st);
 void launch_dp_band(int gap, bool local, const DPArgs& a, hipStream_t st);
 
 // DP task order by class (key bits from classbit up): 0 the long tasks
