@@ -1831,6 +1831,12 @@ struct PairRows {
     }
     return v;
   }
+  // past the four rows of p0 without reading them
+  __device__ __forceinline__ void skip4() {
+    p0 = p1;
+    p1 = p2;
+    p2 = p3;
+  }
 };
 
 // One query row (k_dp's dp_row, global mode) for two tasks.  Z holds S - 7:
@@ -1908,13 +1914,22 @@ __device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W +
 // class in byte 0, half 1's as 4 + class in byte 2, 12 in bytes 1 and 3 for
 // zero high bytes).  Target classes 0..3 only (class 4 would select py's
 // byte 0): the task's target sequence holds no class 4.
-template <int W, int O>
+// FIRST: the row's vertical terms go first, before anything that reads the
+// profiles (they may still be in flight from ds_bpermute).
+template <int W, int O, bool FIRST = false>
 __device__ __forceinline__ void dp_row16_prof(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
                                               uint32_t px, uint32_t py, s16x2 h1, s16x2 vw) {
+  if constexpr (FIRST) {
+    // in place, rising b: V[b + 1] and Z[b + 1] are still the last row's
+#pragma unroll
+    for (int b = 1; b < W; ++b) V[b] = __builtin_elementwise_max(Z[b + 1], V[b + 1]);
+    V[W] = vw;
+    __builtin_amdgcn_sched_barrier(0);
+  }
   s16x2 hp = h1;
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
-    const s16x2 v = b == W ? vw : __builtin_elementwise_max(Z[b + 1], V[b + 1]);
+    const s16x2 v = FIRST ? V[b] : b == W ? vw : __builtin_elementwise_max(Z[b + 1], V[b + 1]);
     const s16x2 h = b == 1 ? h1 : __builtin_elementwise_max(Z[b - 1], hp);
     const s16x2 pt = Z[b] + pk_val(__builtin_amdgcn_perm(py, px, T[O + b - 1]));
     Z[b] = __builtin_elementwise_max(pt, __builtin_elementwise_max(v, h)) + pk_splat(-5);
@@ -1930,6 +1945,19 @@ template <int K>
 __device__ __forceinline__ uint32_t query_profile(uint32_t Q) {
   const uint32_t sel = __builtin_amdgcn_perm(Q, Q, K * 0x01010101u) ^ 0x03020100u;
   return __builtin_amdgcn_perm(0x06060606u, 0x0606060Au, sel);
+}
+
+// The same profiles as a table across the lanes: lane l holds the profile of
+// class l mod 16 (classes 4 and up: no byte at 10, as class 7 above), so
+// ds_bpermute reads the profile of the class in bits 2-5 of its byte address
+// (it takes the source lane from address bits 2-7, the wavefront's 64
+// lanes; bits 6-7 land on a copy of the table).
+__device__ __forceinline__ uint32_t query_profile_table() {
+  const uint32_t q = threadIdx.x & 15u;
+  return 0x06060606u + (q < 4u ? 4u << (8u * q) : 0u);
+}
+__device__ __forceinline__ uint32_t query_profile_lookup(uint32_t addr, uint32_t table) {
+  return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(addr), static_cast<int>(table)));
 }
 
 // End node of the last row (k_dp's global selection: strict '<' from the
@@ -2095,21 +2123,44 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
       dp16_end<G, SCORE>(Z, M, a.dres, task, h, rws);
     }
   };
+  // PROF with U = 4: the four rows' query profiles from the lane table,
+  // looked up together before the rows (px / py: halves 0 / 1)
+  constexpr bool kTab = PROF && U == 4;
+  const uint32_t qtab = kTab ? query_profile_table() : 0u;
+  uint32_t px[4], py[4];
+  // rows 0-3 of w (a PairRows word): row i's classes in bits 4 i (half 0)
+  // and 16 + 4 i (half 1)
+  auto lookup4 = [&](uint32_t w) {
+    px[0] = query_profile_lookup(w << 2, qtab);
+    py[0] = query_profile_lookup(w >> 14, qtab);
+    px[1] = query_profile_lookup(w >> 2, qtab);
+    py[1] = query_profile_lookup(w >> 18, qtab);
+    px[2] = query_profile_lookup(w >> 6, qtab);
+    py[2] = query_profile_lookup(w >> 22, qtab);
+    px[3] = query_profile_lookup(w >> 10, qtab);
+    py[3] = query_profile_lookup(w >> 26, qtab);
+  };
   auto row = [&](auto oi, int j, int r) {
     constexpr int i = decltype(oi)::value;
-    const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
-    T[W + i] = tp.next(j);          // target position r + W + kLead = r (mod 16)
+    // target position r + W + kLead = r (mod 16) (kTab: set before the rows)
+    if constexpr (!kTab) T[W + i] = tp.next(j);
     if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
-    if constexpr (PROF) {
-      T[W + i] |= 0x0C040C00u;
+    if constexpr (kTab) {
       const int d4 = 4 * (r & (kDriftRows - 1));
-      dp_row16_prof<W, i>(Z, V, T, query_profile<0>(Q), query_profile<2>(Q), pk_splat(kH1 + 2 + d4),
-                          pk_splat(kVW + 2 * W + d4));
-    } else if constexpr (DRIFT) {
-      const int d4 = 4 * (r & (kDriftRows - 1));
-      dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
+      dp_row16_prof<W, i, i == 0>(Z, V, T, px[i], py[i], pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
     } else {
-      dp_row16<W, i, EDGE, SCORE>(Z, V, M, T, Q);
+      const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
+      if constexpr (PROF) {
+        T[W + i] |= 0x0C040C00u;
+        const int d4 = 4 * (r & (kDriftRows - 1));
+        dp_row16_prof<W, i>(Z, V, T, query_profile<0>(Q), query_profile<2>(Q), pk_splat(kH1 + 2 + d4),
+                            pk_splat(kVW + 2 * W + d4));
+      } else if constexpr (DRIFT) {
+        const int d4 = 4 * (r & (kDriftRows - 1));
+        dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
+      } else {
+        dp_row16<W, i, EDGE, SCORE>(Z, V, M, T, Q);
+      }
     }
     if (r + 1 == ev) {
       if (rows[0] == ev) end_node(task0, 0, ev);
@@ -2136,6 +2187,14 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
       if constexpr (U == 1) {
         row(std::integral_constant<int, 0>{}, ju, r);
       } else {
+        if constexpr (kTab) {
+          // the four rows' query profiles (qp.p0 holds their classes), then
+          // their target positions, while the lookups are under way
+          lookup4(qp.p0);
+          qp.skip4();
+#pragma unroll
+          for (int i = 0; i < 4; ++i) T[W + i] = tp.next(i) | 0x0C040C00u;
+        }
         row(std::integral_constant<int, 0>{}, 0, r);
         if (r + 1 >= rmax) break;
         row(std::integral_constant<int, 1>{}, 1, r + 1);
