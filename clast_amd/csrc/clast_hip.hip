@@ -171,7 +171,7 @@ struct clast_index_s {
   DevArray<int32_t> seq_len, seq_win, seq_ref;
   DevArray<int2> seq_info;  // (length, window) per target sequence
   DevArray<uint8_t> seq_clean;  // 1: no star / X class in the sequence (16-bit DP), 3: no class 4 either
-  DevArray<uint32_t> tnib;      // 4-bit copy of tbase (16-bit DP)
+  DevArray<uint32_t> tnib;      // 4-bit copy of tbase and its mirror (16-bit DP)
   uint32_t ncell = 0;
   DevArray<uint32_t> ekey32;
   DevArray<uint64_t> ekey64;
@@ -194,7 +194,7 @@ struct clast_query_s {
   DevArray<uint32_t> qslot, pos_off, pos_strand;
   DevArray<int32_t> qlen, qref, qwin, smin;
   DevArray<uint8_t> qclean;  // no star / X class in the strand (16-bit DP)
-  DevArray<uint32_t> qnib;   // 4-bit copy of qbase, other -> 7 (16-bit DP)
+  DevArray<uint32_t> qnib;   // 4-bit copy of qbase, other -> 7, and its mirror (16-bit DP)
   std::vector<int32_t> h_qlen, h_qref, h_qwin, h_id_begin;
   bool smin_valid = false;
   double smin_key[4] = {0, 0, 0, 0};
@@ -369,7 +369,7 @@ static void target_tables(Runtime& rt, clast_index_s& ix, const std::vector<uint
   ix.seq_ref.upload(ix.h_seq_ref.data(), ix.h_seq_ref.size(), st);
   {
     const uint64_t ndw = 2 * (static_cast<uint64_t>(ix.bmax) + 1);
-    ix.tnib.alloc(ndw);
+    ix.tnib.alloc(2 * ndw);  // with its mirror
     k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(ix.tbase.ptr, ndw, 0, ix.tnib.ptr);
   }
   ix.seq_clean.alloc(std::max(ix.nseq, 1));
@@ -521,7 +521,7 @@ static void query_finish(Runtime& rt, clast_query_s& q, const QueryLayout& L) {
   q.qwin.upload(q.h_qwin.data(), q.h_qwin.size(), st);
   {
     const uint64_t ndw = 2 * (static_cast<uint64_t>(q.bmax) + 1);
-    q.qnib.alloc(ndw);
+    q.qnib.alloc(2 * ndw);  // with its mirror
     k_pack_nibbles<<<grid_for(ndw, kBlock), kBlock, 0, st>>>(q.qbase.ptr, ndw, 1, q.qnib.ptr);
   }
   q.qclean.alloc(std::max<uint64_t>(q.nseq, 1));

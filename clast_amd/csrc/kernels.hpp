@@ -1404,7 +1404,7 @@ struct DPArgs {
   // [split[1], split[2]) to k_dp16 with 'X' past the sequence end, the rest
   // to k_dp
   const uint32_t* split;
-  const uint32_t* qnib;    // 4-bit copies of qbase / tbase (k_pack_nibbles), read by k_dp16
+  const uint32_t* qnib;    // 4-bit copies of qbase / tbase with mirrors (k_pack_nibbles), read by k_dp16
   const uint32_t* tnib;
   int4* dres;              // per task 2 * hit + direction: (move + end, move, match, score)
   int32_t q_bmax, t_bmax;  // last 16-byte block of qbase / tbase (both padded to whole blocks)
@@ -1736,6 +1736,16 @@ __device__ __forceinline__ s16x2 pk_mad4(s16x2 m, s16x2 z) {
 // of the byte class array (padded to whole 16-byte blocks, so whole 8-byte
 // units here).  The query copy maps class 4 (other) to 7, which matches no
 // target class (classes 0..4 in the bands k_dp16 runs).
+// Nibbles of a dword in reverse order.
+__device__ __forceinline__ uint32_t nibble_reverse(uint32_t x) {
+  x = __builtin_amdgcn_perm(0u, x, 0x00010203u);
+  return ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
+}
+
+// nib holds 2 ndw dwords: the classes in slot order (dword d: slots 8 d ..
+// 8 d + 7, slot 8 d + k in bits 4 k), then the same classes mirrored (slot
+// 8 ndw + k holds slot 8 ndw - 1 - k), so a reader going down the slots
+// reads the mirror upwards (NibWin).
 static __global__ __launch_bounds__(kBlock) void k_pack_nibbles(const uint8_t* __restrict__ bytes, uint64_t ndw,
                                                                 int query, uint32_t* __restrict__ nib) {
   const uint64_t d = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -1751,58 +1761,54 @@ static __global__ __launch_bounds__(kBlock) void k_pack_nibbles(const uint8_t* _
   // byte k's nibble to bits 4k: pairs of bytes first, then the pairs
   lo |= lo >> 4;
   hi |= hi >> 4;
-  nib[d] = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
-}
-
-// Nibbles of a dword in reverse order.
-__device__ __forceinline__ uint32_t nibble_reverse(uint32_t x) {
-  x = __builtin_amdgcn_perm(0u, x, 0x00010203u);
-  return ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
+  const uint32_t w = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+  nib[d] = w;
+  nib[2 * ndw - 1 - d] = nibble_reverse(w);
 }
 
 // Sequential reader of one half's slots g, g + step, g + 2 step, ... of a
-// 4-bit class array in groups of 16, on a schedule shared by the wavefront
-// (every lane is at the same group): a group's window spans two aligned
-// 16-slot units held in r0 / r1; after group k the one of them that no later
-// group reads (r[k & 1]) is reloaded with the unit after the other, so each
-// load is requested 16 rows before its use and lands in its own register.
-// Units outside [0, umax] are clamped; their slots are never used.
+// 4-bit class array with its mirror (k_pack_nibbles; umax: its last 16-slot
+// unit, mirror included) in groups of 16, on a schedule shared by the
+// wavefront (every lane is at the same group).  A reader going down the
+// slots (step -1) goes up the mirror, so every reader moves up: a group's
+// window spans two aligned 16-slot units held in r0 / r1; after group k the
+// one of them that no later group reads (r[k & 1]) is reloaded with the unit
+// after the other, so each load is requested 16 rows before its use and
+// lands in its own register.  Units outside [0, umax] are clamped; their
+// slots are never used (nor are the other half's slots a reader may reach
+// past its own end).
 struct NibWin {
   int32_t u;       // unit requested next
-  int32_t step;    // +1 / -1 (memory direction)
   uint32_t s4;     // bit offset of the window inside its first unit
   uint2 r0, r1;
 
   __device__ __forceinline__ static uint2 load(const uint2* base, int32_t i, int32_t umax) {
     return base[min(max(i, 0), umax)];
   }
-  __device__ __forceinline__ void init(const uint2* base, int32_t umax, int64_t g0, int st) {
-    step = st;
-    const int64_t lo = st > 0 ? g0 : g0 - 15;  // first group's lowest slot
+  __device__ __forceinline__ void init(const uint2* base, int32_t umax, int64_t g0, int step) {
+    const int64_t nslot = 8 * (static_cast<int64_t>(umax) + 1);  // slots of each half
+    // the first group's first slot (slot g's mirror: 2 nslot - 1 - g)
+    const int64_t lo = step > 0 ? g0 : 2 * nslot - 1 - g0;
     const int32_t u0 = static_cast<int32_t>(lo >> 4);
     s4 = static_cast<uint32_t>(lo & 15) * 4u;
-    // group 0: forward reloads its low unit (r0) next, backward its high one
-    r0 = load(base, st > 0 ? u0 : u0 + 1, umax);
-    r1 = load(base, st > 0 ? u0 + 1 : u0, umax);
-    u = st > 0 ? u0 + 2 : u0 - 1;
+    r0 = load(base, u0, umax);
+    r1 = load(base, u0 + 1, umax);
+    u = u0 + 2;
   }
   // Group k, k & 1 == ODD: its 16 classes in stream order, rows 0-7 in w0
   // and 8-15 in w1 (row j in bits 4 (j & 7)).
   template <int ODD>
   __device__ __forceinline__ void group(const uint2* base, int32_t umax, uint32_t& w0, uint32_t& w1) {
-    const uint2 a = ODD ? r1 : r0;  // reloaded below
-    const uint2 b = ODD ? r0 : r1;
-    const bool fwd = step > 0;
-    const uint2 lo = fwd ? a : b, hi = fwd ? b : a;
+    const uint2 lo = ODD ? r1 : r0;  // reloaded below
+    const uint2 hi = ODD ? r0 : r1;
     // bits [s4, s4 + 64) of hi:lo
     const bool big = s4 >= 32u;
     const uint32_t x0 = big ? lo.y : lo.x, x1 = big ? hi.x : lo.y, x2 = big ? hi.y : hi.x;
     const uint32_t sh = s4 & 31u;
-    const uint32_t m0 = __builtin_amdgcn_alignbit(x1, x0, sh), m1 = __builtin_amdgcn_alignbit(x2, x1, sh);
-    w0 = fwd ? m0 : nibble_reverse(m1);
-    w1 = fwd ? m1 : nibble_reverse(m0);
+    w0 = __builtin_amdgcn_alignbit(x1, x0, sh);
+    w1 = __builtin_amdgcn_alignbit(x2, x1, sh);
     const uint2 v = load(base, u, umax);
-    u += step;
+    u += 1;
     if (ODD) r1 = v;
     else r0 = v;
   }
@@ -2033,7 +2039,8 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
   const int L = a.lmer;
   const uint2* tn2 = reinterpret_cast<const uint2*>(a.tnib);
   const uint2* qn2 = reinterpret_cast<const uint2*>(a.qnib);
-  const int32_t tumax = a.t_bmax, qumax = a.q_bmax;  // one 16-slot unit per 16-byte block
+  // one 16-slot unit per 16-byte block, the mirror as many again
+  const int32_t tumax = 2 * a.t_bmax + 1, qumax = 2 * a.q_bmax + 1;
   NibWin tw[2], qw[2];
   int rows[2];
   // EDGE: ring position j (target index tb + ts j) is 'X' from j = xk[h] on
