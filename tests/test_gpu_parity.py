@@ -354,11 +354,14 @@ def test_dp_16bit_lanes(gap, monkeypatch):
 
 
 @pytest.mark.parametrize("with_n", [False, True], ids=["no_n", "n_in_two"])
-@pytest.mark.parametrize("gap", [4, 8, 16])
+@pytest.mark.parametrize("gap", [3, 4, 5, 6, 7, 8, 9, 12, 16])
 def test_dp16_query_profiles(gap, with_n, monkeypatch):
     # The 16-bit score pass with 9 or more bands takes its diagonal terms
     # from per-row query profiles when every task of a 64-pair chunk lies on
     # a target sequence without N; other chunks use the class-code xor.
+    # Gaps 4-8 (four rows per loop step) look the profiles up in a lane
+    # table, wider ones build them per row, gap 3 (7 bands) has none; every
+    # gap reads its class streams from the mirrored 4-bit arrays.
     # Every 5th read carries an N (the query's other class, which matches no
     # target letter).  Targets: no N at all (every chunk on profiles), or N
     # in two of six sequences (both kinds of chunk in one launch).  Rows equal
