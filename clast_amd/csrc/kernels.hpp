@@ -1898,14 +1898,9 @@ constexpr int kDriftRows = 256;
 // d, and 0 <= d <= 2 W + 4 (kDriftRows - 1) <= 1086, inside the 16-bit
 // margin above k_dp16's range proof's scores (at most 30001, and the
 // diagonal adds at most 12).
-//
-// MC: match counts too (the second pass), as dp_row16 keeps them: every
-// value of one cell carries the same drift, so its compares (h < v, the gap
-// terms against the diagonal) are k_dp's, and the diagonal term w = 6 + 4 mc
-// gives mc = w >> 3.
-template <int W, int O, bool MC = false>
-__device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
-                                               const uint32_t (&T)[W + 4], uint32_t Qs, s16x2 h1, s16x2 vw) {
+template <int W, int O>
+__device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
+                                               uint32_t Qs, s16x2 h1, s16x2 vw) {
   s16x2 hp = h1;
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
@@ -1913,12 +1908,7 @@ __device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W +
     const s16x2 h = b == 1 ? h1 : __builtin_elementwise_max(Z[b - 1], hp);
     const s16x2 w = pk_val(__builtin_amdgcn_perm(0x06060606u, 0x0606060Au, T[O + b - 1] ^ Qs));
     const s16x2 pt = Z[b] + w;
-    const s16x2 vh = __builtin_elementwise_max(v, h);
-    if constexpr (MC) {
-      const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
-      M[b] = pk_val(pk_pick(pk_less(vh, pt), pk_bits(M[b] + (w >> 3)), ngap));
-    }
-    Z[b] = __builtin_elementwise_max(pt, vh) + pk_splat(-5);
+    Z[b] = __builtin_elementwise_max(pt, __builtin_elementwise_max(v, h)) + pk_splat(-5);
     V[b] = v;
     hp = h;
   }
@@ -1932,11 +1922,9 @@ __device__ __forceinline__ void dp_row16_drift(s16x2 (&Z)[W + 2], s16x2 (&V)[W +
 // byte 0): the task's target sequence holds no class 4.
 // FIRST: the row's vertical terms go first, before anything that reads the
 // profiles (they may still be in flight from ds_bpermute).
-// MC: match counts too, as in dp_row16_drift.
-template <int W, int O, bool FIRST = false, bool MC = false>
-__device__ __forceinline__ void dp_row16_prof(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], s16x2 (&M)[W + 2],
-                                              const uint32_t (&T)[W + 4], uint32_t px, uint32_t py, s16x2 h1,
-                                              s16x2 vw) {
+template <int W, int O, bool FIRST = false>
+__device__ __forceinline__ void dp_row16_prof(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 2], const uint32_t (&T)[W + 4],
+                                              uint32_t px, uint32_t py, s16x2 h1, s16x2 vw) {
   if constexpr (FIRST) {
     // in place, rising b: V[b + 1] and Z[b + 1] are still the last row's
 #pragma unroll
@@ -1949,14 +1937,8 @@ __device__ __forceinline__ void dp_row16_prof(s16x2 (&Z)[W + 2], s16x2 (&V)[W + 
   for (int b = 1; b <= W; ++b) {
     const s16x2 v = FIRST ? V[b] : b == W ? vw : __builtin_elementwise_max(Z[b + 1], V[b + 1]);
     const s16x2 h = b == 1 ? h1 : __builtin_elementwise_max(Z[b - 1], hp);
-    const s16x2 w = pk_val(__builtin_amdgcn_perm(py, px, T[O + b - 1]));
-    const s16x2 pt = Z[b] + w;
-    const s16x2 vh = __builtin_elementwise_max(v, h);
-    if constexpr (MC) {
-      const uint32_t ngap = pk_pick(pk_less(h, v), pk_bits(M[b + 1]), pk_bits(M[b - 1]));
-      M[b] = pk_val(pk_pick(pk_less(vh, pt), pk_bits(M[b] + (w >> 3)), ngap));
-    }
-    Z[b] = __builtin_elementwise_max(pt, vh) + pk_splat(-5);
+    const s16x2 pt = Z[b] + pk_val(__builtin_amdgcn_perm(py, px, T[O + b - 1]));
+    Z[b] = __builtin_elementwise_max(pt, __builtin_elementwise_max(v, h)) + pk_splat(-5);
     V[b] = v;
     hp = h;
   }
@@ -2006,28 +1988,25 @@ __device__ __forceinline__ void dp16_end(const s16x2 (&Z)[2 * G + 3], const s16x
   dres[task] = make_int4(rows + e, rows, mt, sc);
 }
 
-// dp16_end in drifted units (dp_row16_drift): band b's Z is
-// Z~[b] - 2 - 2 b - d4 at the row whose drift is 2 b + d4.  SCORE: the
-// match count is written as 0.
-template <int G, bool SCORE = true>
-__device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], const s16x2 (&M)[2 * G + 3],
-                                               int4* dres, uint32_t task, int half, int rows, int d4) {
+// dp16_end of the score pass in drifted units (dp_row16_drift): band b's Z
+// is Z~[b] - 2 - 2 b - d4 at the row whose drift is 2 b + d4.
+template <int G>
+__device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], int4* dres, uint32_t task, int half,
+                                               int rows, int d4) {
   constexpr int W = 2 * G + 1;
   // compared with the common d4 left in (band 0, never drifted, gets it)
   s16x2 bz = Z[0] + pk_splat(d4);
-  uint32_t bm = pk_bits(M[0]), be = pk_bits(pk_splat(-1));
+  uint32_t be = pk_bits(pk_splat(-1));
 #pragma unroll
   for (int b = 1; b <= W; ++b) {
     const s16x2 z = Z[b] - pk_splat(2 + 2 * b);
     const uint32_t m = pk_less(bz, z);
     bz = __builtin_elementwise_max(bz, z);
-    if constexpr (!SCORE) bm = pk_pick(m, pk_bits(M[b]), bm);
     be = pk_pick(m, pk_bits(pk_splat(b - 1)), be);
   }
   const int sc = (half ? bz.y : bz.x) - d4 - (kGapOpen + kGapExt);
-  const int mt = SCORE ? 0 : half ? pk_val(bm).y : pk_val(bm).x;
   const int e = (half ? pk_val(be).y : pk_val(be).x) - G;
-  dres[task] = make_int4(rows + e, rows, mt, sc);
+  dres[task] = make_int4(rows + e, rows, 0, sc);
 }
 
 // Tasks order[2t] and order[2t + 1] (neighbours in the longest-first order,
@@ -2043,10 +2022,10 @@ __device__ __forceinline__ void dp16_end_drift(const s16x2 (&Z)[2 * G + 3], cons
 template <int G, bool EDGE, bool SCORE, bool PROF>
 __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint32_t n, uint32_t chunk) {
   constexpr int W = 2 * G + 1;
-  // outside EDGE the rows run in drifted units (dp_row16_drift; with PROF,
-  // dp_row16_prof), with match counts unless SCORE
-  constexpr bool DRIFT = !EDGE;
-  static_assert(!PROF || DRIFT, "query profiles in the drifted rows only");
+  // the score pass outside EDGE runs in drifted units (dp_row16_drift; with
+  // PROF, dp_row16_prof)
+  constexpr bool DRIFT = SCORE && !EDGE;
+  static_assert(!PROF || DRIFT, "query profiles in the drifted score pass only");
   // the target streams start kLead positions early, so that both streams
   // enter groups on rows r = 0 (mod 16) with the same group parity
   constexpr int kLead = (32 - (W & 31)) & 31;
@@ -2146,7 +2125,7 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
   // k_dp's first)
   auto end_node = [&](uint32_t task, int h, int rws) {
     if constexpr (DRIFT) {
-      dp16_end_drift<G, SCORE>(Z, M, a.dres, task, h, rws, 4 * ((rws - 1) & (kDriftRows - 1)));
+      dp16_end_drift<G>(Z, a.dres, task, h, rws, 4 * ((rws - 1) & (kDriftRows - 1)));
     } else {
       dp16_end<G, SCORE>(Z, M, a.dres, task, h, rws);
     }
@@ -2175,19 +2154,17 @@ __device__ __forceinline__ void dp16_chunk(const DPArgs& a, uint32_t first, uint
     if constexpr (EDGE) T[W + i] |= (r + W >= xk[0] ? 0x8u : 0u) | (r + W >= xk[1] ? 0x80000u : 0u);
     if constexpr (kTab) {
       const int d4 = 4 * (r & (kDriftRows - 1));
-      dp_row16_prof<W, i, i == 0, !SCORE>(Z, V, M, T, px[i], py[i], pk_splat(kH1 + 2 + d4),
-                                          pk_splat(kVW + 2 * W + d4));
+      dp_row16_prof<W, i, i == 0>(Z, V, T, px[i], py[i], pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
     } else {
       const uint32_t Q = qp.next(j);  // j: the row inside the group (mod 4 is enough)
       if constexpr (PROF) {
         T[W + i] |= 0x0C040C00u;
         const int d4 = 4 * (r & (kDriftRows - 1));
-        dp_row16_prof<W, i, false, !SCORE>(Z, V, M, T, query_profile<0>(Q), query_profile<2>(Q),
-                                           pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
+        dp_row16_prof<W, i>(Z, V, T, query_profile<0>(Q), query_profile<2>(Q), pk_splat(kH1 + 2 + d4),
+                            pk_splat(kVW + 2 * W + d4));
       } else if constexpr (DRIFT) {
         const int d4 = 4 * (r & (kDriftRows - 1));
-        dp_row16_drift<W, i, !SCORE>(Z, V, M, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4),
-                                     pk_splat(kVW + 2 * W + d4));
+        dp_row16_drift<W, i>(Z, V, T, Q | 0x0C000C00u, pk_splat(kH1 + 2 + d4), pk_splat(kVW + 2 * W + d4));
       } else {
         dp_row16<W, i, EDGE, SCORE>(Z, V, M, T, Q);
       }
@@ -2288,7 +2265,7 @@ static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4
   // (dp_row16_prof) when every task of the chunk lies on a target sequence
   // without class 4 (clean flag 3)
   auto run = [&](uint32_t c) {
-    if constexpr (!EDGE && 2 * G + 1 >= 9) {
+    if constexpr (SCORE && !EDGE && 2 * G + 1 >= 9) {
       const uint32_t t = c * 64 + (threadIdx.x & 63u);
       bool prof = true;
       if (2 * t < n) {
