@@ -33,9 +33,9 @@
  * once; searches of one query set that overlap in time must use the same
  * cut_off, total_db_size, K and lambda (a search stores the query's
  * per-strand score thresholds for those, under a lock of that query).
- * Create and destroy a handle on one thread, and not while it is searched.  A thread's runtimes go to a pool when
- * the thread ends and are reused by later threads; the pool lives until the
- * process ends.
+ * Create and destroy a handle on one thread, and not while it is searched.
+ * A thread's runtimes go to a pool when the thread ends and are reused by
+ * later threads; the pool lives until the process ends.
  */
 #ifndef CLAST_HIP_H_
 #define CLAST_HIP_H_
